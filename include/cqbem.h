@@ -1,0 +1,120 @@
+/*
+ * cqbem.h -- C-ABI of the MI355X (gfx950) CQBEM precompute.
+ *
+ * Assembles the convolution-quadrature boundary-element weight matrices G_n, H_n,
+ * B_trans_n and B_angular_n (n = 0..L-1) of a closed triangle surface mesh, in
+ * collocation or Galerkin form, on one GPU.  Plain pointers and sizes only.
+ *
+ * Reference interfaces replaced (rsugimoto/SurfaceOnlyDeformables):
+ *   ElastodynamicsCQBEMMatrixCollection::compute_H_matrices   src/elastodynamics_cqbem_H.cu:18-200
+ *   ElastodynamicsCQBEMMatrixCollection::compute_G_matrices   src/elastodynamics_cqbem_G.cu:18-183
+ *   ElastodynamicsCQBEMMatrixCollection::compute_B_trans_matrices   src/elastodynamics_cqbem_B_trans.cu:14-164
+ *   ElastodynamicsCQBEMMatrixCollection::compute_B_angular_tensors  src/elastodynamics_cqbem_B_angular.cu:14-179
+ *   ElastodynamicsCQBEMMatrixCollection::compute_B_euler_matrices   src/elastodynamics_cqbem_matrix_collection.cpp:280-305
+ *   the kernel-level seam of include/elastodynamics_cqbem_collocation_kernels.cuh:15-77 and
+ *   include/elastodynamics_cqbem_galerkin_kernels.cuh:18-108 (one call per frequency there; one call per
+ *   job here, all frequencies and the inverse transform included).
+ *
+ * Layouts follow include/type_declaration.hpp:5,16-48 of the reference: double, row-major, 64-bit sizes.
+ *   G, H        L x (3 nv) x (3 nv)       H_0 includes the C (jump) matrix (H.cu:184-193); not negated
+ *   B_trans     L x (3 nv) x 3
+ *   B_angular   L x (3 nv) x 3 x 3        includes + B_trans (x) (x_v - cm), cm = 0 (B_angular.cu:167-175)
+ *   B           L x (3 nv) x 6            [B_trans | B_euler]  (matrix_collection.cpp:168-172)
+ * Every function returns CQBEM_OK (0) or a negative status; cqbem_last_error() describes the last failure.
+ * Calls on one context are serialized by the context; distinct contexts may be used from distinct threads.
+ */
+#ifndef CQBEM_ABI_H_
+#define CQBEM_ABI_H_
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CQBEM_OK = 0,
+    CQBEM_ERR_ARG = -1,       /* invalid argument */
+    CQBEM_ERR_HIP = -2,       /* HIP runtime failure (no device, launch failure, ...) */
+    CQBEM_ERR_NOMEM = -3,     /* job does not fit the context's HBM budget or device memory */
+    CQBEM_ERR_UNSUPPORTED = -4,
+    CQBEM_ERR_BUSY = -5,      /* serving: no free slot (retry later) */
+    CQBEM_ERR_NOTFOUND = -6
+};
+
+enum { CQBEM_BDF1 = 1, CQBEM_BDF2 = 2 };
+enum { CQBEM_FP64 = 0, CQBEM_MIXED = 1 };
+enum { CQBEM_G = 1, CQBEM_H = 2, CQBEM_B_TRANS = 4, CQBEM_B_ANGULAR = 8, CQBEM_ALL = 15 };
+
+typedef struct cqbem_context cqbem_context;
+
+/* Surface mesh (reference: V, F after load_mesh and centering, physics_base_matrix_collection.cpp:32-57). */
+typedef struct {
+    const double *V;                  /* nv x 3, centred on the volume centroid */
+    int64_t nv;
+    const int32_t *F;                 /* nf x 3, outward counter-clockwise */
+    int64_t nf;
+    const int32_t *vertex_map_inverse; /* nullable; traction-discontinuity vertex map (G_collocation.cu:81-88) */
+} cqbem_mesh;
+
+/* Material, CQ and quadrature parameters (reference: matrix_collection.cpp:59-69, G.cu:19-24,
+ * elastics_bem_matrix_collection.cpp:19-38).  L and R are explicit: the reference derives
+ * L = max_time_history + 1 and R = 1e-10^(1/(2 max_time_history)). */
+typedef struct {
+    double c1, c2, rho, nu;
+    double dt, R;
+    int32_t L;
+    int32_t bdf;          /* CQBEM_BDF1 | CQBEM_BDF2 */
+    int32_t order;        /* gaussian_quadrature_order, 1..5 */
+    int32_t subdivision;  /* quadrature_subdivision, >= 1 */
+    int32_t galerkin;     /* 0 collocation, 1 Galerkin */
+    int32_t precision;    /* CQBEM_FP64 | CQBEM_MIXED */
+    int32_t which;        /* bitmask of CQBEM_G | CQBEM_H | CQBEM_B_TRANS | CQBEM_B_ANGULAR */
+} cqbem_params;
+
+/* Output buffers; any may be NULL.  on_device != 0: device pointers on the context's GPU;
+ * otherwise host pointers (the library stages through device memory). */
+typedef struct {
+    double *G, *H, *B_trans, *B_angular, *B;
+    int32_t on_device;
+} cqbem_outputs;
+
+/* Creates a context on HIP device `device`; hbm_budget_bytes = 0 means 90% of free device memory. */
+int cqbem_create(cqbem_context **ctx, int device, uint64_t hbm_budget_bytes);
+int cqbem_destroy(cqbem_context *ctx);
+const char *cqbem_last_error(const cqbem_context *ctx);
+
+/* Blocking assembly of one job (replaces compute_H/G/B_trans/B_angular_matrices + compute_B_euler_matrices). */
+int cqbem_assemble(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params *params, cqbem_outputs *out);
+
+/* Device bytes cqbem_assemble needs for a job (outputs on device included when outputs_on_device != 0). */
+int cqbem_job_footprint(const cqbem_mesh *mesh, const cqbem_params *params, int outputs_on_device, uint64_t *bytes);
+
+/* Laplace-domain matrices of one CQ frequency index l (0..L/2), complex interleaved, row-major:
+ * U, P (3nv x 3nv), Bt (3nv x 3), Ba (3nv x 9).  Host pointers; any may be NULL.  Used for parity checks
+ * of the per-frequency assembly (the U_l / P_l matrices of G.cu:150 / H.cu:155). */
+int cqbem_laplace(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params *params, int32_t l, double *U,
+                  double *P, double *Bt, double *Ba);
+
+/* CQ frequencies s_l = gamma(R zeta^l)/dt, l = 0..L/2, complex interleaved (G.cu:19-24,65). Returns L/2+1. */
+int cqbem_frequencies(int32_t L, double R, double dt, int32_t bdf, double *s_out);
+
+/* Timings of the last cqbem_assemble on this context, milliseconds: [0] total, [1] quadrature kernels,
+ * [2] singular kernels, [3] inverse transform, [4] host staging. */
+int cqbem_last_timings(const cqbem_context *ctx, double *ms5);
+
+/* ---- serving: continuous batching of assembly jobs on one context ----
+ * cqbem_submit admits a job when its device footprint fits the budget next to the jobs in flight,
+ * copies the mesh, and enqueues it on a free stream slot; results land in `out` (device pointers
+ * required: matrices stay resident).  The caller keeps `out` buffers alive until the job is done. */
+int cqbem_submit(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params *params, const cqbem_outputs *out,
+                 uint64_t *job_id);
+/* *done = 1 once the job finished (its scratch is then released); status of the job in the return value. */
+int cqbem_poll(cqbem_context *ctx, uint64_t job_id, int32_t *done);
+int cqbem_wait(cqbem_context *ctx, uint64_t job_id);
+/* Bytes currently reserved by in-flight jobs. */
+int cqbem_reserved_bytes(const cqbem_context *ctx, uint64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,208 @@
+"""ctypes binding of include/cqbem.h (libcqbem.so, built in-tree for gfx950).
+
+The product path: every call goes to the HIP library.  If the library is not
+built, loading fails with an error instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libcqbem.so")
+
+CQBEM_OK = 0
+ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_BUSY, ERR_NOTFOUND = -1, -2, -3, -4, -5, -6
+BDF1, BDF2 = 1, 2
+FP64, MIXED = 0, 1
+G, H, B_TRANS, B_ANGULAR, ALL = 1, 2, 4, 8, 15
+
+EXPORTED = [
+    "cqbem_create", "cqbem_destroy", "cqbem_last_error", "cqbem_assemble", "cqbem_job_footprint",
+    "cqbem_laplace", "cqbem_frequencies", "cqbem_last_timings", "cqbem_submit", "cqbem_poll",
+    "cqbem_wait", "cqbem_reserved_bytes",
+]
+
+
+class cqbem_mesh(C.Structure):
+    _fields_ = [("V", C.POINTER(C.c_double)), ("nv", C.c_int64), ("F", C.POINTER(C.c_int32)), ("nf", C.c_int64),
+                ("vertex_map_inverse", C.POINTER(C.c_int32))]
+
+
+class cqbem_params(C.Structure):
+    _fields_ = [("c1", C.c_double), ("c2", C.c_double), ("rho", C.c_double), ("nu", C.c_double),
+                ("dt", C.c_double), ("R", C.c_double), ("L", C.c_int32), ("bdf", C.c_int32),
+                ("order", C.c_int32), ("subdivision", C.c_int32), ("galerkin", C.c_int32),
+                ("precision", C.c_int32), ("which", C.c_int32)]
+
+
+class cqbem_outputs(C.Structure):
+    _fields_ = [("G", C.c_void_p), ("H", C.c_void_p), ("B_trans", C.c_void_p), ("B_angular", C.c_void_p),
+                ("B", C.c_void_p), ("on_device", C.c_int32)]
+
+
+_lib = None
+
+
+class CQBEMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"cqbem error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m surfaceonlydeformables_amd.build` "
+                               "(or __graft_entry__.build()) first")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        L.cqbem_create.argtypes = [P(C.c_void_p), C.c_int, C.c_uint64]
+        L.cqbem_destroy.argtypes = [C.c_void_p]
+        L.cqbem_last_error.argtypes = [C.c_void_p]
+        L.cqbem_last_error.restype = C.c_char_p
+        L.cqbem_assemble.argtypes = [C.c_void_p, P(cqbem_mesh), P(cqbem_params), P(cqbem_outputs)]
+        L.cqbem_job_footprint.argtypes = [P(cqbem_mesh), P(cqbem_params), C.c_int, P(C.c_uint64)]
+        L.cqbem_laplace.argtypes = [C.c_void_p, P(cqbem_mesh), P(cqbem_params), C.c_int32, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p]
+        L.cqbem_frequencies.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_int32, P(C.c_double)]
+        L.cqbem_last_timings.argtypes = [C.c_void_p, P(C.c_double)]
+        L.cqbem_submit.argtypes = [C.c_void_p, P(cqbem_mesh), P(cqbem_params), P(cqbem_outputs), P(C.c_uint64)]
+        L.cqbem_poll.argtypes = [C.c_void_p, C.c_uint64, P(C.c_int32)]
+        L.cqbem_wait.argtypes = [C.c_void_p, C.c_uint64]
+        L.cqbem_reserved_bytes.argtypes = [C.c_void_p, P(C.c_uint64)]
+        _lib = L
+    return _lib
+
+
+def make_mesh(V, F, vmap=None):
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    F = np.ascontiguousarray(F, dtype=np.int32)
+    m = cqbem_mesh(V.ctypes.data_as(C.POINTER(C.c_double)), len(V), F.ctypes.data_as(C.POINTER(C.c_int32)), len(F),
+                   None)
+    keep = [V, F]
+    if vmap is not None:
+        vm = np.ascontiguousarray(vmap, dtype=np.int32)
+        m.vertex_map_inverse = vm.ctypes.data_as(C.POINTER(C.c_int32))
+        keep.append(vm)
+    m._keep = keep
+    return m
+
+
+def make_params(c1, c2, rho, nu, dt, R, L, bdf=BDF2, order=5, subdivision=2, galerkin=False, precision=FP64,
+                which=ALL):
+    return cqbem_params(c1, c2, rho, nu, dt, R, L, bdf, order, subdivision, int(galerkin), precision, which)
+
+
+def frequencies(L, R, dt, bdf):
+    out = np.zeros(2 * (L // 2 + 1))
+    n = lib().cqbem_frequencies(L, R, dt, bdf, out.ctypes.data_as(C.POINTER(C.c_double)))
+    if n < 0:
+        raise CQBEMError(n, "invalid frequency parameters")
+    return out[0::2] + 1j * out[1::2]
+
+
+class Context:
+    """One GPU context (device, stream, HBM budget)."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        self._h = C.c_void_p()
+        rc = lib().cqbem_create(C.byref(self._h), device, hbm_budget_bytes)
+        if rc != CQBEM_OK:
+            raise CQBEMError(rc, "cqbem_create failed (no HIP device?)")
+
+    def close(self):
+        if self._h:
+            lib().cqbem_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != CQBEM_OK:
+            raise CQBEMError(rc, lib().cqbem_last_error(self._h).decode())
+
+    @property
+    def handle(self):
+        return self._h
+
+    def last_timings(self):
+        t = np.zeros(5)
+        self._check(lib().cqbem_last_timings(self._h, t.ctypes.data_as(C.POINTER(C.c_double))))
+        return dict(zip(["total", "quadrature", "singular", "transform", "host"], t.tolist()))
+
+    def assemble(self, mesh: cqbem_mesh, params: cqbem_params, want=("G", "H", "B_trans", "B_angular", "B")):
+        """Host-output assembly: returns dict of numpy arrays (L, 3nv, 3nv) / (L, 3nv, 3) / (L, 3nv, 3, 3) / (L, 3nv, 6)."""
+        nv, L = mesh.nv, params.L
+        n3 = 3 * nv
+        shapes = {"G": (L, n3, n3), "H": (L, n3, n3), "B_trans": (L, n3, 3), "B_angular": (L, n3, 3, 3),
+                  "B": (L, n3, 6)}
+        bits = {"G": G, "H": H, "B_trans": B_TRANS, "B_angular": B_ANGULAR, "B": B_TRANS | B_ANGULAR}
+        res = {}
+        out = cqbem_outputs()
+        for k in want:
+            if params.which & bits[k] == bits[k]:
+                res[k] = np.zeros(shapes[k])
+                setattr(out, k, res[k].ctypes.data)
+        out.on_device = 0
+        self._check(lib().cqbem_assemble(self._h, C.byref(mesh), C.byref(params), C.byref(out)))
+        return res
+
+    def assemble_device(self, mesh: cqbem_mesh, params: cqbem_params, ptrs: dict):
+        """Device-output assembly into caller-owned device buffers {name: device pointer int}."""
+        out = cqbem_outputs()
+        for k, p in ptrs.items():
+            setattr(out, k, p)
+        out.on_device = 1
+        self._check(lib().cqbem_assemble(self._h, C.byref(mesh), C.byref(params), C.byref(out)))
+
+    def laplace(self, mesh: cqbem_mesh, params: cqbem_params, l: int):
+        n3 = 3 * mesh.nv
+        U = np.zeros((n3, n3), np.complex128) if params.which & G else None
+        P = np.zeros((n3, n3), np.complex128) if params.which & H else None
+        Bt = np.zeros((n3, 3), np.complex128) if params.which & (B_TRANS | B_ANGULAR) else None
+        Ba = np.zeros((n3, 3, 3), np.complex128) if params.which & (B_TRANS | B_ANGULAR) else None
+        ptr = lambda a: None if a is None else a.ctypes.data
+        self._check(lib().cqbem_laplace(self._h, C.byref(mesh), C.byref(params), l, ptr(U), ptr(P), ptr(Bt), ptr(Ba)))
+        return {"U": U, "P": P, "Bt": Bt, "Ba": Ba}
+
+    # serving
+    def submit(self, mesh, params, ptrs: dict) -> int:
+        out = cqbem_outputs()
+        for k, p in ptrs.items():
+            setattr(out, k, p)
+        out.on_device = 1
+        jid = C.c_uint64()
+        rc = lib().cqbem_submit(self._h, C.byref(mesh), C.byref(params), C.byref(out), C.byref(jid))
+        if rc == ERR_BUSY:
+            return -1
+        self._check(rc)
+        return jid.value
+
+    def poll(self, job_id: int) -> bool:
+        done = C.c_int32()
+        self._check(lib().cqbem_poll(self._h, job_id, C.byref(done)))
+        return bool(done.value)
+
+    def wait(self, job_id: int):
+        self._check(lib().cqbem_wait(self._h, job_id))
+
+    def reserved_bytes(self) -> int:
+        b = C.c_uint64()
+        self._check(lib().cqbem_reserved_bytes(self._h, C.byref(b)))
+        return b.value
+
+
+def job_footprint(mesh, params, outputs_on_device=True) -> int:
+    b = C.c_uint64()
+    rc = lib().cqbem_job_footprint(C.byref(mesh), C.byref(params), int(outputs_on_device), C.byref(b))
+    if rc != CQBEM_OK:
+        raise CQBEMError(rc, "invalid job")
+    return b.value

@@ -1,0 +1,29 @@
+// Galerkin assembly (non-singular and Sauter-Schwab singular face pairs).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "cqbem_kernels.hpp"
+
+namespace cqbem {
+
+struct GalerkinJob {
+    Geom g;
+    const double4 *quad;  // composite triangle rule (barycentric b0,b1,b2, weight summing to 1)
+    int npts;
+    const double2 *line;  // composite Gauss rule on [0,1]
+    int nline;
+    const Freq *freqs;    // first frequency of the job's range
+    int nl;               // number of frequencies
+    Mat mat;
+    int num_cus;
+};
+
+// Accumulates the Laplace-domain spectra of all frequencies of `job` into
+// U, P (nl x 3nv x 3nv), Bt (nl x 3nv x 3), Ba (nl x 3nv x 9); any may be null.
+// The buffers are zero-filled first.  tm[1] += non-singular ms, tm[2] += singular ms.
+void run_galerkin(const GalerkinJob &job, const std::vector<int> &vs_ptr, const std::vector<int> &vs_face, const int *F,
+                  cd *U, cd *P, cd *Bt, cd *Ba, double *tm, hipStream_t st);
+
+}  // namespace cqbem

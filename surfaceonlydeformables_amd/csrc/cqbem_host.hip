@@ -1,0 +1,684 @@
+// Host driver behind include/cqbem.h: job preparation (normals, vertex stars,
+// quadrature and transform tables), device memory under an HBM budget,
+// row-slab scheduling of the quadrature kernels, the inverse CQ transform and
+// the C / B post-processing.  Replaces compute_{H,G,B_trans,B_angular}_matrices
+// and compute_B_euler_matrices of the reference's
+// ElastodynamicsCQBEMMatrixCollection (src/elastodynamics_cqbem_*.cu,
+// src/elastodynamics_cqbem_matrix_collection.cpp:280-305).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/cqbem.h"
+#include "cqbem_galerkin.hpp"
+#include "cqbem_kernels.hpp"
+#include "cqbem_rules.hpp"
+
+using namespace cqbem;
+
+namespace {
+
+struct Err {
+    int code;
+    std::string msg;
+};
+
+#define HIPCHK(expr)                                                                                     \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            throw Err{e_ == hipErrorOutOfMemory ? CQBEM_ERR_NOMEM : CQBEM_ERR_HIP,                       \
+                      std::string(#expr) + ": " + hipGetErrorString(e_)};                               \
+    } while (0)
+
+// Device allocation owned by a job; released on scope exit.
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t n) {
+        if (p) (void)hipFree(p), p = nullptr;
+        bytes = n;
+        if (n) HIPCHK(hipMalloc(&p, n));
+    }
+    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+template <class T> void upload(DBuf &b, const std::vector<T> &v, hipStream_t st) {
+    b.alloc(v.size() * sizeof(T));
+    if (!v.empty()) HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+void check_inputs(const cqbem_mesh *mesh, const cqbem_params *p) {
+    if (!mesh || !p) throw Err{CQBEM_ERR_ARG, "null mesh or params"};
+    if (!mesh->V || !mesh->F || mesh->nv < 3 || mesh->nf < 1) throw Err{CQBEM_ERR_ARG, "empty mesh"};
+    if (mesh->nv > (1 << 24) || mesh->nf > (1 << 26)) throw Err{CQBEM_ERR_ARG, "mesh too large"};
+    for (int64_t k = 0; k < 3 * mesh->nf; k++)
+        if (mesh->F[k] < 0 || mesh->F[k] >= mesh->nv) throw Err{CQBEM_ERR_ARG, "face index out of range"};
+    if (mesh->vertex_map_inverse)
+        for (int64_t k = 0; k < mesh->nv; k++)
+            if (mesh->vertex_map_inverse[k] < 0) throw Err{CQBEM_ERR_ARG, "negative vertex_map_inverse entry"};
+    if (!(p->c1 > 0) || !(p->c2 > 0) || !(p->rho > 0) || !(p->dt > 0) || !(p->R > 0))
+        throw Err{CQBEM_ERR_ARG, "c1, c2, rho, dt and R must be positive"};
+    if (p->L < 2 || p->L > 4096) throw Err{CQBEM_ERR_ARG, "L must be in [2, 4096]"};
+    if (p->bdf != CQBEM_BDF1 && p->bdf != CQBEM_BDF2) throw Err{CQBEM_ERR_ARG, "bdf must be 1 or 2"};
+    if (p->order < 1 || p->order > 5) throw Err{CQBEM_ERR_ARG, "order must be in [1, 5]"};
+    if (p->subdivision < 1 || p->subdivision > 16) throw Err{CQBEM_ERR_ARG, "subdivision must be in [1, 16]"};
+    if (p->precision != CQBEM_FP64 && p->precision != CQBEM_MIXED) throw Err{CQBEM_ERR_ARG, "unknown precision"};
+    if ((p->which & ~CQBEM_ALL) != 0) throw Err{CQBEM_ERR_ARG, "unknown matrix bits in `which`"};
+    if (!p->galerkin && (p->which & CQBEM_H) && mesh->vertex_map_inverse)
+        throw Err{CQBEM_ERR_UNSUPPORTED, "collocation C matrix of a vertex-split mesh needs the original mesh"};
+}
+
+}  // namespace
+
+// --------------------------------------------------------------- context --
+// A serving job: owns copies of its mesh and runs on one stream slot in a worker thread.
+struct JobRecord {
+    uint64_t id = 0;
+    std::vector<double> V;
+    std::vector<int32_t> F, vmap;
+    cqbem_mesh mesh{};
+    cqbem_params params{};
+    cqbem_outputs out{};
+    uint64_t bytes = 0;
+    int slot = -1;
+    std::atomic<int> done{0};
+    int status = 0;
+    std::string error;
+    std::thread worker;
+};
+
+struct cqbem_context {
+    int device = 0;
+    uint64_t budget = 0;
+    int num_cus = 256;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    double timings[5] = {0, 0, 0, 0, 0};
+    std::mutex mu;
+    // serving
+    std::atomic<uint64_t> next_job{1};
+    uint64_t reserved = 0;
+    std::unordered_map<uint64_t, std::shared_ptr<JobRecord>> jobs;
+    std::vector<hipStream_t> slots;
+    std::vector<uint64_t> slot_job;
+};
+
+namespace {
+
+struct Prepared {
+    int nv, nf, L, Lf;
+    std::vector<double> N, area;
+    std::vector<int> vs_ptr, vs_face, sg_ptr, sg_face;
+    std::vector<double4> quad;
+    std::vector<double2> line;
+    std::vector<Freq> freqs;
+    std::vector<double2> tab;
+    Mat mat;
+};
+
+Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
+    Prepared P;
+    P.nv = (int)mesh->nv, P.nf = (int)mesh->nf, P.L = p->L, P.Lf = p->L / 2 + 1;
+    const double *V = mesh->V;
+    const int *F = mesh->F;
+    const int *vm = mesh->vertex_map_inverse;
+    P.N.resize(3 * (size_t)P.nf), P.area.resize(P.nf);
+    for (int f = 0; f < P.nf; f++) {
+        const V3 c = crs(sub(vtx(V, F[3 * f + 1]), vtx(V, F[3 * f])), sub(vtx(V, F[3 * f + 2]), vtx(V, F[3 * f])));
+        const double n = nrm(c);
+        if (!(n > 0)) throw Err{CQBEM_ERR_ARG, "degenerate (zero-area) face " + std::to_string(f)};
+        P.N[3 * f] = c.x / n, P.N[3 * f + 1] = c.y / n, P.N[3 * f + 2] = c.z / n, P.area[f] = 0.5 * n;
+    }
+    // vertex stars
+    P.vs_ptr.assign(P.nv + 1, 0);
+    for (int k = 0; k < 3 * P.nf; k++) P.vs_ptr[F[k] + 1]++;
+    for (int v = 0; v < P.nv; v++) P.vs_ptr[v + 1] += P.vs_ptr[v];
+    P.vs_face.resize(3 * (size_t)P.nf);
+    {
+        std::vector<int> pos(P.vs_ptr.begin(), P.vs_ptr.end() - 1);
+        for (int f = 0; f < P.nf; f++)
+            for (int c = 0; c < 3; c++) P.vs_face[pos[F[3 * f + c]]++] = (f << 2) | c;
+    }
+    // singular faces per collocation vertex: faces with a corner equal to (or mapped like) the vertex;
+    // the first such corner in face order is the polar/Guiggiani centre (G_collocation.cu:78-88)
+    P.sg_ptr.assign(P.nv + 1, 0);
+    if (!vm) {
+        P.sg_ptr = P.vs_ptr;
+        P.sg_face = P.vs_face;
+    } else {
+        std::unordered_map<int, std::vector<int>> groups;
+        for (int v = 0; v < P.nv; v++) groups[vm[v]].push_back(v);
+        std::vector<std::vector<int>> lists(P.nv);
+        for (int i = 0; i < P.nv; i++) {
+            std::vector<int> faces;
+            for (int v : groups[vm[i]])
+                for (int t = P.vs_ptr[v]; t < P.vs_ptr[v + 1]; t++) faces.push_back(P.vs_face[t] >> 2);
+            std::sort(faces.begin(), faces.end());
+            faces.erase(std::unique(faces.begin(), faces.end()), faces.end());
+            for (int f : faces) {
+                int c = 0;
+                while (c < 3 && !(F[3 * f + c] == i || vm[F[3 * f + c]] == vm[i])) c++;
+                lists[i].push_back((f << 2) | c);
+            }
+        }
+        for (int i = 0; i < P.nv; i++) P.sg_ptr[i + 1] = P.sg_ptr[i] + (int)lists[i].size();
+        for (auto &l : lists) P.sg_face.insert(P.sg_face.end(), l.begin(), l.end());
+    }
+    for (const TriPoint &t : composite_tri_rule(p->order, p->subdivision)) P.quad.push_back(make_double4(t.b[0], t.b[1], t.b[2], t.w));
+    for (auto &xw : composite_line01(p->order, p->subdivision)) P.line.push_back(make_double2(xw.first, xw.second));
+    const auto s = cq_frequencies(p->L, p->R, p->dt, p->bdf);
+    for (auto sl : s) {
+        Freq f;
+        f.s = mkc(sl.real(), sl.imag());
+        const std::complex<double> a = sl / p->c1, b = sl / p->c2, is = 1.0 / sl, is2 = is * is;
+        f.sc1 = mkc(a.real(), a.imag()), f.sc2 = mkc(b.real(), b.imag());
+        f.is = mkc(is.real(), is.imag()), f.is2 = mkc(is2.real(), is2.imag());
+        P.freqs.push_back(f);
+    }
+    for (auto t : transform_table(p->L, p->R)) P.tab.push_back(make_double2(t.real(), t.imag()));
+    Mat &m = P.mat;
+    m.c1 = p->c1, m.c2 = p->c2, m.rho = p->rho;
+    m.ic1 = 1.0 / p->c1, m.ic2 = 1.0 / p->c2, m.ic1s = m.ic1 * m.ic1, m.ic2s = m.ic2 * m.ic2;
+    m.kap = p->c2 * p->c2 / (p->c1 * p->c1), m.six_c2s = 6.0 * p->c2 * p->c2;
+    m.g_scale = 1.0 / (4.0 * kPi * p->rho);
+    return P;
+}
+
+struct DevMesh {
+    DBuf V, F, N, area, vs_ptr, vs_face, sg_ptr, sg_face, vmap, quad, line, freqs, tab;
+    Geom g;
+};
+
+void upload_mesh(DevMesh &d, const cqbem_mesh *mesh, const Prepared &P, hipStream_t st) {
+    d.V.alloc(sizeof(double) * 3 * P.nv);
+    HIPCHK(hipMemcpyAsync(d.V.p, mesh->V, sizeof(double) * 3 * P.nv, hipMemcpyHostToDevice, st));
+    d.F.alloc(sizeof(int) * 3 * P.nf);
+    HIPCHK(hipMemcpyAsync(d.F.p, mesh->F, sizeof(int) * 3 * P.nf, hipMemcpyHostToDevice, st));
+    if (mesh->vertex_map_inverse) {
+        d.vmap.alloc(sizeof(int) * P.nv);
+        HIPCHK(hipMemcpyAsync(d.vmap.p, mesh->vertex_map_inverse, sizeof(int) * P.nv, hipMemcpyHostToDevice, st));
+    }
+    upload(d.N, P.N, st), upload(d.area, P.area, st), upload(d.vs_ptr, P.vs_ptr, st), upload(d.vs_face, P.vs_face, st);
+    upload(d.sg_ptr, P.sg_ptr, st), upload(d.sg_face, P.sg_face, st), upload(d.quad, P.quad, st), upload(d.line, P.line, st);
+    upload(d.freqs, P.freqs, st), upload(d.tab, P.tab, st);
+    Geom &g = d.g;
+    g.V = d.V.as<double>(), g.F = d.F.as<int>(), g.N = d.N.as<double>(), g.area = d.area.as<double>();
+    g.vs_ptr = d.vs_ptr.as<int>(), g.vs_face = d.vs_face.as<int>(), g.sg_ptr = d.sg_ptr.as<int>(), g.sg_face = d.sg_face.as<int>();
+    g.vmap = mesh->vertex_map_inverse ? d.vmap.as<int>() : nullptr;
+    g.nv = P.nv, g.nf = P.nf;
+}
+
+size_t matrix_bytes(int nv, int L) { return (size_t)L * 9 * (size_t)nv * nv * sizeof(double); }
+
+// Rows of collocation vertices per slab so that the G/H spectra of one slab fit `bytes`.
+int slab_rows(int nv, int Lf, int nmat, size_t bytes) {
+    const size_t per_row = (size_t)nmat * Lf * 3 * 3 * (size_t)nv * sizeof(cd);
+    size_t rows = per_row ? bytes / per_row : (size_t)nv;
+    rows = std::min<size_t>(std::max<size_t>(rows, 1), (size_t)nv);
+    return (int)std::min<size_t>(rows, 65535);
+}
+
+struct Timer {
+    hipEvent_t a, b;
+    hipStream_t st;
+    double *acc;
+    Timer(hipStream_t s, double *dst) : st(s), acc(dst) {
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        HIPCHK(hipEventRecord(a, st));
+    }
+    ~Timer() {
+        if (hipEventRecord(b, st) == hipSuccess && hipEventSynchronize(b) == hipSuccess) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, a, b) == hipSuccess) *acc += ms;
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+};
+
+// Device footprint of a job: scratch spectra (one slab), B spectra/outputs and, when
+// outputs are not caller-provided device buffers, the staged outputs.
+uint64_t footprint(const cqbem_mesh *mesh, const cqbem_params *p, int outputs_on_device, size_t slab_budget) {
+    const int nv = (int)mesh->nv, L = p->L, Lf = L / 2 + 1;
+    const int nmat = ((p->which & CQBEM_G) ? 1 : 0) + ((p->which & CQBEM_H) ? 1 : 0);
+    uint64_t bytes = 0;
+    if (nmat) {
+        const int rows = slab_rows(nv, Lf, nmat, slab_budget);
+        bytes += (uint64_t)nmat * Lf * 3 * (uint64_t)rows * 3 * nv * sizeof(cd);
+        if (!outputs_on_device) bytes += (uint64_t)nmat * matrix_bytes(nv, L);
+    }
+    bytes += (uint64_t)Lf * 3 * nv * 12 * sizeof(cd) + (uint64_t)L * 3 * nv * 18 * sizeof(double);
+    bytes += (uint64_t)mesh->nf * 64 + (uint64_t)nv * 64 + (1 << 20);
+    return bytes;
+}
+
+// Runs one job on stream `st`.  `slab_budget` bounds the Laplace-domain scratch.
+void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const cqbem_params *p, const cqbem_outputs *out,
+             size_t slab_budget, double *tm, bool sync_outputs) {
+    const auto t_host0 = std::chrono::steady_clock::now();
+    Prepared P = prepare(mesh, p);
+    const int nv = P.nv, L = P.L, Lf = P.Lf;
+    const long long n3 = 3LL * nv;
+    DevMesh d;
+    upload_mesh(d, mesh, P, st);
+    std::vector<double> Chost;
+    DBuf dC, dCidx, dCval;
+    const bool wantG = (p->which & CQBEM_G) && out->G, wantH = (p->which & CQBEM_H) && out->H;
+    const bool wantBt = (p->which & CQBEM_B_TRANS) && out->B_trans;
+    const bool wantBa = (p->which & CQBEM_B_ANGULAR) && (out->B_angular || out->B);
+    const bool wantB = wantBt || wantBa || out->B;
+    tm[4] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
+
+    // outputs: caller device buffers, or staged device buffers copied back at the end
+    DBuf sG, sH, sBt, sBa, sB6;
+    double *G = nullptr, *H = nullptr;
+    if (wantG) {
+        if (out->on_device) G = out->G;
+        else sG.alloc(matrix_bytes(nv, L)), G = sG.as<double>();
+    }
+    if (wantH) {
+        if (out->on_device) H = out->H;
+        else sH.alloc(matrix_bytes(nv, L)), H = sH.as<double>();
+    }
+    DBuf specB, specBa;
+    double *Wbt = nullptr, *Wba = nullptr, *W6 = nullptr;
+    if (wantB) {
+        specB.alloc((size_t)Lf * n3 * 3 * sizeof(cd));
+        specBa.alloc((size_t)Lf * n3 * 9 * sizeof(cd));
+        if (out->on_device && out->B_trans) Wbt = out->B_trans;
+        else sBt.alloc((size_t)L * n3 * 3 * sizeof(double)), Wbt = sBt.as<double>();
+        if (out->on_device && out->B_angular) Wba = out->B_angular;
+        else sBa.alloc((size_t)L * n3 * 9 * sizeof(double)), Wba = sBa.as<double>();
+        if (out->B) {
+            if (out->on_device) W6 = out->B;
+            else sB6.alloc((size_t)L * n3 * 6 * sizeof(double)), W6 = sB6.as<double>();
+        }
+    }
+    const Geom &g = d.g;
+    const double4 *quad = d.quad.as<double4>();
+    const double2 *line = d.line.as<double2>();
+    const Freq *freqs = d.freqs.as<Freq>();
+    const double2 *tab = d.tab.as<double2>();
+    const int npts = (int)P.quad.size(), nline = (int)P.line.size();
+
+    if (!p->galerkin) {
+        const int nmat = (wantG ? 1 : 0) + (wantH ? 1 : 0);
+        if (nmat) {
+            const int rows = slab_rows(nv, Lf, nmat, slab_budget);
+            const long long lstride = 3LL * rows * n3;  // one frequency of one slab
+            DBuf specG, specH;
+            if (wantG) specG.alloc((size_t)Lf * lstride * sizeof(cd));
+            if (wantH) specH.alloc((size_t)Lf * lstride * sizeof(cd));
+            for (int r0 = 0; r0 < nv; r0 += rows) {
+                const int nr = std::min(rows, nv - r0);
+                {
+                    Timer t(st, &tm[1]);
+                    HIPCHK(launch_colloc_gh(g, quad, npts, freqs, 0, Lf, P.mat, r0, nr, specG.as<cd>(), specH.as<cd>(), n3,
+                                            lstride, st));
+                }
+                {
+                    Timer t(st, &tm[2]);
+                    if (wantG)
+                        HIPCHK(launch_colloc_singular(0, g, line, nline, freqs, 0, Lf, P.mat, r0, nr, specG.as<cd>(), n3,
+                                                      lstride, nullptr, nullptr, st));
+                    if (wantH)
+                        HIPCHK(launch_colloc_singular(2, g, line, nline, freqs, 0, Lf, P.mat, r0, nr, specH.as<cd>(), n3,
+                                                      lstride, nullptr, nullptr, st));
+                }
+                {
+                    Timer t(st, &tm[3]);
+                    const long long E = 3LL * nr * n3;
+                    if (wantG) HIPCHK(launch_transform(specG.as<cd>(), E, Lf, tab, L, G + 3LL * r0 * n3, n3 * n3, ctx->num_cus, st));
+                    if (wantH) HIPCHK(launch_transform(specH.as<cd>(), E, Lf, tab, L, H + 3LL * r0 * n3, n3 * n3, ctx->num_cus, st));
+                }
+            }
+        }
+        if (wantB) {
+            {
+                Timer t(st, &tm[1]);
+                HIPCHK(launch_colloc_b(g, quad, npts, freqs, 0, Lf, P.mat, 0, nv, specB.as<cd>(), specBa.as<cd>(), st));
+            }
+            {
+                Timer t(st, &tm[2]);
+                HIPCHK(launch_colloc_singular(1, g, line, nline, freqs, 0, Lf, P.mat, 0, nv, nullptr, n3, 0, specB.as<cd>(),
+                                              specBa.as<cd>(), st));
+            }
+        }
+        if (wantH) {
+            Chost = c_matrix_collocation(mesh->V, nv, mesh->F, P.nf, P.N.data(), p->nu);
+            upload(dC, Chost, st);
+            HIPCHK(launch_add_c_colloc(H, dC.as<double>(), nv, st));
+        }
+    } else {
+        GalerkinJob gj{g, quad, npts, line, nline, freqs, Lf, P.mat, ctx->num_cus};
+        const int nmat = (wantG ? 1 : 0) + (wantH ? 1 : 0);
+        if (nmat || wantB) {
+            // Galerkin spectra are assembled for the full matrix (face pairs scatter to any row)
+            DBuf specG, specH;
+            const long long lstride = n3 * n3;
+            if (wantG) specG.alloc((size_t)Lf * lstride * sizeof(cd));
+            if (wantH) specH.alloc((size_t)Lf * lstride * sizeof(cd));
+            run_galerkin(gj, P.vs_ptr, P.vs_face, mesh->F, specG.as<cd>(), specH.as<cd>(), wantB ? specB.as<cd>() : nullptr,
+                         wantB ? specBa.as<cd>() : nullptr, tm, st);
+            Timer t(st, &tm[3]);
+            if (wantG) HIPCHK(launch_transform(specG.as<cd>(), n3 * n3, Lf, tab, L, G, n3 * n3, ctx->num_cus, st));
+            if (wantH) HIPCHK(launch_transform(specH.as<cd>(), n3 * n3, Lf, tab, L, H, n3 * n3, ctx->num_cus, st));
+        }
+        if (wantH) {
+            // 1/2 P1 mass matrix per component (elastics_bem_matrix_collection.cpp:447-467), summed per entry on the host
+            std::unordered_map<long long, double> acc;
+            for (int f = 0; f < P.nf; f++) {
+                const double A2 = 2.0 * P.area[f];
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++)
+                        for (int k = 0; k < 3; k++)
+                            acc[(3LL * mesh->F[3 * f + a] + k) * n3 + 3LL * mesh->F[3 * f + b] + k] += (a == b) ? A2 / 24.0 : A2 / 48.0;
+            }
+            std::vector<long long> idx;
+            std::vector<double> val;
+            for (auto &kv : acc) idx.push_back(kv.first), val.push_back(kv.second);
+            upload(dCidx, idx, st), upload(dCval, val, st);
+            HIPCHK(launch_add_triplets(H, dCidx.as<long long>(), dCval.as<double>(), (long long)idx.size(), st));
+        }
+    }
+    if (wantB) {
+        Timer t(st, &tm[3]);
+        HIPCHK(launch_transform(specB.as<cd>(), n3 * 3, Lf, tab, L, Wbt, n3 * 3, ctx->num_cus, st));
+        HIPCHK(launch_transform(specBa.as<cd>(), n3 * 9, Lf, tab, L, Wba, n3 * 9, ctx->num_cus, st));
+        HIPCHK(launch_b_post(Wbt, Wba, W6, g.V, nv, L, st));
+    }
+    if (!out->on_device && sync_outputs) {
+        const auto t0 = std::chrono::steady_clock::now();
+        if (wantG) HIPCHK(hipMemcpyAsync(out->G, G, matrix_bytes(nv, L), hipMemcpyDeviceToHost, st));
+        if (wantH) HIPCHK(hipMemcpyAsync(out->H, H, matrix_bytes(nv, L), hipMemcpyDeviceToHost, st));
+        if (wantB && out->B_trans) HIPCHK(hipMemcpyAsync(out->B_trans, Wbt, (size_t)L * n3 * 3 * 8, hipMemcpyDeviceToHost, st));
+        if (wantB && out->B_angular) HIPCHK(hipMemcpyAsync(out->B_angular, Wba, (size_t)L * n3 * 9 * 8, hipMemcpyDeviceToHost, st));
+        if (W6) HIPCHK(hipMemcpyAsync(out->B, W6, (size_t)L * n3 * 6 * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        tm[4] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    HIPCHK(hipStreamSynchronize(st));
+}
+
+int fail(cqbem_context *ctx, const Err &e) {
+    if (ctx) ctx->last_error = e.msg;
+    return e.code;
+}
+
+}  // namespace
+
+// ================================================================= C-ABI ===
+extern "C" {
+
+int cqbem_create(cqbem_context **out, int device, uint64_t hbm_budget_bytes) {
+    if (!out) return CQBEM_ERR_ARG;
+    *out = nullptr;
+    auto ctx = std::make_unique<cqbem_context>();
+    try {
+        int n = 0;
+        HIPCHK(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) throw Err{CQBEM_ERR_ARG, "no such HIP device"};
+        HIPCHK(hipSetDevice(device));
+        ctx->device = device;
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, device));
+        ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        ctx->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(0.9 * (double)fr);
+        HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    } catch (const Err &e) {
+        return e.code;
+    }
+    *out = ctx.release();
+    return CQBEM_OK;
+}
+
+int cqbem_destroy(cqbem_context *ctx) {
+    if (!ctx) return CQBEM_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    for (auto &kv : ctx->jobs)
+        if (kv.second->worker.joinable()) kv.second->worker.join();
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream), (void)hipStreamDestroy(ctx->stream);
+    for (hipStream_t s : ctx->slots) (void)hipStreamSynchronize(s), (void)hipStreamDestroy(s);
+    delete ctx;
+    return CQBEM_OK;
+}
+
+const char *cqbem_last_error(const cqbem_context *ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+int cqbem_frequencies(int32_t L, double R, double dt, int32_t bdf, double *s_out) {
+    if (L < 2 || !(R > 0) || !(dt > 0) || (bdf != 1 && bdf != 2) || !s_out) return CQBEM_ERR_ARG;
+    const auto s = cq_frequencies(L, R, dt, bdf);
+    for (size_t l = 0; l < s.size(); l++) s_out[2 * l] = s[l].real(), s_out[2 * l + 1] = s[l].imag();
+    return (int)s.size();
+}
+
+int cqbem_job_footprint(const cqbem_mesh *mesh, const cqbem_params *params, int outputs_on_device, uint64_t *bytes) {
+    try {
+        check_inputs(mesh, params);
+        if (!bytes) throw Err{CQBEM_ERR_ARG, "null bytes"};
+        *bytes = footprint(mesh, params, outputs_on_device, (size_t)8 << 30);
+    } catch (const Err &e) {
+        return e.code;
+    }
+    return CQBEM_OK;
+}
+
+int cqbem_assemble(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params *params, cqbem_outputs *out) {
+    if (!ctx) return CQBEM_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const auto t0 = std::chrono::steady_clock::now();
+    double tm[5] = {0, 0, 0, 0, 0};
+    try {
+        check_inputs(mesh, params);
+        if (!out) throw Err{CQBEM_ERR_ARG, "null outputs"};
+        HIPCHK(hipSetDevice(ctx->device));
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        const uint64_t avail = std::min<uint64_t>(ctx->budget, (uint64_t)fr);
+        // everything but the Laplace-domain scratch, then give the scratch what is left (capped)
+        const uint64_t fixed = footprint(mesh, params, out->on_device, 0);
+        if (fixed > avail) throw Err{CQBEM_ERR_NOMEM, "job does not fit the HBM budget"};
+        const size_t slab = (size_t)std::min<uint64_t>(avail - fixed, (uint64_t)48 << 30);
+        run_job(ctx, ctx->stream, mesh, params, out, slab, tm, true);
+    } catch (const Err &e) {
+        return fail(ctx, e);
+    }
+    tm[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::copy(tm, tm + 5, ctx->timings);
+    return CQBEM_OK;
+}
+
+int cqbem_last_timings(const cqbem_context *ctx, double *ms5) {
+    if (!ctx || !ms5) return CQBEM_ERR_ARG;
+    std::copy(ctx->timings, ctx->timings + 5, ms5);
+    return CQBEM_OK;
+}
+
+int cqbem_laplace(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params *params, int32_t l, double *U, double *Pm,
+                  double *Bt, double *Ba) {
+    if (!ctx) return CQBEM_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    try {
+        check_inputs(mesh, params);
+        if (l < 0 || l > params->L / 2) throw Err{CQBEM_ERR_ARG, "frequency index out of range"};
+        HIPCHK(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        Prepared P = prepare(mesh, params);
+        DevMesh d;
+        upload_mesh(d, mesh, P, st);
+        const int nv = P.nv;
+        const long long n3 = 3LL * nv;
+        const bool wG = U && (params->which & CQBEM_G), wH = Pm && (params->which & CQBEM_H);
+        const bool wB = (Bt || Ba) && (params->which & (CQBEM_B_TRANS | CQBEM_B_ANGULAR));
+        DBuf dU, dP, dBt, dBa;
+        if (wG) dU.alloc((size_t)n3 * n3 * sizeof(cd));
+        if (wH) dP.alloc((size_t)n3 * n3 * sizeof(cd));
+        dBt.alloc((size_t)n3 * 3 * sizeof(cd)), dBa.alloc((size_t)n3 * 9 * sizeof(cd));
+        const double4 *quad = d.quad.as<double4>();
+        const double2 *line = d.line.as<double2>();
+        const Freq *freqs = d.freqs.as<Freq>();
+        const int npts = (int)P.quad.size(), nline = (int)P.line.size();
+        if (!params->galerkin) {
+            if (wG || wH) {
+                HIPCHK(launch_colloc_gh(d.g, quad, npts, freqs, l, 1, P.mat, 0, nv, dU.as<cd>(), dP.as<cd>(), n3, n3 * n3, st));
+                if (wG) HIPCHK(launch_colloc_singular(0, d.g, line, nline, freqs, l, 1, P.mat, 0, nv, dU.as<cd>(), n3, n3 * n3, nullptr, nullptr, st));
+                if (wH) HIPCHK(launch_colloc_singular(2, d.g, line, nline, freqs, l, 1, P.mat, 0, nv, dP.as<cd>(), n3, n3 * n3, nullptr, nullptr, st));
+            }
+            if (wB) {
+                HIPCHK(launch_colloc_b(d.g, quad, npts, freqs, l, 1, P.mat, 0, nv, dBt.as<cd>(), dBa.as<cd>(), st));
+                HIPCHK(launch_colloc_singular(1, d.g, line, nline, freqs, l, 1, P.mat, 0, nv, nullptr, n3, 0, dBt.as<cd>(),
+                                              dBa.as<cd>(), st));
+            }
+        } else {
+            GalerkinJob gj{d.g, quad, npts, line, nline, freqs + l, 1, P.mat, ctx->num_cus};
+            double tm[5] = {0, 0, 0, 0, 0};
+            run_galerkin(gj, P.vs_ptr, P.vs_face, mesh->F, wG ? dU.as<cd>() : nullptr, wH ? dP.as<cd>() : nullptr,
+                         wB ? dBt.as<cd>() : nullptr, wB ? dBa.as<cd>() : nullptr, tm, st);
+        }
+        if (wG) HIPCHK(hipMemcpyAsync(U, dU.p, dU.bytes, hipMemcpyDeviceToHost, st));
+        if (wH) HIPCHK(hipMemcpyAsync(Pm, dP.p, dP.bytes, hipMemcpyDeviceToHost, st));
+        if (wB && Bt) HIPCHK(hipMemcpyAsync(Bt, dBt.p, dBt.bytes, hipMemcpyDeviceToHost, st));
+        if (wB && Ba) HIPCHK(hipMemcpyAsync(Ba, dBa.p, dBa.bytes, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    } catch (const Err &e) {
+        return fail(ctx, e);
+    }
+    return CQBEM_OK;
+}
+
+// ---- serving ----
+static constexpr int kSlots = 4;
+
+int cqbem_submit(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params *params, const cqbem_outputs *out,
+                 uint64_t *job_id) {
+    if (!ctx || !job_id) return CQBEM_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    try {
+        check_inputs(mesh, params);
+        if (!out || !out->on_device) throw Err{CQBEM_ERR_ARG, "serving needs device-resident outputs"};
+        HIPCHK(hipSetDevice(ctx->device));
+        if (ctx->slots.empty()) {
+            ctx->slots.resize(kSlots), ctx->slot_job.assign(kSlots, 0);
+            for (auto &s : ctx->slots) HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        }
+        // reap finished jobs' slots
+        for (int k = 0; k < kSlots; k++) {
+            const uint64_t jid = ctx->slot_job[k];
+            if (jid && ctx->jobs.count(jid) && ctx->jobs[jid]->done.load()) ctx->slot_job[k] = 0;
+        }
+        const size_t slab = (size_t)2 << 30;
+        const uint64_t bytes = footprint(mesh, params, 1, slab);
+        if (bytes > ctx->budget) throw Err{CQBEM_ERR_NOMEM, "job footprint exceeds the HBM budget"};
+        int slot = -1;
+        for (int k = 0; k < kSlots && slot < 0; k++)
+            if (!ctx->slot_job[k]) slot = k;
+        if (slot < 0 || ctx->reserved + bytes > ctx->budget) {
+            ctx->last_error = "no free slot or HBM budget";
+            return CQBEM_ERR_BUSY;
+        }
+        auto job = std::make_shared<JobRecord>();
+        job->id = ctx->next_job++;
+        job->V.assign(mesh->V, mesh->V + 3 * mesh->nv);
+        job->F.assign(mesh->F, mesh->F + 3 * mesh->nf);
+        if (mesh->vertex_map_inverse) job->vmap.assign(mesh->vertex_map_inverse, mesh->vertex_map_inverse + mesh->nv);
+        job->mesh = cqbem_mesh{job->V.data(), mesh->nv, job->F.data(), mesh->nf,
+                               job->vmap.empty() ? nullptr : job->vmap.data()};
+        job->params = *params;
+        job->out = *out;
+        job->bytes = bytes;
+        job->slot = slot;
+        ctx->reserved += bytes;
+        ctx->slot_job[slot] = job->id;
+        ctx->jobs[job->id] = job;
+        hipStream_t st = ctx->slots[slot];
+        JobRecord *jr = job.get();
+        const int dev = ctx->device;
+        job->worker = std::thread([ctx, jr, st, dev, slab]() {
+            double tm[5] = {0, 0, 0, 0, 0};
+            try {
+                HIPCHK(hipSetDevice(dev));
+                run_job(ctx, st, &jr->mesh, &jr->params, &jr->out, slab, tm, false);
+            } catch (const Err &e) {
+                jr->status = e.code, jr->error = e.msg;
+            }
+            {
+                std::lock_guard<std::mutex> lk2(ctx->mu);
+                ctx->reserved -= jr->bytes;
+            }
+            jr->done.store(1);
+        });
+        *job_id = job->id;
+    } catch (const Err &e) {
+        return fail(ctx, e);
+    }
+    return CQBEM_OK;
+}
+
+int cqbem_poll(cqbem_context *ctx, uint64_t job_id, int32_t *done) {
+    if (!ctx || !done) return CQBEM_ERR_ARG;
+    std::shared_ptr<JobRecord> job;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        auto it = ctx->jobs.find(job_id);
+        if (it == ctx->jobs.end()) return CQBEM_ERR_NOTFOUND;
+        job = it->second;
+    }
+    *done = job->done.load();
+    if (*done && job->status) {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->last_error = job->error;
+        return job->status;
+    }
+    return CQBEM_OK;
+}
+
+int cqbem_wait(cqbem_context *ctx, uint64_t job_id) {
+    if (!ctx) return CQBEM_ERR_ARG;
+    std::shared_ptr<JobRecord> job;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        auto it = ctx->jobs.find(job_id);
+        if (it == ctx->jobs.end()) return CQBEM_ERR_NOTFOUND;
+        job = it->second;
+    }
+    if (job->worker.joinable()) job->worker.join();
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    for (int k = 0; k < (int)ctx->slot_job.size(); k++)
+        if (ctx->slot_job[k] == job_id) ctx->slot_job[k] = 0;
+    ctx->jobs.erase(job_id);
+    if (job->status) {
+        ctx->last_error = job->error;
+        return job->status;
+    }
+    return CQBEM_OK;
+}
+
+int cqbem_reserved_bytes(const cqbem_context *ctx, uint64_t *bytes) {
+    if (!ctx || !bytes) return CQBEM_ERR_ARG;
+    *bytes = ctx->reserved;
+    return CQBEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,556 @@
+// gfx950 kernels of the CQBEM precompute (collocation path, inverse CQ
+// transform, jump/B post-processing).
+//
+// Work decomposition (collocation, regular pairs): one lane owns one output
+// block (row vertex i, column vertex j, frequency l) and integrates the faces
+// of j's vertex star with the basis function of j's corner.  Each block is
+// therefore written exactly once, with no atomics and no read-modify-write;
+// lanes of a wavefront share (i, l), so x_i and the frequency constants live
+// in scalar registers.  Faces touching i (the singular ones) are skipped and
+// integrated by the singular kernels, which own rows (i, l) and add into them.
+//
+// Reference counterparts: compute_elastodynamic_{G,H,B_trans,B_angular}_kernel_collocation
+// (src/elastodynamics_cqbem_*_collocation.cu) and the inverse transform loops
+// of src/elastodynamics_cqbem_{G,H,B_trans,B_angular}.cu.
+#include <hip/hip_runtime.h>
+
+#include "cqbem_device.hpp"
+#include "cqbem_kernels.hpp"
+
+namespace cqbem {
+
+__device__ inline bool same_vertex(const Geom &g, int a, int b) {
+    return a == b || (g.vmap != nullptr && g.vmap[a] == g.vmap[b]);
+}
+
+// ------------------------------------------------------------------------
+// G and H, regular (non-touching) faces.  Grid: (ceil(nv/64), nl, rows).
+// ------------------------------------------------------------------------
+template <bool DG, bool DH>
+__global__ void __launch_bounds__(64) k_colloc_gh(Geom g, const double4 *__restrict__ quad, int npts,
+                                                   const Freq *__restrict__ freqs, int l0, Mat m, int row0,
+                                                   cd *__restrict__ UG, cd *__restrict__ UH, long long ld,
+                                                   long long lstride) {
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int il = blockIdx.z;
+    const int i = row0 + il;
+    const int l = blockIdx.y;
+    if (j >= g.nv) return;
+    const Freq f = freqs[l0 + l];
+    const double x0 = g.V[3 * i], x1 = g.V[3 * i + 1], x2 = g.V[3 * i + 2];
+    cd ga[6], gb = {0, 0}, hh[9];
+    for (int k = 0; k < 6; k++) ga[k] = cd{0, 0};
+    for (int k = 0; k < 9; k++) hh[k] = cd{0, 0};
+
+    const int t_end = g.vs_ptr[j + 1];
+    for (int t = g.vs_ptr[j]; t < t_end; ++t) {
+        const int pk = g.vs_face[t];
+        const int fc = pk >> 2, c = pk & 3;
+        const int a0 = g.F[3 * fc], a1 = g.F[3 * fc + 1], a2 = g.F[3 * fc + 2];
+        if (same_vertex(g, i, a0) || same_vertex(g, i, a1) || same_vertex(g, i, a2)) continue;
+        const double *p0 = g.V + 3 * a0, *p1 = g.V + 3 * a1, *p2 = g.V + 3 * a2;
+        const double y00 = p0[0], y01 = p0[1], y02 = p0[2];
+        const double y10 = p1[0], y11 = p1[1], y12 = p1[2];
+        const double y20 = p2[0], y21 = p2[1], y22 = p2[2];
+        const double n0 = g.N[3 * fc], n1 = g.N[3 * fc + 1], n2 = g.N[3 * fc + 2];
+        const double ar = g.area[fc];
+        cd v1[3], v2[3], s2 = {0, 0}, m4[6];
+        if (DH) {
+            for (int k = 0; k < 3; k++) v1[k] = v2[k] = cd{0, 0};
+            for (int k = 0; k < 6; k++) m4[k] = cd{0, 0};
+        }
+        for (int q = 0; q < npts; ++q) {
+            const double4 Q = quad[q];
+            const double r0 = Q.x * y00 + Q.y * y10 + Q.z * y20 - x0;
+            const double r1 = Q.x * y01 + Q.y * y11 + Q.z * y21 - x1;
+            const double r2 = Q.x * y02 + Q.y * y12 + Q.z * y22 - x2;
+            const double r = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+            const double ir = 1.0 / r, ir2 = ir * ir;
+            const double h0 = r0 * ir, h1 = r1 * ir, h2 = r2 * ir;
+            const double bc = c == 0 ? Q.x : (c == 1 ? Q.y : Q.z);
+            const double wL = Q.w * ar * bc;
+            const Expo E = expo(r, f);
+            if (DG) {
+                cd al, be;
+                g_coef(E, ir, ir2, m, al, be);
+                const double wr = wL * ir;
+                al = al * wr;
+                fma_cr(gb, be, wr);
+                fma_cr(ga[0], al, h0 * h0);
+                fma_cr(ga[1], al, h1 * h1);
+                fma_cr(ga[2], al, h2 * h2);
+                fma_cr(ga[3], al, h0 * h1);
+                fma_cr(ga[4], al, h0 * h2);
+                fma_cr(ga[5], al, h1 * h2);
+            }
+            if (DH) {
+                cd c1, c2, c4;
+                h_coef(E, ir, ir2, f, m, c1, c2, c4);
+                const double rn = h0 * n0 + h1 * n1 + h2 * n2;
+                c1 = c1 * wL;
+                c2 = c2 * wL;
+                c4 = c4 * (wL * rn);
+                fma_cr(v1[0], c1, h0), fma_cr(v1[1], c1, h1), fma_cr(v1[2], c1, h2);
+                fma_cr(v2[0], c2, h0), fma_cr(v2[1], c2, h1), fma_cr(v2[2], c2, h2);
+                fma_cr(s2, c2, rn);
+                fma_cr(m4[0], c4, h0 * h0);
+                fma_cr(m4[1], c4, h1 * h1);
+                fma_cr(m4[2], c4, h2 * h2);
+                fma_cr(m4[3], c4, h0 * h1);
+                fma_cr(m4[4], c4, h0 * h2);
+                fma_cr(m4[5], c4, h1 * h2);
+            }
+        }
+        if (DH) {
+            const double n[3] = {n0, n1, n2};
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) {
+                    cd v = m4[sidx(a, b)];
+                    fma_cr(v, v1[a], n[b]);
+                    fma_cr(v, v2[b], n[a]);
+                    if (a == b) v += s2;
+                    hh[a * 3 + b] += v;
+                }
+        }
+    }
+    const long long base = (long long)l * lstride + (long long)(3 * il) * ld + 3 * (long long)j;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            if (DG) {
+                cd v = ga[sidx(a, b)];
+                if (a == b) v += gb;
+                UG[base + a * ld + b] = v;
+            }
+            if (DH) UH[base + a * ld + b] = hh[a * 3 + b] * kInv4Pi;
+        }
+}
+
+// Sum of a double over a 256-thread block.
+__device__ inline double block_sum256(double v, double *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    return red[0] + red[1] + red[2] + red[3];
+}
+
+// ------------------------------------------------------------------------
+// B_trans and B_angular, regular faces.  Grid: (nl, rows), 256 threads; lanes
+// stride over the faces, then a block reduction.
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_colloc_b(Geom g, const double4 *__restrict__ quad, int npts,
+                                                   const Freq *__restrict__ freqs, int l0, Mat m, int row0,
+                                                   cd *__restrict__ Bt, cd *__restrict__ Ba, long long nv3) {
+    __shared__ double red[4];
+    const int l = blockIdx.x;
+    const int i = row0 + blockIdx.y;
+    const Freq f = freqs[l0 + l];
+    const double x0 = g.V[3 * i], x1 = g.V[3 * i + 1], x2 = g.V[3 * i + 2];
+    cd bt[6], ba[18];
+    for (int k = 0; k < 6; k++) bt[k] = cd{0, 0};
+    for (int k = 0; k < 18; k++) ba[k] = cd{0, 0};
+    for (int fc = threadIdx.x; fc < g.nf; fc += 256) {
+        const int a0 = g.F[3 * fc], a1 = g.F[3 * fc + 1], a2 = g.F[3 * fc + 2];
+        if (same_vertex(g, i, a0) || same_vertex(g, i, a1) || same_vertex(g, i, a2)) continue;
+        const double *p0 = g.V + 3 * a0, *p1 = g.V + 3 * a1, *p2 = g.V + 3 * a2;
+        const double n[3] = {g.N[3 * fc], g.N[3 * fc + 1], g.N[3 * fc + 2]};
+        const double ar = g.area[fc];
+        cd mt[3] = {{0, 0}, {0, 0}, {0, 0}}, m2 = {0, 0}, M[6], S = {0, 0}, S3 = {0, 0};
+        for (int k = 0; k < 6; k++) M[k] = cd{0, 0};
+        for (int q = 0; q < npts; ++q) {
+            const double4 Q = quad[q];
+            const double r0 = Q.x * p0[0] + Q.y * p1[0] + Q.z * p2[0] - x0;
+            const double r1 = Q.x * p0[1] + Q.y * p1[1] + Q.z * p2[1] - x1;
+            const double r2 = Q.x * p0[2] + Q.y * p1[2] + Q.z * p2[2] - x2;
+            const double r = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+            const double ir = 1.0 / r, ir2 = ir * ir;
+            const double h[3] = {r0 * ir, r1 * ir, r2 * ir};
+            const double w = Q.w * ar;
+            const Expo E = expo(r, f);
+            cd t1, t2, u1, u12, u3;
+            b_coef(E, ir, ir2, f, m, t1, t2, u1, u12, u3);
+            const double rn = h[0] * n[0] + h[1] * n[1] + h[2] * n[2];
+            t1 = t1 * w;
+            u1 = u1 * w;
+            for (int a = 0; a < 3; a++) fma_cr(mt[a], t1, h[a]);
+            fma_cr(m2, t2, w * rn);
+            for (int a = 0; a < 3; a++)
+                for (int b = a; b < 3; b++) fma_cr(M[sidx(a, b)], u1, h[a] * h[b]);
+            fma_cr(S, u12, w);
+            fma_cr(S3, u3, w);
+        }
+        for (int a = 0; a < 3; a++)
+            for (int b = a; b < 3; b++) {
+                cd v = {0, 0};
+                fma_cr(v, mt[a], n[b]);
+                fma_cr(v, mt[b], n[a]);
+                if (a == b) v += m2;
+                bt[sidx(a, b)] += v;
+                for (int k = 0; k < 3; k++) {
+                    cd u = {0, 0};
+                    fma_cr(u, M[sidx(a, k)], n[b]);
+                    fma_cr(u, M[sidx(b, k)], n[a]);
+                    fma_cr(u, S, (b == k ? n[a] : 0.0) + (a == k ? n[b] : 0.0));
+                    if (a == b) fma_cr(u, S3, n[k]);
+                    ba[sidx(a, b) * 3 + k] += u;
+                }
+            }
+    }
+    // block reduction and symmetric expansion
+    cd rbt[6], rba[18];
+    for (int k = 0; k < 6; k++) rbt[k] = cd{block_sum256(bt[k].x, red), block_sum256(bt[k].y, red)};
+    for (int k = 0; k < 18; k++) rba[k] = cd{block_sum256(ba[k].x, red), block_sum256(ba[k].y, red)};
+    if (threadIdx.x == 0) {
+        cd *bt_out = Bt + (long long)l * nv3 * 3 + (long long)(3 * i) * 3;
+        cd *ba_out = Ba + (long long)l * nv3 * 9 + (long long)(3 * i) * 9;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                bt_out[a * 3 + b] = rbt[sidx(a, b)];
+                for (int k = 0; k < 3; k++) ba_out[a * 9 + b * 3 + k] = rba[sidx(a, b) * 3 + k];
+            }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Singular faces (faces touching the collocation vertex).  One lane per
+// (row vertex, frequency); the lane owns those rows and adds into them.
+//   KIND 0: G, weakly singular polar rule (integrator_collocation.hpp:19-63)
+//   KIND 1: B_trans + B_angular, same polar rule
+//   KIND 2: H, Guiggiani finite-part rule (integrator_collocation.hpp:65-124)
+// ------------------------------------------------------------------------
+__device__ inline void unit3(double v[3]) {
+    const double n = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (n > 0) v[0] /= n, v[1] /= n, v[2] /= n;
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(64) k_colloc_singular(Geom g, const double2 *__restrict__ line, int nline,
+                                                         const Freq *__restrict__ freqs, int l0, int nl, Mat m, int row0,
+                                                         int nrows, cd *__restrict__ U, long long ld, long long lstride,
+                                                         cd *__restrict__ Bt, cd *__restrict__ Ba, long long nv3) {
+    const int idx = blockIdx.x * 64 + threadIdx.x;
+    if (idx >= nrows * nl) return;
+    const int il = idx / nl, l = idx % nl;
+    const int i = row0 + il;
+    const Freq f = freqs[l0 + l];
+    for (int t = g.sg_ptr[i]; t < g.sg_ptr[i + 1]; ++t) {
+        const int pk = g.sg_face[t];
+        const int fc = pk >> 2, c = pk & 3;
+        const int j[3] = {g.F[3 * fc + c], g.F[3 * fc + (c + 1) % 3], g.F[3 * fc + (c + 2) % 3]};
+        double y1[3], y2[3], y3[3], n[3];
+        for (int k = 0; k < 3; k++)
+            y1[k] = g.V[3 * j[0] + k], y2[k] = g.V[3 * j[1] + k], y3[k] = g.V[3 * j[2] + k], n[k] = g.N[3 * fc + k];
+        double v1[3], v2[3];
+        for (int k = 0; k < 3; k++) v1[k] = y2[k] - y1[k], v2[k] = y3[k] - y1[k];
+        if (KIND == 0 || KIND == 1) {
+            // polar frame around y1
+            double u1[3] = {v1[0], v1[1], v1[2]}, u2[3] = {v2[0], v2[1], v2[2]};
+            unit3(u1), unit3(u2);
+            const double xh[3] = {u1[0], u1[1], u1[2]};
+            double cxu[3] = {xh[1] * u2[2] - xh[2] * u2[1], xh[2] * u2[0] - xh[0] * u2[2], xh[0] * u2[1] - xh[1] * u2[0]};
+            double yh[3] = {cxu[1] * xh[2] - cxu[2] * xh[1], cxu[2] * xh[0] - cxu[0] * xh[2], cxu[0] * xh[1] - cxu[1] * xh[0]};
+            unit3(yh);
+            double e[3] = {v1[0] - v2[0], v1[1] - v2[1], v1[2] - v2[2]};
+            unit3(e);
+            const double d = v1[0] * e[0] + v1[1] * e[1] + v1[2] * e[2];
+            double v12[3] = {v1[0] - d * e[0], v1[1] - d * e[1], v1[2] - d * e[2]};
+            const double a = sqrt(v12[0] * v12[0] + v12[1] * v12[1] + v12[2] * v12[2]);
+            unit3(v12);
+            const double alpha = acos(u1[0] * u2[0] + u1[1] * u2[1] + u1[2] * u2[2]);
+            const double cb = fmin(1.0, fmax(-1.0, v12[0] * u1[0] + v12[1] * u1[1] + v12[2] * u1[2]));
+            const double beta = acos(cb);
+            // barycentric projection setup (barycentric_coordinates.cuh:15-38)
+            const double d00 = v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2];
+            const double d01 = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
+            const double d11 = v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2];
+            const double iden = 1.0 / (d00 * d11 - d01 * d01);
+            cd ga[3][6], gb[3];
+            cd mt[3], m2, M[6], S, S3;
+            if (KIND == 0) {
+                for (int k = 0; k < 3; k++) {
+                    gb[k] = cd{0, 0};
+                    for (int e2 = 0; e2 < 6; e2++) ga[k][e2] = cd{0, 0};
+                }
+            } else {
+                for (int k = 0; k < 3; k++) mt[k] = cd{0, 0};
+                for (int k = 0; k < 6; k++) M[k] = cd{0, 0};
+                m2 = S = S3 = cd{0, 0};
+            }
+            for (int it = 0; it < nline; ++it) {
+                const double th = alpha * line[it].x, wth = alpha * line[it].y;
+                double sth, cth;
+                sincos(th, &sth, &cth);
+                const double h[3] = {xh[0] * cth + yh[0] * sth, xh[1] * cth + yh[1] * sth, xh[2] * cth + yh[2] * sth};
+                const double rmax = a / cos(th - beta);
+                const double rn = h[0] * n[0] + h[1] * n[1] + h[2] * n[2];
+                for (int ir_ = 0; ir_ < nline; ++ir_) {
+                    const double r = rmax * line[ir_].x, w = wth * rmax * line[ir_].y;
+                    const double ir = 1.0 / r, ir2 = ir * ir;
+                    const Expo E = expo(r, f);
+                    if (KIND == 0) {
+                        // clamped projection barycentrics of y = y1 + r h
+                        const double c0 = r * h[0], c1 = r * h[1], c2 = r * h[2];
+                        const double d20 = c0 * v1[0] + c1 * v1[1] + c2 * v1[2];
+                        const double d21 = c0 * v2[0] + c1 * v2[1] + c2 * v2[2];
+                        double L1 = (d11 * d20 - d01 * d21) * iden, L2 = (d00 * d21 - d01 * d20) * iden;
+                        double L0 = 1.0 - (L1 + L2);
+                        L0 = fmin(1.0, fmax(0.0, L0)), L1 = fmin(1.0, fmax(0.0, L1)), L2 = fmin(1.0, fmax(0.0, L2));
+                        const double Lb[3] = {L0, L1, L2};
+                        cd al, be;
+                        g_coef(E, ir, ir2, m, al, be);  // r * u_hat coefficients
+                        for (int k = 0; k < 3; k++) {
+                            const double wk = w * Lb[k];
+                            const cd ak = al * wk;
+                            fma_cr(gb[k], be, wk);
+                            fma_cr(ga[k][0], ak, h[0] * h[0]);
+                            fma_cr(ga[k][1], ak, h[1] * h[1]);
+                            fma_cr(ga[k][2], ak, h[2] * h[2]);
+                            fma_cr(ga[k][3], ak, h[0] * h[1]);
+                            fma_cr(ga[k][4], ak, h[0] * h[2]);
+                            fma_cr(ga[k][5], ak, h[1] * h[2]);
+                        }
+                    } else {
+                        cd t1, t2, u1c, u12, u3;
+                        b_coef(E, ir, ir2, f, m, t1, t2, u1c, u12, u3);
+                        const double wr = w * r;  // polar rules integrate r * kernel
+                        t1 = t1 * wr;
+                        u1c = u1c * wr;
+                        for (int a2 = 0; a2 < 3; a2++) fma_cr(mt[a2], t1, h[a2]);
+                        fma_cr(m2, t2, wr * rn);
+                        for (int a2 = 0; a2 < 3; a2++)
+                            for (int b = a2; b < 3; b++) fma_cr(M[sidx(a2, b)], u1c, h[a2] * h[b]);
+                        fma_cr(S, u12, wr);
+                        fma_cr(S3, u3, wr);
+                    }
+                }
+            }
+            const long long base = (long long)l * lstride + (long long)(3 * il) * ld;
+            if (KIND == 0) {
+                for (int k = 0; k < 3; k++)
+                    for (int a2 = 0; a2 < 3; a2++)
+                        for (int b = 0; b < 3; b++) {
+                            cd v = ga[k][sidx(a2, b)];
+                            if (a2 == b) v += gb[k];
+                            U[base + a2 * ld + 3 * (long long)j[k] + b] += v;
+                        }
+            } else {
+                cd *bt_out = Bt + (long long)l * nv3 * 3 + (long long)(3 * i) * 3;
+                cd *ba_out = Ba + (long long)l * nv3 * 9 + (long long)(3 * i) * 9;
+                for (int a2 = 0; a2 < 3; a2++)
+                    for (int b = 0; b < 3; b++) {
+                        cd v = {0, 0};
+                        fma_cr(v, mt[a2], n[b]);
+                        fma_cr(v, mt[b], n[a2]);
+                        if (a2 == b) v += m2;
+                        bt_out[a2 * 3 + b] += v;
+                        for (int k = 0; k < 3; k++) {
+                            cd u = {0, 0};
+                            fma_cr(u, M[sidx(a2, k)], n[b]);
+                            fma_cr(u, M[sidx(b, k)], n[a2]);
+                            fma_cr(u, S, (b == k ? n[a2] : 0.0) + (a2 == k ? n[b] : 0.0));
+                            if (a2 == b) fma_cr(u, S3, n[k]);
+                            ba_out[a2 * 9 + b * 3 + k] += u;
+                        }
+                    }
+            }
+        } else {
+            // Guiggiani: rho-theta on the unit right triangle around y1
+            double cr[3] = {v1[1] * v2[2] - v1[2] * v2[1], v1[2] * v2[0] - v1[0] * v2[2], v1[0] * v2[1] - v1[1] * v2[0]};
+            const double J = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+            cd out[27];
+            for (int k = 0; k < 27; k++) out[k] = cd{0, 0};
+            const double half_pi = 1.5707963267948966192;
+            for (int it = 0; it < nline; ++it) {
+                const double th = half_pi * line[it].x, wth = half_pi * line[it].y;
+                double st, ct;
+                sincos(th, &st, &ct);
+                double Av[3] = {ct * v1[0] + st * v2[0], ct * v1[1] + st * v2[1], ct * v1[2] + st * v2[2]};
+                const double A = sqrt(Av[0] * Av[0] + Av[1] * Av[1] + Av[2] * Av[2]);
+                const double Ah[3] = {Av[0] / A, Av[1] / A, Av[2] / A};
+                const double rho_bar = 1.0 / (cos(th - 0.78539816339744830962) * 1.4142135623730950488);
+                const double rn = Ah[0] * n[0] + Ah[1] * n[1] + Ah[2] * n[2];
+                // cancelling term f_pq = (1/4pi) kap (Ah_p n_q - Ah_q n_p) / A^2 * J, basis (1,0,0)
+                const double fs = kInv4Pi * m.kap * J / (A * A);
+                cd S1[3], S2[3], S4[3];
+                for (int k = 0; k < 3; k++) S1[k] = S2[k] = S4[k] = cd{0, 0};
+                double sinv = 0.0;
+                for (int ir_ = 0; ir_ < nline; ++ir_) {
+                    const double rho = rho_bar * line[ir_].x, w = rho_bar * line[ir_].y;
+                    const double r = rho * A;
+                    const double ir = 1.0 / r, ir2 = ir * ir;
+                    const Expo E = expo(r, f);
+                    cd c1, c2, c4;
+                    h_coef(E, ir, ir2, f, m, c1, c2, c4);
+                    const double Lb[3] = {1.0 - rho * (ct + st), rho * ct, rho * st};
+                    for (int k = 0; k < 3; k++) {
+                        const double wk = w * Lb[k] * J * rho;
+                        fma_cr(S1[k], c1, wk);
+                        fma_cr(S2[k], c2, wk);
+                        fma_cr(S4[k], c4, wk);
+                    }
+                    sinv += w / rho;
+                }
+                const double lg = log(rho_bar * A);
+                for (int p = 0; p < 3; p++)
+                    for (int qq = 0; qq < 3; qq++) {
+                        const double fpq = fs * (Ah[p] * n[qq] - Ah[qq] * n[p]);
+                        for (int k = 0; k < 3; k++) {
+                            cd v = {0, 0};
+                            fma_cr(v, S1[k], Ah[p] * n[qq]);
+                            fma_cr(v, S2[k], n[p] * Ah[qq] + (p == qq ? rn : 0.0));
+                            fma_cr(v, S4[k], rn * Ah[p] * Ah[qq]);
+                            v = v * kInv4Pi;
+                            if (k == 0) v.x += fpq * (lg - sinv);
+                            fma_cr(out[(p * 3 + qq) * 3 + k], v, wth);
+                        }
+                    }
+            }
+            const long long base = (long long)l * lstride + (long long)(3 * il) * ld;
+            for (int p = 0; p < 3; p++)
+                for (int qq = 0; qq < 3; qq++)
+                    for (int k = 0; k < 3; k++) U[base + p * ld + 3 * (long long)j[k] + qq] += out[(p * 3 + qq) * 3 + k];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Inverse CQ transform: W_n[e] = sum_l Re(X_l[e] * T[n][l]), with the table
+// T[n][l] = R^-n / L * mult_l * zeta^(-n l) (G.cu:166-179).  One lane per
+// entry, NC weights per pass; the table index is uniform (scalar loads).
+// ------------------------------------------------------------------------
+template <int NC>
+__global__ void __launch_bounds__(256) k_transform(const cd *__restrict__ spec, long long E, int Lf,
+                                                    const double2 *__restrict__ tab, int L, int n0,
+                                                    double *__restrict__ out, long long out_nstride) {
+    const long long stride = (long long)gridDim.x * 256;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < E; e += stride) {
+        double acc[NC];
+#pragma unroll
+        for (int k = 0; k < NC; k++) acc[k] = 0.0;
+        for (int l = 0; l < Lf; l++) {
+            const cd X = spec[(long long)l * E + e];
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+                const int n = n0 + k;
+                if (n < L) {
+                    const double2 T = tab[(long long)n * Lf + l];
+                    acc[k] = fma(X.x, T.x, fma(-X.y, T.y, acc[k]));
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            const int n = n0 + k;
+            if (n < L) out[(long long)n * out_nstride + e] = acc[k];
+        }
+    }
+}
+
+// H_0 += C on the 3x3 diagonal blocks (collocation, H.cu:186-191).
+__global__ void k_add_c_colloc(double *H0, const double *C, int nv) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= 9 * nv) return;
+    const int v = t / 9, a = (t % 9) / 3, b = t % 3;
+    const long long n3 = 3LL * nv;
+    H0[(3LL * v + a) * n3 + 3 * v + b] += C[t];
+}
+
+// H_0 += C for Galerkin: sparse (row, col, value) triplets, one lane each, rows unique per launch.
+__global__ void k_add_triplets(double *H0, const long long *idx, const double *val, long long n) {
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    H0[idx[t]] += val[t];
+}
+
+// B_angular += B_trans (x) x_v and B = [B_trans | B_euler] (B_angular.cu:167-175, matrix_collection.cpp:280-305).
+__global__ void k_b_post(double *Bt, double *Ba, double *B6, const double *V, int nv, int L) {
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long nr = 3LL * nv;
+    if (t >= nr * L) return;
+    const long long n = t / nr, r = t % nr;
+    const double *bt = Bt + (n * nr + r) * 3;
+    double *ba = Ba + (n * nr + r) * 9;
+    const double *x = V + 3 * (r / 3);
+    for (int j = 0; j < 3; j++)
+        for (int k = 0; k < 3; k++) ba[j * 3 + k] += bt[j] * x[k];
+    if (B6) {
+        double *b6 = B6 + (n * nr + r) * 6;
+        for (int j = 0; j < 3; j++) b6[j] = bt[j];
+        for (int j = 0; j < 3; j++) b6[3 + j] = ba[((j + 2) % 3) * 3 + (j + 1) % 3] - ba[((j + 1) % 3) * 3 + (j + 2) % 3];
+    }
+}
+
+// ------------------------------------------------------------- launchers --
+hipError_t launch_colloc_gh(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
+                            int row0, int nrows, cd *UG, cd *UH, long long ld, long long lstride, hipStream_t st) {
+    dim3 grid((g.nv + 63) / 64, nl, nrows);
+    if (UG && UH)
+        hipLaunchKernelGGL((k_colloc_gh<true, true>), grid, dim3(64), 0, st, g, quad, npts, freqs, l0, m, row0, UG, UH, ld,
+                           lstride);
+    else if (UG)
+        hipLaunchKernelGGL((k_colloc_gh<true, false>), grid, dim3(64), 0, st, g, quad, npts, freqs, l0, m, row0, UG, UH, ld,
+                           lstride);
+    else if (UH)
+        hipLaunchKernelGGL((k_colloc_gh<false, true>), grid, dim3(64), 0, st, g, quad, npts, freqs, l0, m, row0, UG, UH, ld,
+                           lstride);
+    return hipGetLastError();
+}
+
+hipError_t launch_colloc_b(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
+                           int row0, int nrows, cd *Bt, cd *Ba, hipStream_t st) {
+    dim3 grid(nl, nrows);
+    hipLaunchKernelGGL(k_colloc_b, grid, dim3(256), 0, st, g, quad, npts, freqs, l0, m, row0, Bt, Ba, 3LL * g.nv);
+    return hipGetLastError();
+}
+
+hipError_t launch_colloc_singular(int kind, const Geom &g, const double2 *line, int nline, const Freq *freqs, int l0, int nl,
+                                  const Mat &m, int row0, int nrows, cd *U, long long ld, long long lstride, cd *Bt,
+                                  cd *Ba, hipStream_t st) {
+    const int n = nrows * nl;
+    dim3 grid((n + 63) / 64);
+    if (kind == 0)
+        hipLaunchKernelGGL((k_colloc_singular<0>), grid, dim3(64), 0, st, g, line, nline, freqs, l0, nl, m, row0, nrows, U,
+                           ld, lstride, Bt, Ba, 3LL * g.nv);
+    else if (kind == 1)
+        hipLaunchKernelGGL((k_colloc_singular<1>), grid, dim3(64), 0, st, g, line, nline, freqs, l0, nl, m, row0, nrows, U,
+                           ld, lstride, Bt, Ba, 3LL * g.nv);
+    else
+        hipLaunchKernelGGL((k_colloc_singular<2>), grid, dim3(64), 0, st, g, line, nline, freqs, l0, nl, m, row0, nrows, U,
+                           ld, lstride, Bt, Ba, 3LL * g.nv);
+    return hipGetLastError();
+}
+
+hipError_t launch_transform(const cd *spec, long long E, int Lf, const double2 *tab, int L, double *out, long long out_nstride,
+                            int num_cus, hipStream_t st) {
+    long long blocks = (E + 255) / 256;
+    const long long cap = (long long)num_cus * 16;
+    if (blocks > cap) blocks = cap;
+    if (blocks < 1) blocks = 1;
+    for (int n0 = 0; n0 < L; n0 += 32) {
+        hipLaunchKernelGGL(k_transform<32>, dim3((unsigned)blocks), dim3(256), 0, st, spec, E, Lf, tab, L, n0, out,
+                           out_nstride);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_add_c_colloc(double *H0, const double *C, int nv, hipStream_t st) {
+    hipLaunchKernelGGL(k_add_c_colloc, dim3((9 * nv + 255) / 256), dim3(256), 0, st, H0, C, nv);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_triplets(double *H0, const long long *idx, const double *val, long long n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_add_triplets, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, H0, idx, val, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_b_post(double *Bt, double *Ba, double *B6, const double *V, int nv, int L, hipStream_t st) {
+    const long long n = 3LL * nv * L;
+    hipLaunchKernelGGL(k_b_post, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, Bt, Ba, B6, V, nv, L);
+    return hipGetLastError();
+}
+
+}  // namespace cqbem

@@ -1,0 +1,36 @@
+// Kernel-side data structures and launchers shared by the host driver.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "cqbem_device.hpp"
+
+namespace cqbem {
+
+// Device-resident mesh of one job.
+struct Geom {
+    const double *V;     // nv x 3
+    const int *F;        // nf x 3
+    const double *N;     // nf x 3 unit normals
+    const double *area;  // nf
+    const int *vs_ptr;   // nv+1: vertex star CSR (faces incident to each vertex)
+    const int *vs_face;  // (face << 2) | corner, ascending face index
+    const int *sg_ptr;   // nv+1: faces singular for each collocation vertex
+    const int *sg_face;  // (face << 2) | first matching corner
+    const int *vmap;     // nullable vertex_map_inverse
+    int nv, nf;
+};
+
+hipError_t launch_colloc_gh(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
+                            int row0, int nrows, cd *UG, cd *UH, long long ld, long long lstride, hipStream_t st);
+hipError_t launch_colloc_b(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
+                           int row0, int nrows, cd *Bt, cd *Ba, hipStream_t st);
+hipError_t launch_colloc_singular(int kind, const Geom &g, const double2 *line, int nline, const Freq *freqs, int l0, int nl,
+                                  const Mat &m, int row0, int nrows, cd *U, long long ld, long long lstride, cd *Bt,
+                                  cd *Ba, hipStream_t st);
+hipError_t launch_transform(const cd *spec, long long E, int Lf, const double2 *tab, int L, double *out, long long out_nstride,
+                            int num_cus, hipStream_t st);
+hipError_t launch_add_c_colloc(double *H0, const double *C, int nv, hipStream_t st);
+hipError_t launch_add_triplets(double *H0, const long long *idx, const double *val, long long n, hipStream_t st);
+hipError_t launch_b_post(double *Bt, double *Ba, double *B6, const double *V, int nv, int L, hipStream_t st);
+
+}  // namespace cqbem
