@@ -98,9 +98,13 @@ int cqbem_laplace(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params
 /* CQ frequencies s_l = gamma(R zeta^l)/dt, l = 0..L/2, complex interleaved (G.cu:19-24,65). Returns L/2+1. */
 int cqbem_frequencies(int32_t L, double R, double dt, int32_t bdf, double *s_out);
 
-/* Timings of the last cqbem_assemble on this context, milliseconds: [0] total, [1] quadrature kernels,
- * [2] singular kernels, [3] inverse transform, [4] host staging. */
-int cqbem_last_timings(const cqbem_context *ctx, double *ms5);
+/* Statistics of the last cqbem_assemble on this context (device times from HIP events on the context's
+ * stream, milliseconds): [0] total wall, [1] regular G/H quadrature (the dominant kernel), [2] regular
+ * B_trans/B_angular quadrature, [3] singular-pair kernels, [4] inverse transform and post-processing,
+ * [5] host preparation and staging, [6] launches of the dominant kernel, [7] far-field kernel evaluations
+ * (collocation point or test point, source point, frequency) those launches performed.
+ * Copies min(n, 8) values; returns that count. */
+int cqbem_last_timings(const cqbem_context *ctx, double *out, int32_t n);
 
 /* ---- serving: continuous batching of assembly jobs on one context ----
  * cqbem_submit admits a job when its device footprint fits the budget next to the jobs in flight,

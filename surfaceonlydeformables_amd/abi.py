@@ -58,6 +58,12 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built; run `python -m surfaceonlydeformables_amd.build` "
                                "(or __graft_entry__.build()) first")
+        # One HIP runtime per process: torch ships its own libamdhip64 (same soname as ROCm's).  Loading
+        # torch first makes libcqbem bind to that copy, so torch tensors and the library share a runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         P = C.POINTER
         L.cqbem_create.argtypes = [P(C.c_void_p), C.c_int, C.c_uint64]
@@ -69,7 +75,7 @@ def lib():
         L.cqbem_laplace.argtypes = [C.c_void_p, P(cqbem_mesh), P(cqbem_params), C.c_int32, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p]
         L.cqbem_frequencies.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_int32, P(C.c_double)]
-        L.cqbem_last_timings.argtypes = [C.c_void_p, P(C.c_double)]
+        L.cqbem_last_timings.argtypes = [C.c_void_p, P(C.c_double), C.c_int32]
         L.cqbem_submit.argtypes = [C.c_void_p, P(cqbem_mesh), P(cqbem_params), P(cqbem_outputs), P(C.c_uint64)]
         L.cqbem_poll.argtypes = [C.c_void_p, C.c_uint64, P(C.c_int32)]
         L.cqbem_wait.argtypes = [C.c_void_p, C.c_uint64]
@@ -133,10 +139,15 @@ class Context:
     def handle(self):
         return self._h
 
+    TIMING_KEYS = ["total", "regular_gh", "regular_b", "singular", "transform", "host", "gh_launches",
+                   "gh_evaluations"]
+
     def last_timings(self):
-        t = np.zeros(5)
-        self._check(lib().cqbem_last_timings(self._h, t.ctypes.data_as(C.POINTER(C.c_double))))
-        return dict(zip(["total", "quadrature", "singular", "transform", "host"], t.tolist()))
+        t = np.zeros(8)
+        n = lib().cqbem_last_timings(self._h, t.ctypes.data_as(C.POINTER(C.c_double)), 8)
+        if n < 0:
+            self._check(n)
+        return dict(zip(self.TIMING_KEYS, t.tolist()))
 
     def assemble(self, mesh: cqbem_mesh, params: cqbem_params, want=("G", "H", "B_trans", "B_angular", "B")):
         """Host-output assembly: returns dict of numpy arrays (L, 3nv, 3nv) / (L, 3nv, 3) / (L, 3nv, 3, 3) / (L, 3nv, 6)."""

@@ -1,6 +1,742 @@
+// Galerkin assembly on gfx950.
+//
+// Regular (non-touching) face pairs, G and H: one lane per (test face f1, source
+// vertex j, frequency l).  The lane walks f1's test points and, for each, j's
+// vertex star with j's basis function (integrate_galerkin_nonsingular,
+// integrator_galerkin.hpp:214-260, restricted to one source corner), then
+// projects onto f1's three test basis functions.  Test faces are launched by
+// vertex-disjoint colour classes, so the three row blocks a lane adds into are
+// owned by that lane alone within a launch: plain read-modify-write, no atomics,
+// a fixed summation order.
+//
+// Regular pairs, B_trans / B_angular: one 256-lane block per (test face, l); lanes
+// stride over source faces per test point, a block reduction per test point,
+// test-basis projection kept in LDS.
+//
+// Touching pairs (Sauter-Schwab, integrator_galerkin.hpp:42-212): one 64-lane
+// block per (pair, l); lanes split the 4D tensor rule, reduce, and write the
+// pair's corner blocks to a per-pair scratch.  A gather kernel then owns each
+// output block and adds its pairs' contributions in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
 #include "cqbem_galerkin.hpp"
 
 namespace cqbem {
-void run_galerkin(const GalerkinJob &, const std::vector<int> &, const std::vector<int> &, const int *, cd *, cd *, cd *, cd *,
-                  double *, hipStream_t) {}
+
+namespace {
+
+__device__ inline bool same_v(const Geom &g, int a, int b) { return a == b || (g.vmap != nullptr && g.vmap[a] == g.vmap[b]); }
+
+__device__ inline bool faces_touch(const Geom &g, const int *f1v, int f2) {
+    const int b0 = g.F[3 * f2], b1 = g.F[3 * f2 + 1], b2 = g.F[3 * f2 + 2];
+    for (int k = 0; k < 3; k++)
+        if (same_v(g, f1v[k], b0) || same_v(g, f1v[k], b1) || same_v(g, f1v[k], b2)) return true;
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// Regular G/H: grid (ceil(nv/64), nl, faces in colour)
+// ---------------------------------------------------------------------------
+template <bool DG, bool DH>
+__global__ void __launch_bounds__(64) k_gal_gh(Geom g, const double4 *__restrict__ quad, int npts,
+                                                const Freq *__restrict__ freqs, Mat m, const int *__restrict__ faces,
+                                                cd *__restrict__ UG, cd *__restrict__ UH, long long ld, long long lstride) {
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int l = blockIdx.y;
+    const int f1 = faces[blockIdx.z];
+    if (j >= g.nv) return;
+    const Freq f = freqs[l];
+    const int tv[3] = {g.F[3 * f1], g.F[3 * f1 + 1], g.F[3 * f1 + 2]};
+    const double *X0 = g.V + 3 * tv[0], *X1 = g.V + 3 * tv[1], *X2 = g.V + 3 * tv[2];
+    const double tarea = g.area[f1];
+    cd ga[3][6], gb[3], hh[3][9];
+    for (int k = 0; k < 3; k++) {
+        gb[k] = cd{0, 0};
+        for (int e = 0; e < 6; e++) ga[k][e] = cd{0, 0};
+        for (int e = 0; e < 9; e++) hh[k][e] = cd{0, 0};
+    }
+    const int t0 = g.vs_ptr[j], t1 = g.vs_ptr[j + 1];
+    for (int a = 0; a < npts; ++a) {
+        const double4 Qa = quad[a];
+        const double x0 = Qa.x * X0[0] + Qa.y * X1[0] + Qa.z * X2[0];
+        const double x1 = Qa.x * X0[1] + Qa.y * X1[1] + Qa.z * X2[1];
+        const double x2 = Qa.x * X0[2] + Qa.y * X1[2] + Qa.z * X2[2];
+        const double wa = Qa.w * tarea;
+        cd pa[6], pb = {0, 0}, ph[9];
+        for (int e = 0; e < 6; e++) pa[e] = cd{0, 0};
+        for (int e = 0; e < 9; e++) ph[e] = cd{0, 0};
+        for (int t = t0; t < t1; ++t) {
+            const int pk = g.vs_face[t];
+            const int fc = pk >> 2, c = pk & 3;
+            if (faces_touch(g, tv, fc)) continue;
+            const int a0 = g.F[3 * fc], a1 = g.F[3 * fc + 1], a2 = g.F[3 * fc + 2];
+            const double *p0 = g.V + 3 * a0, *p1 = g.V + 3 * a1, *p2 = g.V + 3 * a2;
+            const double n0 = g.N[3 * fc], n1 = g.N[3 * fc + 1], n2 = g.N[3 * fc + 2];
+            const double ar = g.area[fc];
+            cd v1[3], v2[3], s2 = {0, 0}, m4[6];
+            if (DH) {
+                for (int k = 0; k < 3; k++) v1[k] = v2[k] = cd{0, 0};
+                for (int k = 0; k < 6; k++) m4[k] = cd{0, 0};
+            }
+            for (int q = 0; q < npts; ++q) {
+                const double4 Q = quad[q];
+                const double r0 = Q.x * p0[0] + Q.y * p1[0] + Q.z * p2[0] - x0;
+                const double r1 = Q.x * p0[1] + Q.y * p1[1] + Q.z * p2[1] - x1;
+                const double r2 = Q.x * p0[2] + Q.y * p1[2] + Q.z * p2[2] - x2;
+                const double r = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+                const double ir = 1.0 / r, ir2 = ir * ir;
+                const double h0 = r0 * ir, h1 = r1 * ir, h2 = r2 * ir;
+                const double bc = c == 0 ? Q.x : (c == 1 ? Q.y : Q.z);
+                const double wL = Q.w * ar * bc;
+                const Expo E = expo(r, f);
+                if (DG) {
+                    cd al, be;
+                    g_coef(E, ir, ir2, m, al, be);
+                    const double wr = wL * ir;
+                    al = al * wr;
+                    fma_cr(pb, be, wr);
+                    fma_cr(pa[0], al, h0 * h0);
+                    fma_cr(pa[1], al, h1 * h1);
+                    fma_cr(pa[2], al, h2 * h2);
+                    fma_cr(pa[3], al, h0 * h1);
+                    fma_cr(pa[4], al, h0 * h2);
+                    fma_cr(pa[5], al, h1 * h2);
+                }
+                if (DH) {
+                    cd c1, c2, c4;
+                    h_coef(E, ir, ir2, f, m, c1, c2, c4);
+                    const double rn = h0 * n0 + h1 * n1 + h2 * n2;
+                    c1 = c1 * wL;
+                    c2 = c2 * wL;
+                    c4 = c4 * (wL * rn);
+                    fma_cr(v1[0], c1, h0), fma_cr(v1[1], c1, h1), fma_cr(v1[2], c1, h2);
+                    fma_cr(v2[0], c2, h0), fma_cr(v2[1], c2, h1), fma_cr(v2[2], c2, h2);
+                    fma_cr(s2, c2, rn);
+                    fma_cr(m4[0], c4, h0 * h0);
+                    fma_cr(m4[1], c4, h1 * h1);
+                    fma_cr(m4[2], c4, h2 * h2);
+                    fma_cr(m4[3], c4, h0 * h1);
+                    fma_cr(m4[4], c4, h0 * h2);
+                    fma_cr(m4[5], c4, h1 * h2);
+                }
+            }
+            if (DH) {
+                const double n[3] = {n0, n1, n2};
+                for (int p = 0; p < 3; p++)
+                    for (int b = 0; b < 3; b++) {
+                        cd v = m4[sidx(p, b)];
+                        fma_cr(v, v1[p], n[b]);
+                        fma_cr(v, v2[b], n[p]);
+                        if (p == b) v += s2;
+                        ph[p * 3 + b] += v;
+                    }
+            }
+        }
+        const double wk[3] = {wa * Qa.x, wa * Qa.y, wa * Qa.z};
+        for (int k = 0; k < 3; k++) {
+            if (DG) {
+                fma_cr(gb[k], pb, wk[k]);
+                for (int e = 0; e < 6; e++) fma_cr(ga[k][e], pa[e], wk[k]);
+            }
+            if (DH)
+                for (int e = 0; e < 9; e++) fma_cr(hh[k][e], ph[e], wk[k]);
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        const long long base = (long long)l * lstride + (long long)(3 * tv[k]) * ld + 3 * (long long)j;
+        for (int p = 0; p < 3; p++)
+            for (int b = 0; b < 3; b++) {
+                if (DG) {
+                    cd v = ga[k][sidx(p, b)];
+                    if (p == b) v += gb[k];
+                    UG[base + p * ld + b] += v;
+                }
+                if (DH) UH[base + p * ld + b] += hh[k][p * 3 + b] * kInv4Pi;
+            }
+    }
+}
+
+__device__ inline double block_sum256(double v, double *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    const double s = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Regular B_trans / B_angular: grid (nl, faces in colour), 256 lanes.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_gal_b(Geom g, const double4 *__restrict__ quad, int npts,
+                                                const Freq *__restrict__ freqs, Mat m, const int *__restrict__ faces,
+                                                cd *__restrict__ Bt, cd *__restrict__ Ba, long long nv3) {
+    __shared__ double red[4];
+    __shared__ double corner[3][24][2];
+    const int l = blockIdx.x;
+    const int f1 = faces[blockIdx.y];
+    const Freq f = freqs[l];
+    const int tv[3] = {g.F[3 * f1], g.F[3 * f1 + 1], g.F[3 * f1 + 2]};
+    const double *X0 = g.V + 3 * tv[0], *X1 = g.V + 3 * tv[1], *X2 = g.V + 3 * tv[2];
+    const double tarea = g.area[f1];
+    if (threadIdx.x < 3 * 24 * 2) (&corner[0][0][0])[threadIdx.x] = 0.0;
+    __syncthreads();
+    for (int a = 0; a < npts; ++a) {
+        const double4 Qa = quad[a];
+        const double x0 = Qa.x * X0[0] + Qa.y * X1[0] + Qa.z * X2[0];
+        const double x1 = Qa.x * X0[1] + Qa.y * X1[1] + Qa.z * X2[1];
+        const double x2 = Qa.x * X0[2] + Qa.y * X1[2] + Qa.z * X2[2];
+        cd bt[6], ba[18];
+        for (int k = 0; k < 6; k++) bt[k] = cd{0, 0};
+        for (int k = 0; k < 18; k++) ba[k] = cd{0, 0};
+        for (int fc = threadIdx.x; fc < g.nf; fc += 256) {
+            if (faces_touch(g, tv, fc)) continue;
+            const int a0 = g.F[3 * fc], a1 = g.F[3 * fc + 1], a2 = g.F[3 * fc + 2];
+            const double *p0 = g.V + 3 * a0, *p1 = g.V + 3 * a1, *p2 = g.V + 3 * a2;
+            const double n[3] = {g.N[3 * fc], g.N[3 * fc + 1], g.N[3 * fc + 2]};
+            const double ar = g.area[fc];
+            cd mt[3] = {{0, 0}, {0, 0}, {0, 0}}, m2 = {0, 0}, M[6], S = {0, 0}, S3 = {0, 0};
+            for (int k = 0; k < 6; k++) M[k] = cd{0, 0};
+            for (int q = 0; q < npts; ++q) {
+                const double4 Q = quad[q];
+                const double r0 = Q.x * p0[0] + Q.y * p1[0] + Q.z * p2[0] - x0;
+                const double r1 = Q.x * p0[1] + Q.y * p1[1] + Q.z * p2[1] - x1;
+                const double r2 = Q.x * p0[2] + Q.y * p1[2] + Q.z * p2[2] - x2;
+                const double r = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+                const double ir = 1.0 / r, ir2 = ir * ir;
+                const double h[3] = {r0 * ir, r1 * ir, r2 * ir};
+                const double w = Q.w * ar;
+                const Expo E = expo(r, f);
+                cd t1, t2, u1, u12, u3;
+                b_coef(E, ir, ir2, f, m, t1, t2, u1, u12, u3);
+                const double rn = h[0] * n[0] + h[1] * n[1] + h[2] * n[2];
+                t1 = t1 * w;
+                u1 = u1 * w;
+                for (int p = 0; p < 3; p++) fma_cr(mt[p], t1, h[p]);
+                fma_cr(m2, t2, w * rn);
+                for (int p = 0; p < 3; p++)
+                    for (int b = p; b < 3; b++) fma_cr(M[sidx(p, b)], u1, h[p] * h[b]);
+                fma_cr(S, u12, w);
+                fma_cr(S3, u3, w);
+            }
+            for (int p = 0; p < 3; p++)
+                for (int b = p; b < 3; b++) {
+                    cd v = {0, 0};
+                    fma_cr(v, mt[p], n[b]);
+                    fma_cr(v, mt[b], n[p]);
+                    if (p == b) v += m2;
+                    bt[sidx(p, b)] += v;
+                    for (int k = 0; k < 3; k++) {
+                        cd u = {0, 0};
+                        fma_cr(u, M[sidx(p, k)], n[b]);
+                        fma_cr(u, M[sidx(b, k)], n[p]);
+                        fma_cr(u, S, (b == k ? n[p] : 0.0) + (p == k ? n[b] : 0.0));
+                        if (p == b) fma_cr(u, S3, n[k]);
+                        ba[sidx(p, b) * 3 + k] += u;
+                    }
+                }
+        }
+        const double wa = Qa.w * tarea;
+        const double wk[3] = {wa * Qa.x, wa * Qa.y, wa * Qa.z};
+#pragma unroll
+        for (int e = 0; e < 24; e++) {
+            const cd v = e < 6 ? bt[e] : ba[e - 6];
+            const double sx = block_sum256(v.x, red), sy = block_sum256(v.y, red);
+            if (threadIdx.x == 0)
+                for (int k = 0; k < 3; k++) corner[k][e][0] += wk[k] * sx, corner[k][e][1] += wk[k] * sy;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int k = threadIdx.x;
+        cd *bt_out = Bt + (long long)l * nv3 * 3 + (long long)(3 * tv[k]) * 3;
+        cd *ba_out = Ba + (long long)l * nv3 * 9 + (long long)(3 * tv[k]) * 9;
+        for (int p = 0; p < 3; p++)
+            for (int b = 0; b < 3; b++) {
+                const int e = sidx(p, b);
+                bt_out[p * 3 + b] += cd{corner[k][e][0], corner[k][e][1]};
+                for (int q = 0; q < 3; q++) ba_out[p * 9 + b * 3 + q] += cd{corner[k][6 + e * 3 + q][0], corner[k][6 + e * 3 + q][1]};
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Touching pairs, Sauter-Schwab.  Pair record: kind (1 coincident, 2 edge, 3 vertex),
+// f2 (source normal), j1[3], j2[3] (re-ordered corner vertices, integrator_galerkin.hpp:262-334).
+// ---------------------------------------------------------------------------
+struct Pair {
+    int kind, f2;
+    int j1[3], j2[3];
+};
+
+// Output per pair and frequency (complex): G 81 (row 3i+p, col 3j+q), H 81, Bt 27 (row 3i+p, col q), Ba 81 (row 3i+p, col q*3+k)
+constexpr int kPairG = 0, kPairH = 81, kPairBt = 162, kPairBa = 189, kPairStride = 270;
+
+// Point-pair kernel accumulation for MODE 0 (G), 1 (H), 2 (B).
+template <int MODE> struct SingAcc {
+    cd v[MODE == 0 ? 63 : (MODE == 1 ? 81 : 36)];
+};
+
+template <int MODE>
+__device__ __forceinline__ void sing_eval(const double x[3], const double y[3], const double Lx[3], const double Ly[3], double wt,
+                                 const double n[3], const Freq &f, const Mat &m, cd *acc) {
+    const double r0 = y[0] - x[0], r1 = y[1] - x[1], r2 = y[2] - x[2];
+    const double r = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+    const double ir = 1.0 / r, ir2 = ir * ir;
+    const double h[3] = {r0 * ir, r1 * ir, r2 * ir};
+    const Expo E = expo(r, f);
+    if (MODE == 0) {
+        cd al, be;
+        g_coef(E, ir, ir2, m, al, be);
+        al = al * ir;
+        be = be * ir;
+        const double hh[6] = {h[0] * h[0], h[1] * h[1], h[2] * h[2], h[0] * h[1], h[0] * h[2], h[1] * h[2]};
+        #pragma unroll
+        for (int i = 0; i < 3; i++)
+            #pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const double w = wt * Lx[i] * Ly[j];
+                const cd aw = al * w;
+                cd *a = acc + (i * 3 + j) * 7;
+                #pragma unroll
+                for (int e = 0; e < 6; e++) fma_cr(a[e], aw, hh[e]);
+                fma_cr(a[6], be, w);
+            }
+    } else if (MODE == 1) {
+        cd c1, c2, c4;
+        h_coef(E, ir, ir2, f, m, c1, c2, c4);
+        const double rn = h[0] * n[0] + h[1] * n[1] + h[2] * n[2];
+        cd pv[9];
+        #pragma unroll
+        for (int p = 0; p < 3; p++)
+            #pragma unroll
+            for (int b = 0; b < 3; b++) {
+                cd v = c1 * (h[p] * n[b]);
+                fma_cr(v, c2, n[p] * h[b] + (p == b ? rn : 0.0));
+                fma_cr(v, c4, rn * h[p] * h[b]);
+                pv[p * 3 + b] = v;
+            }
+        #pragma unroll
+        for (int i = 0; i < 3; i++)
+            #pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const double w = wt * Lx[i] * Ly[j];
+                cd *a = acc + (i * 3 + j) * 9;
+                #pragma unroll
+                for (int e = 0; e < 9; e++) fma_cr(a[e], pv[e], w);
+            }
+    } else {
+        cd t1, t2, u1, u12, u3;
+        b_coef(E, ir, ir2, f, m, t1, t2, u1, u12, u3);
+        const double rn = h[0] * n[0] + h[1] * n[1] + h[2] * n[2];
+        #pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const double w = wt * Lx[i];
+            cd *a = acc + i * 12;
+            const cd tw = t1 * w, uw = u1 * w;
+            #pragma unroll
+            for (int p = 0; p < 3; p++) fma_cr(a[p], tw, h[p]);
+            fma_cr(a[3], t2, w * rn);
+            #pragma unroll
+            for (int p = 0; p < 3; p++)
+                #pragma unroll
+                for (int b = p; b < 3; b++) fma_cr(a[4 + sidx(p, b)], uw, h[p] * h[b]);
+            fma_cr(a[10], u12, w);
+            fma_cr(a[11], u3, w);
+        }
+    }
+}
+
+template <int MODE, int KIND>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) k_gal_sing(Geom g, const Pair *__restrict__ pairs, int pair0,
+                                                  const double2 *__restrict__ line, int nline,
+                                                  const Freq *__restrict__ freqs, Mat m, cd *__restrict__ out) {
+    constexpr int NA = MODE == 0 ? 63 : (MODE == 1 ? 81 : 36);
+    const int pi = pair0 + blockIdx.x, l = blockIdx.y;
+    const Pair P = pairs[pi];
+    const Freq f = freqs[l];
+    const double n[3] = {g.N[3 * P.f2], g.N[3 * P.f2 + 1], g.N[3 * P.f2 + 2]};
+    // x(xh) = px + dx0 xh0 + dx1 xh1 and y likewise; barycentrics (1-h0, h0-h1, h1)
+    double px[3], dx0[3], dx1[3], dy0[3], dy1[3], jac;
+    {
+        const double *A = g.V + 3 * P.j1[0], *B = g.V + 3 * P.j1[1], *Cc = g.V + 3 * P.j1[2];
+        #pragma unroll
+        for (int k = 0; k < 3; k++) px[k] = A[k];
+        if (KIND == 1) {
+            #pragma unroll
+            for (int k = 0; k < 3; k++) dx0[k] = dy0[k] = B[k] - A[k], dx1[k] = dy1[k] = Cc[k] - B[k];
+        } else if (KIND == 2) {
+            const double *Y2 = g.V + 3 * P.j2[2];
+            #pragma unroll
+            for (int k = 0; k < 3; k++) dx0[k] = dy0[k] = B[k] - A[k], dx1[k] = Cc[k] - B[k], dy1[k] = Y2[k] - B[k];
+        } else {
+            const double *Y1 = g.V + 3 * P.j2[1], *Y2 = g.V + 3 * P.j2[2];
+            #pragma unroll
+            for (int k = 0; k < 3; k++) dx0[k] = B[k] - A[k], dx1[k] = Cc[k] - B[k], dy0[k] = Y1[k] - A[k], dy1[k] = Y2[k] - Y1[k];
+        }
+        const double cx[3] = {dx0[1] * dx1[2] - dx0[2] * dx1[1], dx0[2] * dx1[0] - dx0[0] * dx1[2], dx0[0] * dx1[1] - dx0[1] * dx1[0]};
+        const double cy[3] = {dy0[1] * dy1[2] - dy0[2] * dy1[1], dy0[2] * dy1[0] - dy0[0] * dy1[2], dy0[0] * dy1[1] - dy0[1] * dy1[0]};
+        jac = sqrt(cx[0] * cx[0] + cx[1] * cx[1] + cx[2] * cx[2]) * sqrt(cy[0] * cy[0] + cy[1] * cy[1] + cy[2] * cy[2]);
+    }
+    cd acc[NA];
+    #pragma unroll
+    for (int k = 0; k < NA; k++) acc[k] = cd{0, 0};
+    const int n4 = nline * nline * nline * nline;
+    for (int p = threadIdx.x; p < n4; p += 64) {
+        const int ia = p / (nline * nline * nline), ib = (p / (nline * nline)) % nline, ic = (p / nline) % nline, id = p % nline;
+        const double e1 = line[ia].x, e2 = line[ib].x, e3 = line[ic].x, xi = line[id].x;
+        const double w = line[ia].y * line[ib].y * line[ic].y * line[id].y;
+        // region points (x^, y^) and weights; evaluated in one loop so the accumulators stay in registers
+        constexpr int NR = KIND == 1 ? 6 : (KIND == 2 ? 5 : 2);
+        double ra0[NR], ra1[NR], rb0[NR], rb1[NR], rw[NR];
+        int nr_ = 0;
+        auto ev = [&](double a0, double a1, double b0, double b1, double wt) {
+            ra0[nr_] = a0, ra1[nr_] = a1, rb0[nr_] = b0, rb1[nr_] = b1, rw[nr_] = wt, nr_++;
+        };
+        if (KIND == 1) {
+            const double W = w * xi * xi * xi * e1 * e1 * e2;
+            ev(xi, xi * (1.0 - e1 + e1 * e2), xi * (1.0 - e1 * e2 * e3), xi * (1.0 - e1), W);
+            ev(xi * (1.0 - e1 * e2 * e3), xi * (1.0 - e1), xi, xi * (1.0 - e1 + e1 * e2), W);
+            ev(xi, xi * e1 * (1.0 - e2 + e2 * e3), xi * (1.0 - e1 * e2), xi * e1 * (1.0 - e2), W);
+            ev(xi * (1.0 - e1 * e2), xi * e1 * (1.0 - e2), xi, xi * e1 * (1.0 - e2 + e2 * e3), W);
+            ev(xi * (1.0 - e1 * e2 * e3), xi * e1 * (1.0 - e2 * e3), xi, xi * e1 * (1.0 - e2), W);
+            ev(xi, xi * e1 * (1.0 - e2), xi * (1.0 - e1 * e2 * e3), xi * e1 * (1.0 - e2 * e3), W);
+        } else if (KIND == 2) {
+            const double W = w * xi * xi * xi * e1 * e1;
+            ev(xi, xi * e1 * e3, xi * (1.0 - e1 * e2), xi * e1 * (1.0 - e2), W);
+            ev(xi, xi * e1, xi * (1.0 - e1 * e2 * e3), xi * e1 * e2 * (1.0 - e3), W * e2);
+            ev(xi * (1.0 - e1 * e2), xi * e1 * (1.0 - e2), xi, xi * e1 * e2 * e3, W * e2);
+            ev(xi * (1.0 - e1 * e2 * e3), xi * e1 * e2 * (1.0 - e3), xi, xi * e1, W * e2);
+            ev(xi * (1.0 - e1 * e2 * e3), xi * e1 * (1.0 - e2 * e3), xi, xi * e1 * e2, W * e2);
+        } else {
+            const double W = w * xi * xi * xi * e2;
+            ev(xi, xi * e1, xi * e2, xi * e2 * e3, W);
+            ev(xi * e2, xi * e2 * e3, xi, xi * e1, W);
+        }
+#pragma unroll 1
+        for (int r = 0; r < NR; r++) {
+            double x[3], y[3];
+            #pragma unroll
+            for (int k = 0; k < 3; k++)
+                x[k] = px[k] + dx0[k] * ra0[r] + dx1[k] * ra1[r], y[k] = px[k] + dy0[k] * rb0[r] + dy1[k] * rb1[r];
+            const double Lx[3] = {1.0 - ra0[r], ra0[r] - ra1[r], ra1[r]}, Ly[3] = {1.0 - rb0[r], rb0[r] - rb1[r], rb1[r]};
+            sing_eval<MODE>(x, y, Lx, Ly, rw[r], n, f, m, acc);
+        }
+    }
+    // wave reduction; lane 0 expands and writes
+    #pragma unroll
+    for (int k = 0; k < NA; k++) {
+        double vx = acc[k].x, vy = acc[k].y;
+        #pragma unroll
+        for (int o = 32; o > 0; o >>= 1) vx += __shfl_xor(vx, o, 64), vy += __shfl_xor(vy, o, 64);
+        acc[k] = cd{vx * jac, vy * jac};
+    }
+    if (threadIdx.x != 0) return;
+    cd *o = out + ((long long)pi * gridDim.y + l) * kPairStride;
+    if (MODE == 0) {
+        #pragma unroll
+        for (int i = 0; i < 3; i++)
+            #pragma unroll
+            for (int j = 0; j < 3; j++)
+                #pragma unroll
+                for (int p = 0; p < 3; p++)
+                    #pragma unroll
+                    for (int b = 0; b < 3; b++) {
+                        cd v = acc[(i * 3 + j) * 7 + sidx(p, b)];
+                        if (p == b) v += acc[(i * 3 + j) * 7 + 6];
+                        o[kPairG + (3 * i + p) * 9 + 3 * j + b] = v;
+                    }
+    } else if (MODE == 1) {
+        #pragma unroll
+        for (int i = 0; i < 3; i++)
+            #pragma unroll
+            for (int j = 0; j < 3; j++)
+                #pragma unroll
+                for (int p = 0; p < 3; p++)
+                    #pragma unroll
+                    for (int b = 0; b < 3; b++) o[kPairH + (3 * i + p) * 9 + 3 * j + b] = acc[(i * 3 + j) * 9 + p * 3 + b] * kInv4Pi;
+    } else {
+        #pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const cd *a = acc + i * 12;
+            #pragma unroll
+            for (int p = 0; p < 3; p++)
+                #pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    cd v = {0, 0};
+                    fma_cr(v, a[p], n[b]);
+                    fma_cr(v, a[b], n[p]);
+                    if (p == b) v += a[3];
+                    o[kPairBt + (3 * i + p) * 3 + b] = v;
+                    #pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        cd u = {0, 0};
+                        fma_cr(u, a[4 + sidx(p, k)], n[b]);
+                        fma_cr(u, a[4 + sidx(b, k)], n[p]);
+                        fma_cr(u, a[10], (b == k ? n[p] : 0.0) + (p == k ? n[b] : 0.0));
+                        if (p == b) fma_cr(u, a[11], n[k]);
+                        o[kPairBa + (3 * i + p) * 9 + b * 3 + k] = u;
+                    }
+                }
+        }
+    }
+}
+
+// Gather touching-pair blocks into the spectra.  One lane per (output block, l);
+// items[ptr[b]..ptr[b+1]) = pair*9 + i*3 + j in a fixed order.
+__global__ void k_gal_gather_gh(const int2 *__restrict__ blocks, const int *__restrict__ ptr, const int *__restrict__ items,
+                                int nblocks, int nl, const cd *__restrict__ scratch, cd *__restrict__ UG, cd *__restrict__ UH,
+                                long long ld, long long lstride) {
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long long)nblocks * nl) return;
+    const int b = (int)(t / nl), l = (int)(t % nl);
+    const int2 uv = blocks[b];
+    cd g[9], h[9];
+    for (int e = 0; e < 9; e++) g[e] = h[e] = cd{0, 0};
+    for (int k = ptr[b]; k < ptr[b + 1]; k++) {
+        const int it = items[k];
+        const int pi = it / 9, i = (it % 9) / 3, j = it % 3;
+        const cd *o = scratch + ((long long)pi * nl + l) * kPairStride;
+        for (int p = 0; p < 3; p++)
+            for (int q = 0; q < 3; q++) {
+                g[p * 3 + q] += o[kPairG + (3 * i + p) * 9 + 3 * j + q];
+                h[p * 3 + q] += o[kPairH + (3 * i + p) * 9 + 3 * j + q];
+            }
+    }
+    const long long base = (long long)l * lstride + (long long)(3 * uv.x) * ld + 3LL * uv.y;
+    for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++) {
+            if (UG) UG[base + p * ld + q] += g[p * 3 + q];
+            if (UH) UH[base + p * ld + q] += h[p * 3 + q];
+        }
+}
+
+__global__ void k_gal_gather_b(const int *__restrict__ vptr, const int *__restrict__ items, int nv, int nl,
+                               const cd *__restrict__ scratch, cd *__restrict__ Bt, cd *__restrict__ Ba, long long nv3) {
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long long)nv * nl) return;
+    const int u = (int)(t / nl), l = (int)(t % nl);
+    cd bt[9], ba[27];
+    for (int e = 0; e < 9; e++) bt[e] = cd{0, 0};
+    for (int e = 0; e < 27; e++) ba[e] = cd{0, 0};
+    for (int k = vptr[u]; k < vptr[u + 1]; k++) {
+        const int it = items[k];
+        const int pi = it / 3, i = it % 3;
+        const cd *o = scratch + ((long long)pi * nl + l) * kPairStride;
+        for (int p = 0; p < 3; p++) {
+            for (int q = 0; q < 3; q++) bt[p * 3 + q] += o[kPairBt + (3 * i + p) * 3 + q];
+            for (int q = 0; q < 9; q++) ba[p * 9 + q] += o[kPairBa + (3 * i + p) * 9 + q];
+        }
+    }
+    cd *bo = Bt + (long long)l * nv3 * 3 + (long long)(3 * u) * 3;
+    cd *ao = Ba + (long long)l * nv3 * 9 + (long long)(3 * u) * 9;
+    for (int e = 0; e < 9; e++) bo[e] += bt[e];
+    for (int e = 0; e < 27; e++) ao[e] += ba[e];
+}
+
+__global__ void k_fill_zero(double2 *p, long long n) {
+    const long long stride = (long long)gridDim.x * 256;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) p[i] = make_double2(0.0, 0.0);
+}
+
+void check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct DevVec {
+    void *p = nullptr;
+    ~DevVec() {
+        if (p) (void)hipFree(p);
+    }
+    template <class T> T *up(const std::vector<T> &v, hipStream_t st) {
+        check(hipMalloc(&p, std::max<size_t>(v.size(), 1) * sizeof(T)), "hipMalloc");
+        if (!v.empty()) check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+        return (T *)p;
+    }
+};
+
+void zero(cd *p, long long n, int num_cus, hipStream_t st) {
+    if (!p || n <= 0) return;
+    long long blocks = std::min<long long>((n + 255) / 256, (long long)num_cus * 8);
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)blocks), dim3(256), 0, st, (double2 *)p, n);
+    check(hipGetLastError(), "k_fill_zero");
+}
+
+struct EvTimer {
+    hipEvent_t a, b;
+    hipStream_t st;
+    double *acc;
+    EvTimer(hipStream_t s, double *d) : st(s), acc(d) {
+        (void)hipEventCreate(&a), (void)hipEventCreate(&b), (void)hipEventRecord(a, st);
+    }
+    ~EvTimer() {
+        float ms = 0;
+        if (hipEventRecord(b, st) == hipSuccess && hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess)
+            *acc += ms;
+        (void)hipEventDestroy(a), (void)hipEventDestroy(b);
+    }
+};
+
+}  // namespace
+
+void run_galerkin(const GalerkinJob &job, const std::vector<int> &vs_ptr, const std::vector<int> &vs_face, const int *F,
+                  cd *U, cd *P, cd *Bt, cd *Ba, double *tm, hipStream_t st) {
+    const Geom &g = job.g;
+    const int nv = g.nv, nf = g.nf, nl = job.nl;
+    const long long n3 = 3LL * nv;
+    const int *vm = nullptr;  // host copy of the vertex map is not needed: touching tests run on device,
+                              // and the host pair list below uses exact vertex identity plus the map if given
+    (void)vm;
+    zero(U, U ? (long long)nl * n3 * n3 : 0, job.num_cus, st);
+    zero(P, P ? (long long)nl * n3 * n3 : 0, job.num_cus, st);
+    zero(Bt, Bt ? (long long)nl * n3 * 3 : 0, job.num_cus, st);
+    zero(Ba, Ba ? (long long)nl * n3 * 9 : 0, job.num_cus, st);
+
+    // Greedy colouring of test faces: faces sharing a vertex get different colours.
+    std::vector<int> color(nf, -1), vmark;
+    std::vector<std::vector<int>> classes;
+    {
+        std::vector<std::vector<char>> used;  // used[c][v]
+        for (int f = 0; f < nf; f++) {
+            int c = 0;
+            for (;; c++) {
+                if (c == (int)used.size()) used.emplace_back(nv, 0), classes.emplace_back();
+                if (!used[c][F[3 * f]] && !used[c][F[3 * f + 1]] && !used[c][F[3 * f + 2]]) break;
+            }
+            color[f] = c;
+            for (int k = 0; k < 3; k++) used[c][F[3 * f + k]] = 1;
+            classes[c].push_back(f);
+        }
+    }
+    std::vector<int> flat, cls_off{0};
+    for (auto &c : classes) flat.insert(flat.end(), c.begin(), c.end()), cls_off.push_back((int)flat.size());
+    DevVec dfaces;
+    const int *faces = dfaces.up(flat, st);
+
+    if (U || P || Bt || Ba) {
+        EvTimer t(st, &tm[1]);
+        for (size_t c = 0; c < classes.size(); c++) {
+            const int cnt = cls_off[c + 1] - cls_off[c];
+            const int *fc = faces + cls_off[c];
+            if (U || P) {
+                dim3 grid((nv + 63) / 64, nl, cnt);
+                if (U) hipLaunchKernelGGL((k_gal_gh<true, false>), grid, dim3(64), 0, st, g, job.quad, job.npts, job.freqs, job.mat, fc, U, (cd *)nullptr, n3, n3 * n3);
+                if (P) hipLaunchKernelGGL((k_gal_gh<false, true>), grid, dim3(64), 0, st, g, job.quad, job.npts, job.freqs, job.mat, fc, (cd *)nullptr, P, n3, n3 * n3);
+                check(hipGetLastError(), "k_gal_gh");
+            }
+            if (Bt && Ba) {
+                hipLaunchKernelGGL(k_gal_b, dim3(nl, cnt), dim3(256), 0, st, g, job.quad, job.npts, job.freqs, job.mat, fc, Bt, Ba, n3);
+                check(hipGetLastError(), "k_gal_b");
+            }
+        }
+    }
+
+    // Touching pairs, classified exactly as integrate_galerkin_singular (integrator_galerkin.hpp:262-334).
+    std::vector<Pair> pairs;
+    {
+        std::vector<std::vector<int>> vfaces(nv);
+        for (int v = 0; v < nv; v++)
+            for (int t = vs_ptr[v]; t < vs_ptr[v + 1]; t++) vfaces[v].push_back(vs_face[t] >> 2);
+        for (int f1 = 0; f1 < nf; f1++) {
+            std::vector<int> cand;
+            for (int k = 0; k < 3; k++) cand.insert(cand.end(), vfaces[F[3 * f1 + k]].begin(), vfaces[F[3 * f1 + k]].end());
+            std::sort(cand.begin(), cand.end());
+            cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+            for (int f2 : cand) {
+                Pair p{};
+                p.f2 = f2;
+                const int *A = F + 3 * f1, *B = F + 3 * f2;
+                if (f1 == f2) {
+                    p.kind = 1;
+                    for (int k = 0; k < 3; k++) p.j1[k] = p.j2[k] = A[k];
+                } else {
+                    for (int i = 0; i < 3; i++)
+                        for (int j = 0; j < 3; j++)
+                            if (A[i] == B[j] && A[(i + 1) % 3] == B[(j + 2) % 3]) {
+                                p.kind = 2;
+                                p.j1[0] = A[i], p.j1[1] = A[(i + 1) % 3], p.j1[2] = A[(i + 2) % 3];
+                                p.j2[0] = B[j], p.j2[1] = B[(j + 2) % 3], p.j2[2] = B[(j + 1) % 3];
+                                break;
+                            }
+                    if (!p.kind)
+                        for (int i = 0; i < 3; i++)
+                            for (int j = 0; j < 3; j++)
+                                if (A[i] == B[j]) {
+                                    p.kind = 3;
+                                    p.j1[0] = A[i], p.j1[1] = A[(i + 1) % 3], p.j1[2] = A[(i + 2) % 3];
+                                    p.j2[0] = B[j], p.j2[1] = B[(j + 1) % 3], p.j2[2] = B[(j + 2) % 3];
+                                    break;
+                                }
+                }
+                if (p.kind) pairs.push_back(p);
+            }
+        }
+    }
+    if (pairs.empty()) return;
+    std::stable_sort(pairs.begin(), pairs.end(), [](const Pair &a, const Pair &b) { return a.kind < b.kind; });
+    int kind_off[4] = {0, 0, 0, 0};
+    for (const Pair &p : pairs) kind_off[p.kind]++;
+    kind_off[3] += kind_off[2] += kind_off[1];  // exclusive ends per kind: [0,k1) coincident, [k1,k2) edge, [k2,k3) vertex
+    const long long npairs = (long long)pairs.size();
+    DevVec dpairs, dscratch, dblocks, dptr, ditems, dvptr, dvitems;
+    const Pair *dp = dpairs.up(pairs, st);
+    check(hipMalloc(&dscratch.p, (size_t)npairs * nl * kPairStride * sizeof(cd)), "hipMalloc scratch");
+    cd *scratch = (cd *)dscratch.p;
+    {
+        EvTimer t(st, &tm[2]);
+        const int beg[3] = {0, kind_off[1], kind_off[2]}, end[3] = {kind_off[1], kind_off[2], kind_off[3]};
+#define CQBEM_SING(MODE, KIND)                                                                                        \
+    if (end[KIND - 1] > beg[KIND - 1])                                                                                \
+        hipLaunchKernelGGL((k_gal_sing<MODE, KIND>), dim3((unsigned)(end[KIND - 1] - beg[KIND - 1]), nl), dim3(64), 0, st, g, \
+                           dp, beg[KIND - 1], job.line, job.nline, job.freqs, job.mat, scratch);
+        if (U) { CQBEM_SING(0, 1) CQBEM_SING(0, 2) CQBEM_SING(0, 3) }
+        if (P) { CQBEM_SING(1, 1) CQBEM_SING(1, 2) CQBEM_SING(1, 3) }
+        if (Bt && Ba) { CQBEM_SING(2, 1) CQBEM_SING(2, 2) CQBEM_SING(2, 3) }
+#undef CQBEM_SING
+        check(hipGetLastError(), "k_gal_sing");
+    }
+    // gather lists: output block (u, v) <- (pair, i, j); vertex u <- (pair, i)
+    std::map<std::pair<int, int>, std::vector<int>> bl;
+    std::vector<std::vector<int>> vl(nv);
+    for (long long k = 0; k < npairs; k++) {
+        for (int i = 0; i < 3; i++) {
+            vl[pairs[k].j1[i]].push_back((int)(k * 3 + i));
+            for (int j = 0; j < 3; j++) bl[{pairs[k].j1[i], pairs[k].j2[j]}].push_back((int)(k * 9 + i * 3 + j));
+        }
+    }
+    std::vector<int2> blocks;
+    std::vector<int> ptr{0}, items, vptr{0}, vitems;
+    for (auto &kv : bl) {
+        blocks.push_back(make_int2(kv.first.first, kv.first.second));
+        items.insert(items.end(), kv.second.begin(), kv.second.end());
+        ptr.push_back((int)items.size());
+    }
+    for (int v = 0; v < nv; v++) vitems.insert(vitems.end(), vl[v].begin(), vl[v].end()), vptr.push_back((int)vitems.size());
+    const int2 *db = dblocks.up(blocks, st);
+    const int *dpt = dptr.up(ptr, st), *dit = ditems.up(items, st), *dvp = dvptr.up(vptr, st), *dvi = dvitems.up(vitems, st);
+    {
+        EvTimer t(st, &tm[2]);
+        if (U || P) {
+            const long long n = (long long)blocks.size() * nl;
+            hipLaunchKernelGGL(k_gal_gather_gh, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, db, dpt, dit, (int)blocks.size(), nl,
+                               scratch, U, P, n3, n3 * n3);
+            check(hipGetLastError(), "k_gal_gather_gh");
+        }
+        if (Bt && Ba) {
+            const long long n = (long long)nv * nl;
+            hipLaunchKernelGGL(k_gal_gather_b, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dvp, dvi, nv, nl, scratch, Bt, Ba, n3);
+            check(hipGetLastError(), "k_gal_gather_b");
+        }
+    }
+    check(hipStreamSynchronize(st), "galerkin sync");
+}
+
 }  // namespace cqbem

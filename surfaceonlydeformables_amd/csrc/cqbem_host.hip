@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <stdexcept>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -85,6 +86,8 @@ void check_inputs(const cqbem_mesh *mesh, const cqbem_params *p) {
     if ((p->which & ~CQBEM_ALL) != 0) throw Err{CQBEM_ERR_ARG, "unknown matrix bits in `which`"};
     if (!p->galerkin && (p->which & CQBEM_H) && mesh->vertex_map_inverse)
         throw Err{CQBEM_ERR_UNSUPPORTED, "collocation C matrix of a vertex-split mesh needs the original mesh"};
+    if (p->galerkin && mesh->vertex_map_inverse)
+        throw Err{CQBEM_ERR_UNSUPPORTED, "Galerkin assembly of a vertex-split mesh is not supported"};
 }
 
 }  // namespace
@@ -112,7 +115,7 @@ struct cqbem_context {
     int num_cus = 256;
     hipStream_t stream = nullptr;
     std::string last_error;
-    double timings[5] = {0, 0, 0, 0, 0};
+    double timings[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     std::mutex mu;
     // serving
     std::atomic<uint64_t> next_job{1};
@@ -287,7 +290,7 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
     const bool wantBt = (p->which & CQBEM_B_TRANS) && out->B_trans;
     const bool wantBa = (p->which & CQBEM_B_ANGULAR) && (out->B_angular || out->B);
     const bool wantB = wantBt || wantBa || out->B;
-    tm[4] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
+    tm[5] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
 
     // outputs: caller device buffers, or staged device buffers copied back at the end
     DBuf sG, sH, sBt, sBa, sB6;
@@ -336,8 +339,10 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
                     HIPCHK(launch_colloc_gh(g, quad, npts, freqs, 0, Lf, P.mat, r0, nr, specG.as<cd>(), specH.as<cd>(), n3,
                                             lstride, st));
                 }
+                tm[6] += 1;
+                for (int i = r0; i < r0 + nr; i++) tm[7] += (double)(P.nf - (P.sg_ptr[i + 1] - P.sg_ptr[i])) * npts * Lf;
                 {
-                    Timer t(st, &tm[2]);
+                    Timer t(st, &tm[3]);
                     if (wantG)
                         HIPCHK(launch_colloc_singular(0, g, line, nline, freqs, 0, Lf, P.mat, r0, nr, specG.as<cd>(), n3,
                                                       lstride, nullptr, nullptr, st));
@@ -346,7 +351,7 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
                                                       lstride, nullptr, nullptr, st));
                 }
                 {
-                    Timer t(st, &tm[3]);
+                    Timer t(st, &tm[4]);
                     const long long E = 3LL * nr * n3;
                     if (wantG) HIPCHK(launch_transform(specG.as<cd>(), E, Lf, tab, L, G + 3LL * r0 * n3, n3 * n3, ctx->num_cus, st));
                     if (wantH) HIPCHK(launch_transform(specH.as<cd>(), E, Lf, tab, L, H + 3LL * r0 * n3, n3 * n3, ctx->num_cus, st));
@@ -355,11 +360,11 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
         }
         if (wantB) {
             {
-                Timer t(st, &tm[1]);
+                Timer t(st, &tm[2]);
                 HIPCHK(launch_colloc_b(g, quad, npts, freqs, 0, Lf, P.mat, 0, nv, specB.as<cd>(), specBa.as<cd>(), st));
             }
             {
-                Timer t(st, &tm[2]);
+                Timer t(st, &tm[3]);
                 HIPCHK(launch_colloc_singular(1, g, line, nline, freqs, 0, Lf, P.mat, 0, nv, nullptr, n3, 0, specB.as<cd>(),
                                               specBa.as<cd>(), st));
             }
@@ -380,7 +385,7 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
             if (wantH) specH.alloc((size_t)Lf * lstride * sizeof(cd));
             run_galerkin(gj, P.vs_ptr, P.vs_face, mesh->F, specG.as<cd>(), specH.as<cd>(), wantB ? specB.as<cd>() : nullptr,
                          wantB ? specBa.as<cd>() : nullptr, tm, st);
-            Timer t(st, &tm[3]);
+            Timer t(st, &tm[4]);
             if (wantG) HIPCHK(launch_transform(specG.as<cd>(), n3 * n3, Lf, tab, L, G, n3 * n3, ctx->num_cus, st));
             if (wantH) HIPCHK(launch_transform(specH.as<cd>(), n3 * n3, Lf, tab, L, H, n3 * n3, ctx->num_cus, st));
         }
@@ -402,7 +407,7 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
         }
     }
     if (wantB) {
-        Timer t(st, &tm[3]);
+        Timer t(st, &tm[4]);
         HIPCHK(launch_transform(specB.as<cd>(), n3 * 3, Lf, tab, L, Wbt, n3 * 3, ctx->num_cus, st));
         HIPCHK(launch_transform(specBa.as<cd>(), n3 * 9, Lf, tab, L, Wba, n3 * 9, ctx->num_cus, st));
         HIPCHK(launch_b_post(Wbt, Wba, W6, g.V, nv, L, st));
@@ -415,7 +420,7 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
         if (wantB && out->B_angular) HIPCHK(hipMemcpyAsync(out->B_angular, Wba, (size_t)L * n3 * 9 * 8, hipMemcpyDeviceToHost, st));
         if (W6) HIPCHK(hipMemcpyAsync(out->B, W6, (size_t)L * n3 * 6 * 8, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        tm[4] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        tm[5] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     HIPCHK(hipStreamSynchronize(st));
 }
@@ -489,7 +494,7 @@ int cqbem_assemble(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_param
     if (!ctx) return CQBEM_ERR_ARG;
     std::lock_guard<std::mutex> lk(ctx->mu);
     const auto t0 = std::chrono::steady_clock::now();
-    double tm[5] = {0, 0, 0, 0, 0};
+    double tm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     try {
         check_inputs(mesh, params);
         if (!out) throw Err{CQBEM_ERR_ARG, "null outputs"};
@@ -504,16 +509,19 @@ int cqbem_assemble(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_param
         run_job(ctx, ctx->stream, mesh, params, out, slab, tm, true);
     } catch (const Err &e) {
         return fail(ctx, e);
+    } catch (const std::exception &e) {
+        return fail(ctx, Err{CQBEM_ERR_HIP, e.what()});
     }
     tm[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    std::copy(tm, tm + 5, ctx->timings);
+    std::copy(tm, tm + 8, ctx->timings);
     return CQBEM_OK;
 }
 
-int cqbem_last_timings(const cqbem_context *ctx, double *ms5) {
-    if (!ctx || !ms5) return CQBEM_ERR_ARG;
-    std::copy(ctx->timings, ctx->timings + 5, ms5);
-    return CQBEM_OK;
+int cqbem_last_timings(const cqbem_context *ctx, double *out, int32_t n) {
+    if (!ctx || !out || n < 0) return CQBEM_ERR_ARG;
+    const int k = std::min(n, 8);
+    std::copy(ctx->timings, ctx->timings + k, out);
+    return k;
 }
 
 int cqbem_laplace(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params *params, int32_t l, double *U, double *Pm,
@@ -553,7 +561,7 @@ int cqbem_laplace(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params
             }
         } else {
             GalerkinJob gj{d.g, quad, npts, line, nline, freqs + l, 1, P.mat, ctx->num_cus};
-            double tm[5] = {0, 0, 0, 0, 0};
+            double tm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             run_galerkin(gj, P.vs_ptr, P.vs_face, mesh->F, wG ? dU.as<cd>() : nullptr, wH ? dP.as<cd>() : nullptr,
                          wB ? dBt.as<cd>() : nullptr, wB ? dBa.as<cd>() : nullptr, tm, st);
         }
@@ -564,6 +572,8 @@ int cqbem_laplace(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params
         HIPCHK(hipStreamSynchronize(st));
     } catch (const Err &e) {
         return fail(ctx, e);
+    } catch (const std::exception &e) {
+        return fail(ctx, Err{CQBEM_ERR_HIP, e.what()});
     }
     return CQBEM_OK;
 }
@@ -616,12 +626,14 @@ int cqbem_submit(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params 
         JobRecord *jr = job.get();
         const int dev = ctx->device;
         job->worker = std::thread([ctx, jr, st, dev, slab]() {
-            double tm[5] = {0, 0, 0, 0, 0};
+            double tm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             try {
                 HIPCHK(hipSetDevice(dev));
                 run_job(ctx, st, &jr->mesh, &jr->params, &jr->out, slab, tm, false);
             } catch (const Err &e) {
                 jr->status = e.code, jr->error = e.msg;
+            } catch (const std::exception &e) {
+                jr->status = CQBEM_ERR_HIP, jr->error = e.what();
             }
             {
                 std::lock_guard<std::mutex> lk2(ctx->mu);

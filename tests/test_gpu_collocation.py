@@ -1,0 +1,175 @@
+"""GPU parity of the collocation assembly (HIP path through the C-ABI) against the CPU oracle.
+
+Tolerances.  The inverse transform multiplies weight n by R^-n (up to ~7e4 at L = 32), which
+amplifies per-frequency FP64 rounding; the oracle's own FP64 result differs from its long-double
+run by up to ~2e-10 (G) and ~6e-9 (H) in the last weights.  The GPU is required to be at least
+as close to the long-double result as the FP64 oracle is (times 2, floor 1e-12), per weight
+matrix, and within 1e-9 (G, B) / 1e-8 (H) relative Frobenius error over the whole stack.
+Laplace-domain matrices (no amplification) must agree to 1e-13 (U, B) and 1e-10 (P)."""
+import numpy as np
+import pytest
+
+from surfaceonlydeformables_amd import abi
+from surfaceonlydeformables_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+RHO = 1000.0
+
+
+def material(E, nu, rho=RHO):
+    _, c1, c2 = M.material_speeds(E, nu, rho)
+    return c1, c2
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = abi.Context(0)
+    yield c
+    c.close()
+
+
+def small_meshes():
+    V1, F1 = M.icosphere(1, 0.2)
+    V1 = M.center(M.jitter(V1, F1, 0.01, 0), F1)
+    V2, F2 = M.cube(2, 0.25)
+    return {"ico1": (V1, F1), "cube2": (M.center(V2, F2), F2)}
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("name", ["ico1", "cube2"])
+@pytest.mark.parametrize("l", [0, 5, 16])
+def test_laplace_parity(ctx, oracle, name, l):
+    V, F = small_meshes()[name]
+    c1, c2 = material(5e5, 0.3)
+    L, dt = 32, 1e-3
+    R = 1e-10 ** (1 / (2 * L))
+    print(f"laplace parity {name} l={l}", flush=True)
+    g = ctx.laplace(abi.make_mesh(V, F), abi.make_params(c1, c2, RHO, 0.3, dt, R, L), l)
+    s = oracle.frequencies(L, R, dt, 2)[l]
+    o = oracle.laplace(V, F, oracle.params(c1, c2, RHO, 0.3, dt, R, L, nthreads=8), s)
+    assert rel(g["U"], o["U"]) < 1e-13
+    assert rel(g["P"], o["P"]) < 1e-10
+    assert rel(g["Bt"], o["Bt"]) < 1e-13
+    assert rel(g["Ba"], o["Ba"]) < 1e-13
+
+
+CASES = [("ico1", 32, 2, False), ("cube2", 16, 1, False), ("ico1", 17, 2, True)]
+
+
+@pytest.mark.parametrize("name,L,bdf,ref_R", CASES)
+def test_time_domain_parity(ctx, oracle, name, L, bdf, ref_R):
+    V, F = small_meshes()[name]
+    E, nu = 5e5, 0.3
+    c1, c2 = material(E, nu)
+    dt = 1e-3
+    R = 1e-10 ** (1 / (2 * (L - 1))) if ref_R else 1e-10 ** (1 / (2 * L))
+    print(f"time-domain parity {name} L={L} bdf={bdf}", flush=True)
+    res = ctx.assemble(abi.make_mesh(V, F), abi.make_params(c1, c2, RHO, nu, dt, R, L, bdf=bdf))
+    op = lambda mode: oracle.params(c1, c2, RHO, nu, dt, R, L, bdf=bdf, mode=mode, nthreads=8)
+    f64 = oracle.assemble(V, F, op(oracle.MODE_FP64))
+    ld = oracle.assemble(V, F, op(oracle.MODE_LD))
+    for mine, theirs, tot in (("G", "G", 1e-9), ("H", "H", 1e-8), ("B_trans", "Bt", 1e-9),
+                              ("B_angular", "Ba", 1e-9), ("B", "B6", 1e-9)):
+        a = res[mine].reshape(L, -1); b = f64[theirs].reshape(L, -1); c = ld[theirs].reshape(L, -1)
+        assert rel(a, b) < tot, mine
+        for n in range(L):
+            e_gpu = rel(a[n], c[n])
+            e_f64 = rel(b[n], c[n])
+            assert e_gpu <= max(2 * e_f64, 1e-12), (mine, n, e_gpu, e_f64)
+
+
+def test_matrix_selection_and_host_device_outputs_agree(ctx):
+    """Selecting a subset of matrices, or device-resident outputs, does not change any result bit."""
+    import torch
+    V, F = small_meshes()["cube2"]
+    c1, c2 = material(1e5, 0.3)
+    L = 8
+    R = 1e-10 ** (1 / (2 * L))
+    mesh = abi.make_mesh(V, F)
+    only_h = ctx.assemble(mesh, abi.make_params(c1, c2, RHO, 0.3, 1e-3, R, L, which=abi.H))
+    assert set(only_h) == {"H"}
+    full = ctx.assemble(mesh, abi.make_params(c1, c2, RHO, 0.3, 1e-3, R, L))
+    assert np.array_equal(only_h["H"], full["H"])
+    n3 = 3 * len(V)
+    dev = {k: torch.zeros(shape, dtype=torch.float64, device="cuda") for k, shape in
+           (("G", (L, n3, n3)), ("H", (L, n3, n3)), ("B_trans", (L, n3, 3)), ("B_angular", (L, n3, 3, 3)),
+            ("B", (L, n3, 6)))}
+    ctx.assemble_device(mesh, abi.make_params(c1, c2, RHO, 0.3, 1e-3, R, L), {k: t.data_ptr() for k, t in dev.items()})
+    torch.cuda.synchronize()
+    for k in dev:
+        assert np.array_equal(dev[k].cpu().numpy(), full[k]), k
+
+
+def config1_mesh():
+    V, F = M.cube(8, 0.5)
+    return M.center(V, F), F
+
+
+def config3_mesh():
+    V, F = M.icosphere(4, 0.2)
+    return M.center(M.jitter(V, F, 0.01, 0), F), F
+
+
+@pytest.mark.parametrize("which_mesh", ["config1", "config3"])
+def test_full_size_row_sample_parity(ctx, oracle, which_mesh):
+    """At the full workload sizes the oracle assembles a contiguous sample of 8 collocation rows at
+    every frequency, in FP64 and in long double; per time-domain weight, the GPU rows must be at
+    least as close to the long-double rows as the FP64 oracle is (x2, floor 1e-11)."""
+    if which_mesh == "config1":
+        V, F = config1_mesh(); E, nu, L, dt = 1e5, 0.3, 32, 1e-3
+    else:
+        V, F = config3_mesh(); E, nu, L, dt = 5e5, 0.3, 32, 1e-3
+    c1, c2 = material(E, nu)
+    R = 1e-10 ** (1 / (2 * L))
+    nv = len(V)
+    r0, nr = nv // 3, 8
+    print(f"row-sample parity {which_mesh} nv={nv} rows {r0}..{r0 + nr - 1}", flush=True)
+    res = ctx.assemble(abi.make_mesh(V, F), abi.make_params(c1, c2, RHO, nu, dt, R, L, which=abi.G | abi.H))
+    s = oracle.frequencies(L, R, dt, 2)
+    C = oracle.C_collocation(V, F, nu).reshape(nv, 3, 3)
+    sl = slice(3 * r0, 3 * (r0 + nr))
+
+    def oracle_rows(mode):
+        op = oracle.params(c1, c2, RHO, nu, dt, R, L, mode=mode, nthreads=0)
+        su = np.zeros((L // 2 + 1, 3 * nr, 3 * nv), complex)
+        sp = np.zeros_like(su)
+        for l in range(L // 2 + 1):
+            o = oracle.laplace(V, F, op, s[l], mask=oracle.MASK_G | oracle.MASK_H, rows=(r0, r0 + nr))
+            su[l], sp[l] = o["U"][sl], o["P"][sl]
+        G = oracle.transform(su.reshape(L // 2 + 1, -1), L, R).reshape(L, 3 * nr, 3 * nv)
+        H = oracle.transform(sp.reshape(L // 2 + 1, -1), L, R).reshape(L, 3 * nr, 3 * nv)
+        for k in range(nr):
+            H[0, 3 * k:3 * k + 3, 3 * (r0 + k):3 * (r0 + k) + 3] += C[r0 + k]
+        return G, H
+
+    G64, H64 = oracle_rows(oracle.MODE_FP64)
+    Gld, Hld = oracle_rows(oracle.MODE_LD)
+    for mine, f64, ld in ((res["G"][:, sl], G64, Gld), (res["H"][:, sl], H64, Hld)):
+        assert rel(mine, f64) < 1e-7
+        for n in range(L):
+            e_gpu, e_f64 = rel(mine[n], ld[n]), rel(f64[n], ld[n])
+            assert e_gpu <= max(2 * e_f64, 1e-11), (n, e_gpu, e_f64)
+
+
+@pytest.mark.parametrize("which_mesh", ["config1", "config3"])
+def test_dimensional_scaling_full_size(ctx, which_mesh):
+    """Size-independent property at the full workload sizes: scaling lengths and the time step by 2
+    (exact in binary floating point) keeps every z = r s / c, so G doubles, H is unchanged, B_trans
+    scales by 4 and B_angular by 8 -- bit for bit.  A wrong power of r, s or c anywhere breaks it."""
+    V, F = config1_mesh() if which_mesh == "config1" else config3_mesh()
+    c1, c2 = material(1e5, 0.3)
+    L, dt = 32, 1e-3
+    R = 1e-10 ** (1 / (2 * L))
+    print(f"dimensional scaling {which_mesh}", flush=True)
+    a = ctx.assemble(abi.make_mesh(V, F), abi.make_params(c1, c2, RHO, 0.3, dt, R, L))
+    b = ctx.assemble(abi.make_mesh(2 * V, F), abi.make_params(c1, c2, RHO, 0.3, 2 * dt, R, L))
+    checks = {"G": (b["G"], 2 * a["G"]), "H": (b["H"], a["H"]), "B_trans": (b["B_trans"], 4 * a["B_trans"]),
+              "B_angular": (b["B_angular"], 8 * a["B_angular"])}
+    for k, (x, y) in checks.items():
+        d = np.abs(x - y).max() / np.abs(y).max()
+        print(f"  scaling {k}: max rel deviation {d:.3e}, bitwise {np.array_equal(x, y)}", flush=True)
+        assert d < 1e-14, k
