@@ -44,6 +44,11 @@ def build(verbose: bool = False) -> str:
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", OUT + ".tmp", *objs]
     subprocess.run(cmd, check=True)
     os.replace(OUT + ".tmp", OUT)
+    # command-line driver of the C++ collection facade (links the library in-tree, rpath $ORIGIN)
+    drv = os.path.join(PKG, "collection_driver")
+    subprocess.run([HIPCC, *COMMON, os.path.join(CSRC, "collection_driver.cpp"), "-o", drv + ".tmp", "-L" + PKG,
+                    "-lcqbem", "-Wl,-rpath,$ORIGIN"], check=True)
+    os.replace(drv + ".tmp", drv)
     return OUT
 
 

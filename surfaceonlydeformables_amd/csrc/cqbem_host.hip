@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <stdexcept>
@@ -131,6 +132,11 @@ struct Prepared {
     int nv, nf, L, Lf;
     std::vector<double> N, area;
     std::vector<int> vs_ptr, vs_face, sg_ptr, sg_face;
+    std::vector<int> patch_ptr, patch_verts;  // Morton-ordered column-vertex patches
+    std::vector<int2> patch_faces;            // (face, packed in-patch corner slots, 0xff = outside)
+    std::vector<int> pass_ptr, gather_ptr;    // per patch: first pass; per (pass, slot): CSR into gather_items
+    std::vector<unsigned char> gather_items;  // 3 * face-in-pass + corner, in face order
+    int npatches = 0;
     std::vector<double4> quad;
     std::vector<double2> line;
     std::vector<Freq> freqs;
@@ -186,6 +192,81 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
         for (int i = 0; i < P.nv; i++) P.sg_ptr[i + 1] = P.sg_ptr[i] + (int)lists[i].size();
         for (auto &l : lists) P.sg_face.insert(P.sg_face.end(), l.begin(), l.end());
     }
+    // Column-vertex patches: vertices sorted by the Morton code of their position, cut into runs of
+    // patch_size(); each patch lists every face with a corner in it.
+    {
+        const int C = patch_size();
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int v = 0; v < P.nv; v++)
+            for (int k = 0; k < 3; k++) lo[k] = std::min(lo[k], V[3 * v + k]), hi[k] = std::max(hi[k], V[3 * v + k]);
+        const double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
+        auto spread = [](uint64_t x) {
+            x &= 0x1fffff;
+            x = (x | x << 32) & 0x1f00000000ffffULL;
+            x = (x | x << 16) & 0x1f0000ff0000ffULL;
+            x = (x | x << 8) & 0x100f00f00f00f00fULL;
+            x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+            x = (x | x << 2) & 0x1249249249249249ULL;
+            return x;
+        };
+        std::vector<std::pair<uint64_t, int>> key(P.nv);
+        for (int v = 0; v < P.nv; v++) {
+            uint64_t c[3];
+            for (int k = 0; k < 3; k++) c[k] = (uint64_t)std::min(2097151.0, (V[3 * v + k] - lo[k]) / ext * 2097151.0);
+            key[v] = {spread(c[0]) | spread(c[1]) << 1 | spread(c[2]) << 2, v};
+        }
+        std::sort(key.begin(), key.end());
+        P.npatches = (P.nv + C - 1) / C;
+        P.patch_verts.assign((size_t)P.npatches * C, -1);
+        std::vector<int> slot(P.nv), patch(P.nv);
+        for (int r = 0; r < P.nv; r++) {
+            const int v = key[r].second;
+            P.patch_verts[r] = v, patch[v] = r / C, slot[v] = r % C;
+        }
+        std::vector<std::vector<int2>> lists(P.npatches);
+        for (int f = 0; f < P.nf; f++) {
+            int ps[3] = {patch[F[3 * f]], patch[F[3 * f + 1]], patch[F[3 * f + 2]]};
+            for (int k = 0; k < 3; k++) {
+                bool seen = false;
+                for (int k2 = 0; k2 < k; k2++) seen |= ps[k2] == ps[k];
+                if (seen) continue;
+                int packed = 0;
+                for (int c = 0; c < 3; c++) packed |= (ps[c] == ps[k] ? slot[F[3 * f + c]] : 0xff) << (8 * c);
+                lists[ps[k]].push_back(make_int2(f, packed));
+            }
+        }
+        P.patch_ptr.assign(1, 0);
+        for (auto &l : lists) {
+            P.patch_faces.insert(P.patch_faces.end(), l.begin(), l.end());
+            P.patch_ptr.push_back((int)P.patch_faces.size());
+        }
+        // gather lists: for each pass of patch_pass_faces() faces, one row of C+1 ints (row start, then
+        // per-slot ends) into gather_items, which names each slot's staged (face, corner) entries in face order
+        const int PF = patch_pass_faces();
+        int npass = 0;
+        for (int pc = 0; pc < P.npatches; pc++) {
+            P.pass_ptr.push_back(npass);
+            const int nfc = P.patch_ptr[pc + 1] - P.patch_ptr[pc];
+            for (int b = 0; b < nfc; b += PF, npass++) {
+                std::vector<std::vector<unsigned char>> per(C);
+                for (int fi = 0; fi < PF && b + fi < nfc; fi++) {
+                    const int packed = P.patch_faces[P.patch_ptr[pc] + b + fi].y;
+                    for (int k = 0; k < 3; k++) {
+                        const int sl = (packed >> (8 * k)) & 0xff;
+                        if (sl != 0xff) per[sl].push_back((unsigned char)(3 * fi + k));
+                    }
+                }
+                P.gather_ptr.push_back((int)P.gather_items.size());
+                for (int sl = 0; sl < C; sl++) {
+                    P.gather_items.insert(P.gather_items.end(), per[sl].begin(), per[sl].end());
+                    P.gather_ptr.push_back((int)P.gather_items.size());
+                }
+            }
+        }
+        P.pass_ptr.push_back(npass);
+        if (P.gather_ptr.empty()) P.gather_ptr.assign(C + 1, 0);
+        if (P.gather_items.empty()) P.gather_items.push_back(0);
+    }
     for (const TriPoint &t : composite_tri_rule(p->order, p->subdivision)) P.quad.push_back(make_double4(t.b[0], t.b[1], t.b[2], t.w));
     for (auto &xw : composite_line01(p->order, p->subdivision)) P.line.push_back(make_double2(xw.first, xw.second));
     const auto s = cq_frequencies(p->L, p->R, p->dt, p->bdf);
@@ -207,9 +288,42 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
 }
 
 struct DevMesh {
-    DBuf V, F, N, area, vs_ptr, vs_face, sg_ptr, sg_face, vmap, quad, line, freqs, tab;
+    DBuf V, F, N, area, vs_ptr, vs_face, sg_ptr, sg_face, vmap, quad, line, freqs, tab, patch_ptr, patch_faces, patch_verts;
+    DBuf pass_ptr, gather_ptr, gather_items;
     Geom g;
 };
+
+// Collocation G/H regular-pair kernel variant: "star" (default: one lane per column vertex over its
+// vertex star), "patch" (Morton patches, each face once per patch) or "patch_split" (patch, G and H in
+// separate launches); CQBEM_COLLOC_KERNEL overrides.
+int colloc_variant() {
+    const char *e = std::getenv("CQBEM_COLLOC_KERNEL");
+    if (!e) return 2;
+    if (!std::strcmp(e, "patch")) return 0;
+    if (!std::strcmp(e, "patch_split")) return 1;
+    return 2;
+}
+
+hipError_t launch_gh(const DevMesh &d, const Prepared &P, const Freq *freqs, int l0, int nl, int row0, int nrows, cd *UG,
+                     cd *UH, long long ld, long long lstride, hipStream_t st) {
+    const int var = colloc_variant();
+    const double4 *quad = d.quad.as<double4>();
+    const int npts = (int)P.quad.size();
+    if (var == 2) return launch_colloc_gh(d.g, quad, npts, freqs, l0, nl, P.mat, row0, nrows, UG, UH, ld, lstride, st);
+    const int *pp = d.patch_ptr.as<int>(), *pv = d.patch_verts.as<int>();
+    const int2 *pf = d.patch_faces.as<int2>();
+    const int *psp = d.pass_ptr.as<int>(), *gp = d.gather_ptr.as<int>();
+    const unsigned char *gi = d.gather_items.as<unsigned char>();
+    if (var == 1 && UG && UH) {
+        hipError_t e = launch_colloc_gh_patch(d.g, quad, npts, freqs, l0, nl, P.mat, row0, nrows, pp, pf, pv, psp, gp, gi, P.npatches, UG,
+                                              nullptr, ld, lstride, st);
+        if (e != hipSuccess) return e;
+        return launch_colloc_gh_patch(d.g, quad, npts, freqs, l0, nl, P.mat, row0, nrows, pp, pf, pv, psp, gp, gi, P.npatches, nullptr, UH,
+                                      ld, lstride, st);
+    }
+    return launch_colloc_gh_patch(d.g, quad, npts, freqs, l0, nl, P.mat, row0, nrows, pp, pf, pv, psp, gp, gi, P.npatches, UG, UH, ld,
+                                  lstride, st);
+}
 
 void upload_mesh(DevMesh &d, const cqbem_mesh *mesh, const Prepared &P, hipStream_t st) {
     d.V.alloc(sizeof(double) * 3 * P.nv);
@@ -223,6 +337,8 @@ void upload_mesh(DevMesh &d, const cqbem_mesh *mesh, const Prepared &P, hipStrea
     upload(d.N, P.N, st), upload(d.area, P.area, st), upload(d.vs_ptr, P.vs_ptr, st), upload(d.vs_face, P.vs_face, st);
     upload(d.sg_ptr, P.sg_ptr, st), upload(d.sg_face, P.sg_face, st), upload(d.quad, P.quad, st), upload(d.line, P.line, st);
     upload(d.freqs, P.freqs, st), upload(d.tab, P.tab, st);
+    upload(d.patch_ptr, P.patch_ptr, st), upload(d.patch_faces, P.patch_faces, st), upload(d.patch_verts, P.patch_verts, st);
+    upload(d.pass_ptr, P.pass_ptr, st), upload(d.gather_ptr, P.gather_ptr, st), upload(d.gather_items, P.gather_items, st);
     Geom &g = d.g;
     g.V = d.V.as<double>(), g.F = d.F.as<int>(), g.N = d.N.as<double>(), g.area = d.area.as<double>();
     g.vs_ptr = d.vs_ptr.as<int>(), g.vs_face = d.vs_face.as<int>(), g.sg_ptr = d.sg_ptr.as<int>(), g.sg_face = d.sg_face.as<int>();
@@ -336,8 +452,7 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
                 const int nr = std::min(rows, nv - r0);
                 {
                     Timer t(st, &tm[1]);
-                    HIPCHK(launch_colloc_gh(g, quad, npts, freqs, 0, Lf, P.mat, r0, nr, specG.as<cd>(), specH.as<cd>(), n3,
-                                            lstride, st));
+                    HIPCHK(launch_gh(d, P, freqs, 0, Lf, r0, nr, specG.as<cd>(), specH.as<cd>(), n3, lstride, st));
                 }
                 tm[6] += 1;
                 for (int i = r0; i < r0 + nr; i++) tm[7] += (double)(P.nf - (P.sg_ptr[i + 1] - P.sg_ptr[i])) * npts * Lf;
@@ -550,7 +665,7 @@ int cqbem_laplace(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params
         const int npts = (int)P.quad.size(), nline = (int)P.line.size();
         if (!params->galerkin) {
             if (wG || wH) {
-                HIPCHK(launch_colloc_gh(d.g, quad, npts, freqs, l, 1, P.mat, 0, nv, dU.as<cd>(), dP.as<cd>(), n3, n3 * n3, st));
+                HIPCHK(launch_gh(d, P, freqs, l, 1, 0, nv, dU.as<cd>(), dP.as<cd>(), n3, n3 * n3, st));
                 if (wG) HIPCHK(launch_colloc_singular(0, d.g, line, nline, freqs, l, 1, P.mat, 0, nv, dU.as<cd>(), n3, n3 * n3, nullptr, nullptr, st));
                 if (wH) HIPCHK(launch_colloc_singular(2, d.g, line, nline, freqs, l, 1, P.mat, 0, nv, dP.as<cd>(), n3, n3 * n3, nullptr, nullptr, st));
             }

@@ -22,6 +22,12 @@ struct Geom {
 
 hipError_t launch_colloc_gh(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
                             int row0, int nrows, cd *UG, cd *UH, long long ld, long long lstride, hipStream_t st);
+hipError_t launch_colloc_gh_patch(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
+                                  int row0, int nrows, const int *patch_ptr, const int2 *patch_faces, const int *patch_verts,
+                                  const int *pass_ptr, const int *gather_ptr, const unsigned char *gather_items,
+                                  int npatches, cd *UG, cd *UH, long long ld, long long lstride, hipStream_t st);
+int patch_size();
+int patch_pass_faces();
 hipError_t launch_colloc_b(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
                            int row0, int nrows, cd *Bt, cd *Ba, hipStream_t st);
 hipError_t launch_colloc_singular(int kind, const Geom &g, const double2 *line, int nline, const Freq *freqs, int l0, int nl,

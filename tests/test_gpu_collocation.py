@@ -57,11 +57,14 @@ def test_laplace_parity(ctx, oracle, name, l):
     assert rel(g["Ba"], o["Ba"]) < 1e-13
 
 
-CASES = [("ico1", 32, 2, False), ("cube2", 16, 1, False), ("ico1", 17, 2, True)]
+CASES = [("ico1", 32, 2, False, "star"), ("cube2", 16, 1, False, "star"), ("ico1", 17, 2, True, "star"),
+         ("ico1", 32, 2, False, "patch"), ("cube2", 16, 1, False, "patch_split")]
 
 
-@pytest.mark.parametrize("name,L,bdf,ref_R", CASES)
-def test_time_domain_parity(ctx, oracle, name, L, bdf, ref_R):
+@pytest.mark.parametrize("name,L,bdf,ref_R,variant", CASES)
+def test_time_domain_parity(ctx, oracle, name, L, bdf, ref_R, variant, monkeypatch):
+    """Every variant of the regular-pair G/H kernel (patch, patch_split, star) meets the same bound."""
+    monkeypatch.setenv("CQBEM_COLLOC_KERNEL", variant)
     V, F = small_meshes()[name]
     E, nu = 5e5, 0.3
     c1, c2 = material(E, nu)
@@ -158,8 +161,11 @@ def test_full_size_row_sample_parity(ctx, oracle, which_mesh):
 @pytest.mark.parametrize("which_mesh", ["config1", "config3"])
 def test_dimensional_scaling_full_size(ctx, which_mesh):
     """Size-independent property at the full workload sizes: scaling lengths and the time step by 2
-    (exact in binary floating point) keeps every z = r s / c, so G doubles, H is unchanged, B_trans
-    scales by 4 and B_angular by 8 -- bit for bit.  A wrong power of r, s or c anywhere breaks it."""
+    (exact in binary floating point) keeps every z = r s / c, so G doubles, B_trans scales by 4 and
+    B_angular by 8 -- bit for bit -- and H is unchanged.  H is exact up to the Guiggiani finite-part
+    term f log(rho_bar A) of the singular faces: scaling adds log(2) times the sum of f over the faces
+    around a vertex, which vanishes analytically and to the theta-quadrature error (~1e-9) numerically.
+    A wrong power of r, s or c anywhere breaks these at O(1)."""
     V, F = config1_mesh() if which_mesh == "config1" else config3_mesh()
     c1, c2 = material(1e5, 0.3)
     L, dt = 32, 1e-3
@@ -172,4 +178,7 @@ def test_dimensional_scaling_full_size(ctx, which_mesh):
     for k, (x, y) in checks.items():
         d = np.abs(x - y).max() / np.abs(y).max()
         print(f"  scaling {k}: max rel deviation {d:.3e}, bitwise {np.array_equal(x, y)}", flush=True)
-        assert d < 1e-14, k
+        if k == "H":
+            assert d < 1e-8, k
+        else:
+            assert np.array_equal(x, y), k
