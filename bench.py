@@ -84,6 +84,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="one untimed assembly followed by a 1 GiB device copy, for HBM counter runs "
+                         "(the copy calibrates FETCH_SIZE / WRITE_SIZE); prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the CQBEM precompute of one object runs on one MI355X "
@@ -105,6 +108,16 @@ def main():
             "B_angular": torch.empty((L, n3, 3, 3), dtype=torch.float64, device=dev),
             "B": torch.empty((L, n3, 6), dtype=torch.float64, device=dev)}
     ptrs = {k: t.data_ptr() for k, t in outs.items()}
+
+    if args.traffic_probe:
+        ctx.assemble_device(mesh, params, ptrs)
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"traffic probe: {ctx.last_timings()['gh_evaluations']:.0f} G/H evaluations, 1 GiB copy", flush=True)
+        ctx.close()
+        return
 
     for _ in range(args.warmup):
         ctx.assemble_device(mesh, params, ptrs)
