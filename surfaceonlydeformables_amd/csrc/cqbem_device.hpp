@@ -41,11 +41,113 @@ __host__ __device__ inline void fma_cr(cd &a, cd b, double s) {
     a.y = fma(b.y, s, a.y);
 }
 
-// exp(-z)
+// ---------------------------------------------------------------------------
+// FP64 elementary functions for the kernel arguments.  The CQ exponent
+// z = s r / c has Re z >= 0 (the BDF symbols are A-stable, so Re s_l > 0) and
+// a moderate imaginary part, which allows an exact Cody-Waite reduction
+// (W. J. Cody, W. Waite, "Software Manual for the Elementary Functions", 1980)
+// followed by truncated Taylor series (the first omitted terms are below
+// 1e-17 relative) whose leading terms are summed with their rounding errors
+// carried (Fast2Sum), so e^{-a} stays within 0.65 ulp and sin, cos within
+// 0.4 ulp of 1, at about half the instruction count of the library calls.
+// Contraction is off inside them: every FMA is written out, so the error
+// analysis holds as written.
+// ---------------------------------------------------------------------------
+constexpr double kLog2e = 1.4426950408889634;         // 1/ln 2
+constexpr double kLn2Hi = 0x1.62e42fefa3800p-1;       // ln 2 to 42 bits: k kLn2Hi exact for |k| < 2^11
+constexpr double kLn2Lo = 5.497923018708371e-14;      // ln 2 - kLn2Hi
+constexpr double kTwoOverPi = 0.6366197723675814;     // 2/pi
+constexpr double kPio2A = 0x1.921fb54400000p+0;       // pi/2 to 33 bits: k kPio2A exact for |k| < 2^20
+constexpr double kPio2B = 0x1.0b4611a600000p-34;      // next 33 bits of pi/2
+constexpr double kPio2C = 2.0222662487959506e-21;     // pi/2 - kPio2A - kPio2B
+// |Im z| bound of the exact reduction (|k| < 2^20).  The host rejects jobs whose largest argument
+// max|s_l| diam / min(c1, c2) reaches it.
+constexpr double kTrigArgMax = 1.0e6;
+
+// e^{-a}, a >= -700.  k = rint(-a / ln 2); f + c = -a - k ln 2 with |f| <= ln2/2 (the k kLn2Hi part
+// is exact by Sterbenz); e^(f+c) = (1 + f) + (f^2 p(f) + c (1 + f)) with p the Taylor series of
+// (e^f - 1 - f) / f^2 through f^11 (next term < 5e-18); 1 + f is split exactly.  Scaled by 2^k;
+// -a is clamped at -740, where the result has underflowed to 0.
+__device__ inline double exp_neg(double a) {
+#pragma clang fp contract(off)
+    const double t = fmin(fmax(-a, -740.0), 700.0);
+    const double k = rint(t * kLog2e);
+    const double fh = fma(-k, kLn2Hi, t);
+    const double fl = -k * kLn2Lo;
+    const double f = fh + fl;
+    const double c = (fh - f) + fl;
+    double p = 1.0 / 6227020800.0;     // 1/13!
+    p = fma(p, f, 1.0 / 479001600.0);  // 1/12!
+    p = fma(p, f, 1.0 / 39916800.0);   // 1/11!
+    p = fma(p, f, 1.0 / 3628800.0);
+    p = fma(p, f, 1.0 / 362880.0);
+    p = fma(p, f, 1.0 / 40320.0);
+    p = fma(p, f, 1.0 / 5040.0);
+    p = fma(p, f, 1.0 / 720.0);
+    p = fma(p, f, 1.0 / 120.0);
+    p = fma(p, f, 1.0 / 24.0);
+    p = fma(p, f, 1.0 / 6.0);
+    p = fma(p, f, 0.5);
+    const double tail = fma(f * f, p, fma(c, f, c));
+    const double s1 = 1.0 + f, e1 = (1.0 - s1) + f;
+    return ldexp(s1 + (e1 + tail), (int)k);
+}
+
+// sin y, cos y for |y| < kTrigArgMax.  k = rint(2y/pi); r + rr = y - k pi/2 to ~1e-30 (three-part
+// pi/2, the first two products exact, the subtraction split by TwoSum); sin r through r^17 and
+// cos r through r^18 (next terms < 1e-19), first-order in rr; 1 - r^2/2 split exactly.  The
+// quadrant k mod 4 then selects and signs them.
+__device__ inline void cq_sincos(double y, double *s, double *c) {
+#pragma clang fp contract(off)
+    const double k = rint(y * kTwoOverPi);
+    const double r1 = fma(-k, kPio2A, y);
+    const double t = k * kPio2B;
+    const double r = r1 - t;
+    const double bb = r - r1;
+    double rr = (r1 - (r - bb)) - (t + bb);
+    rr = fma(-k, kPio2C, rr);
+    const double x = r * r;
+    double ps = 1.0 / 355687428096000.0;      // 1/17!
+    ps = fma(ps, x, -1.0 / 1307674368000.0);  // -1/15!
+    ps = fma(ps, x, 1.0 / 6227020800.0);
+    ps = fma(ps, x, -1.0 / 39916800.0);
+    ps = fma(ps, x, 1.0 / 362880.0);
+    ps = fma(ps, x, -1.0 / 5040.0);
+    ps = fma(ps, x, 1.0 / 120.0);
+    ps = fma(ps, x, -1.0 / 6.0);
+    double pc = -1.0 / 6402373705728000.0;    // -1/18!
+    pc = fma(pc, x, 1.0 / 20922789888000.0);  // 1/16!
+    pc = fma(pc, x, -1.0 / 87178291200.0);
+    pc = fma(pc, x, 1.0 / 479001600.0);
+    pc = fma(pc, x, -1.0 / 3628800.0);
+    pc = fma(pc, x, 1.0 / 40320.0);
+    pc = fma(pc, x, -1.0 / 720.0);
+    pc = fma(pc, x, 1.0 / 24.0);
+    const double sr = r + fma(r * x, ps, rr * fma(-0.5, x, 1.0));
+    const double xl = fma(r, r, -x);
+    const double hz = 0.5 * x;
+    const double w = 1.0 - hz, ew = (1.0 - w) - hz;
+    const double cr = w + (ew + fma(x * x, pc, -fma(0.5, xl, rr * r)));
+    const int q = (int)k;
+    const double so = (q & 1) ? cr : sr, co = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -so : so;
+    *c = ((q + 1) & 2) ? -co : co;
+}
+
+// 1/sqrt(d), d > 0: hardware estimate refined by two Newton steps y += y (1 - d y^2) / 2.
+__device__ inline double rsqrt_nr(double d) {
+    double y = __builtin_amdgcn_rsq(d);
+    double e = fma(-d * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-d * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+
+// exp(-z), Re z >= 0
 __device__ inline cd cexp_neg(cd z) {
-    const double m = exp(-z.x);
+    const double m = exp_neg(z.x);
     double sn, cs;
-    sincos(z.y, &sn, &cs);
+    cq_sincos(z.y, &sn, &cs);
     return cd{m * cs, -m * sn};
 }
 

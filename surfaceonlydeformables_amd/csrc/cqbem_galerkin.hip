@@ -89,8 +89,8 @@ __global__ void __launch_bounds__(64) k_gal_gh(Geom g, const double4 *__restrict
                 const double r0 = Q.x * p0[0] + Q.y * p1[0] + Q.z * p2[0] - x0;
                 const double r1 = Q.x * p0[1] + Q.y * p1[1] + Q.z * p2[1] - x1;
                 const double r2 = Q.x * p0[2] + Q.y * p1[2] + Q.z * p2[2] - x2;
-                const double r = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
-                const double ir = 1.0 / r, ir2 = ir * ir;
+                const double d2 = r0 * r0 + r1 * r1 + r2 * r2;
+                const double ir = rsqrt_nr(d2), r = d2 * ir, ir2 = ir * ir;
                 const double h0 = r0 * ir, h1 = r1 * ir, h2 = r2 * ir;
                 const double bc = c == 0 ? Q.x : (c == 1 ? Q.y : Q.z);
                 const double wL = Q.w * ar * bc;
@@ -210,8 +210,8 @@ __global__ void __launch_bounds__(256) k_gal_b(Geom g, const double4 *__restrict
                 const double r0 = Q.x * p0[0] + Q.y * p1[0] + Q.z * p2[0] - x0;
                 const double r1 = Q.x * p0[1] + Q.y * p1[1] + Q.z * p2[1] - x1;
                 const double r2 = Q.x * p0[2] + Q.y * p1[2] + Q.z * p2[2] - x2;
-                const double r = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
-                const double ir = 1.0 / r, ir2 = ir * ir;
+                const double d2 = r0 * r0 + r1 * r1 + r2 * r2;
+                const double ir = rsqrt_nr(d2), r = d2 * ir, ir2 = ir * ir;
                 const double h[3] = {r0 * ir, r1 * ir, r2 * ir};
                 const double w = Q.w * ar;
                 const Expo E = expo(r, f);
@@ -289,8 +289,8 @@ template <int MODE>
 __device__ __forceinline__ void sing_eval(const double x[3], const double y[3], const double Lx[3], const double Ly[3], double wt,
                                  const double n[3], const Freq &f, const Mat &m, cd *acc) {
     const double r0 = y[0] - x[0], r1 = y[1] - x[1], r2 = y[2] - x[2];
-    const double r = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
-    const double ir = 1.0 / r, ir2 = ir * ir;
+    const double d2 = r0 * r0 + r1 * r1 + r2 * r2;
+    const double ir = rsqrt_nr(d2), r = d2 * ir, ir2 = ir * ir;
     const double h[3] = {r0 * ir, r1 * ir, r2 * ir};
     const Expo E = expo(r, f);
     if (MODE == 0) {

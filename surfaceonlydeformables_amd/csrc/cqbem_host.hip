@@ -89,6 +89,20 @@ void check_inputs(const cqbem_mesh *mesh, const cqbem_params *p) {
         throw Err{CQBEM_ERR_UNSUPPORTED, "collocation C matrix of a vertex-split mesh needs the original mesh"};
     if (p->galerkin && mesh->vertex_map_inverse)
         throw Err{CQBEM_ERR_UNSUPPORTED, "Galerkin assembly of a vertex-split mesh is not supported"};
+    // the device sincos reduces |Im(s) r / c| <= max|s_l| diam / min(c1, c2) exactly below kTrigArgMax
+    double lo[3] = {mesh->V[0], mesh->V[1], mesh->V[2]}, hi[3] = {lo[0], lo[1], lo[2]};
+    for (int64_t k = 0; k < mesh->nv; k++)
+        for (int d = 0; d < 3; d++) {
+            const double x = mesh->V[3 * k + d];
+            if (!std::isfinite(x)) throw Err{CQBEM_ERR_ARG, "non-finite vertex coordinate"};
+            lo[d] = std::min(lo[d], x), hi[d] = std::max(hi[d], x);
+        }
+    const double diam = std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                                  (hi[2] - lo[2]) * (hi[2] - lo[2]));
+    double smax = 0.0;
+    for (const auto &s : cq_frequencies(p->L, p->R, p->dt, p->bdf)) smax = std::max(smax, std::abs(s));
+    if (smax * diam / std::min(p->c1, p->c2) >= kTrigArgMax)
+        throw Err{CQBEM_ERR_UNSUPPORTED, "max |s| * diameter / c exceeds 1e6: time step too small for the mesh size"};
 }
 
 }  // namespace
@@ -137,7 +151,13 @@ struct Prepared {
     std::vector<int> pass_ptr, gather_ptr;    // per patch: first pass; per (pass, slot): CSR into gather_items
     std::vector<unsigned char> gather_items;  // 3 * face-in-pass + corner, in face order
     int npatches = 0;
-    std::vector<double4> quad;
+    // pair form: patches of pair_patch_size() vertices, passes of pair_pass_faces() faces
+    std::vector<int> pair_patch_pass, pair_patch_verts;
+    std::vector<int2> pair_pass_info;   // (first face entry, nfaces | nsub << 16)
+    std::vector<int4> pair_pass_faces;  // (face, corner codes), code = slot | sub-step << 10, -1 outside
+    int pair_npatches = 0;
+    int npts = 0;               // points of the composite triangle rule
+    std::vector<double4> quad;  // the rule, padded to an even count with a zero-weight point
     std::vector<double2> line;
     std::vector<Freq> freqs;
     std::vector<double2> tab;
@@ -267,7 +287,70 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
         if (P.gather_ptr.empty()) P.gather_ptr.assign(C + 1, 0);
         if (P.gather_items.empty()) P.gather_items.push_back(0);
     }
+    // Pair-form patches: vertices in the same Morton order, grown into a patch while it stays within
+    // pair_patch_size() vertices and its face list within kPairTargetPasses full passes (a patch lists every
+    // face with a corner in it, in passes of pair_pass_faces() faces).  Within a pass every in-patch corner
+    // gets a sub-step: its rank among the pass's updates of that vertex, in face order.
+    {
+        constexpr int kPairTargetPasses = 4;
+        const int C = pair_patch_size(), PF = pair_pass_faces();
+        std::vector<int> order(P.patch_verts.begin(), P.patch_verts.end());
+        order.erase(std::remove(order.begin(), order.end(), -1), order.end());
+        std::vector<int> slot(P.nv), patch(P.nv), mark(P.nf, -1);
+        std::vector<std::vector<int>> lists;
+        for (int r = 0; r < P.nv;) {
+            const int pc = (int)lists.size();
+            lists.emplace_back();
+            int cnt = 0;
+            while (r < P.nv && cnt < C) {
+                const int v = order[r];
+                int fresh = 0;
+                for (int t = P.vs_ptr[v]; t < P.vs_ptr[v + 1]; t++) fresh += mark[P.vs_face[t] >> 2] != pc;
+                if (cnt > 0 && (int)lists[pc].size() + fresh > kPairTargetPasses * PF) break;
+                for (int t = P.vs_ptr[v]; t < P.vs_ptr[v + 1]; t++) {
+                    const int f = P.vs_face[t] >> 2;
+                    if (mark[f] != pc) mark[f] = pc, lists[pc].push_back(f);
+                }
+                patch[v] = pc, slot[v] = cnt++, r++;
+            }
+        }
+        P.pair_npatches = (int)lists.size();
+        P.pair_patch_verts.assign((size_t)P.pair_npatches * C, -1);
+        for (int v = 0; v < P.nv; v++) P.pair_patch_verts[(size_t)patch[v] * C + slot[v]] = v;
+        P.pair_patch_pass.assign(1, 0);
+        std::vector<int> cnt(C);
+        for (int pc = 0; pc < P.pair_npatches; pc++) {
+            auto &l = lists[pc];
+            std::sort(l.begin(), l.end());
+            for (size_t b = 0; b < l.size(); b += PF) {
+                const int nf = (int)std::min<size_t>(PF, l.size() - b);
+                std::fill(cnt.begin(), cnt.end(), 0);
+                int nsub = 0;
+                const int first = (int)P.pair_pass_faces.size();
+                for (int k = 0; k < nf; k++) {
+                    const int f = l[b + k];
+                    int code[3];
+                    for (int c = 0; c < 3; c++) {
+                        const int v = F[3 * f + c];
+                        if (patch[v] != pc) {
+                            code[c] = -1;
+                            continue;
+                        }
+                        const int sub = cnt[slot[v]]++;
+                        nsub = std::max(nsub, sub + 1);
+                        code[c] = slot[v] | sub << 10;
+                    }
+                    P.pair_pass_faces.push_back(make_int4(f, code[0], code[1], code[2]));
+                }
+                if (nsub > 0x7fff) throw Err{CQBEM_ERR_UNSUPPORTED, "vertex valence too large for the pair schedule"};
+                P.pair_pass_info.push_back(make_int2(first, nf | nsub << 16));
+            }
+            P.pair_patch_pass.push_back((int)P.pair_pass_info.size());
+        }
+    }
     for (const TriPoint &t : composite_tri_rule(p->order, p->subdivision)) P.quad.push_back(make_double4(t.b[0], t.b[1], t.b[2], t.w));
+    P.npts = (int)P.quad.size();
+    if (P.quad.size() & 1) P.quad.push_back(make_double4(1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 0.0));
     for (auto &xw : composite_line01(p->order, p->subdivision)) P.line.push_back(make_double2(xw.first, xw.second));
     const auto s = cq_frequencies(p->L, p->R, p->dt, p->bdf);
     for (auto sl : s) {
@@ -290,31 +373,39 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
 struct DevMesh {
     DBuf V, F, N, area, vs_ptr, vs_face, sg_ptr, sg_face, vmap, quad, line, freqs, tab, patch_ptr, patch_faces, patch_verts;
     DBuf pass_ptr, gather_ptr, gather_items;
+    DBuf pair_patch_pass, pair_pass_info, pair_pass_faces, pair_patch_verts;
     Geom g;
 };
 
-// Collocation G/H regular-pair kernel variant: "star" (default: one lane per column vertex over its
-// vertex star), "patch" (Morton patches, each face once per patch) or "patch_split" (patch, G and H in
-// separate launches); CQBEM_COLLOC_KERNEL overrides.
+// Collocation G/H regular-face kernel variant: "pair" (default: Morton patches, each face evaluated
+// once per patch by a lane pair), "star" (one lane per column vertex over its vertex star), "patch" or
+// "patch_split" (Morton patches, two lanes per face holding all three corners); CQBEM_COLLOC_KERNEL
+// overrides.
+enum { kVarPatch = 0, kVarPatchSplit = 1, kVarStar = 2, kVarPair = 3 };
 int colloc_variant() {
     const char *e = std::getenv("CQBEM_COLLOC_KERNEL");
-    if (!e) return 2;
-    if (!std::strcmp(e, "patch")) return 0;
-    if (!std::strcmp(e, "patch_split")) return 1;
-    return 2;
+    if (!e) return kVarPair;
+    if (!std::strcmp(e, "patch")) return kVarPatch;
+    if (!std::strcmp(e, "patch_split")) return kVarPatchSplit;
+    if (!std::strcmp(e, "star")) return kVarStar;
+    return kVarPair;
 }
 
 hipError_t launch_gh(const DevMesh &d, const Prepared &P, const Freq *freqs, int l0, int nl, int row0, int nrows, cd *UG,
                      cd *UH, long long ld, long long lstride, hipStream_t st) {
-    const int var = colloc_variant();
+    int var = colloc_variant();
     const double4 *quad = d.quad.as<double4>();
-    const int npts = (int)P.quad.size();
-    if (var == 2) return launch_colloc_gh(d.g, quad, npts, freqs, l0, nl, P.mat, row0, nrows, UG, UH, ld, lstride, st);
+    const int npts = P.npts;
+    if (var == kVarPair)
+        return launch_colloc_gh_pair(d.g, quad, (int)P.quad.size(), freqs, l0, nl, P.mat, row0, nrows,
+                                     d.pair_patch_pass.as<int>(), d.pair_pass_info.as<int2>(), d.pair_pass_faces.as<int4>(),
+                                     d.pair_patch_verts.as<int>(), P.pair_npatches, UG, UH, ld, lstride, st);
+    if (var == kVarStar) return launch_colloc_gh(d.g, quad, npts, freqs, l0, nl, P.mat, row0, nrows, UG, UH, ld, lstride, st);
     const int *pp = d.patch_ptr.as<int>(), *pv = d.patch_verts.as<int>();
     const int2 *pf = d.patch_faces.as<int2>();
     const int *psp = d.pass_ptr.as<int>(), *gp = d.gather_ptr.as<int>();
     const unsigned char *gi = d.gather_items.as<unsigned char>();
-    if (var == 1 && UG && UH) {
+    if (var == kVarPatchSplit && UG && UH) {
         hipError_t e = launch_colloc_gh_patch(d.g, quad, npts, freqs, l0, nl, P.mat, row0, nrows, pp, pf, pv, psp, gp, gi, P.npatches, UG,
                                               nullptr, ld, lstride, st);
         if (e != hipSuccess) return e;
@@ -339,6 +430,8 @@ void upload_mesh(DevMesh &d, const cqbem_mesh *mesh, const Prepared &P, hipStrea
     upload(d.freqs, P.freqs, st), upload(d.tab, P.tab, st);
     upload(d.patch_ptr, P.patch_ptr, st), upload(d.patch_faces, P.patch_faces, st), upload(d.patch_verts, P.patch_verts, st);
     upload(d.pass_ptr, P.pass_ptr, st), upload(d.gather_ptr, P.gather_ptr, st), upload(d.gather_items, P.gather_items, st);
+    upload(d.pair_patch_pass, P.pair_patch_pass, st), upload(d.pair_pass_info, P.pair_pass_info, st);
+    upload(d.pair_pass_faces, P.pair_pass_faces, st), upload(d.pair_patch_verts, P.pair_patch_verts, st);
     Geom &g = d.g;
     g.V = d.V.as<double>(), g.F = d.F.as<int>(), g.N = d.N.as<double>(), g.area = d.area.as<double>();
     g.vs_ptr = d.vs_ptr.as<int>(), g.vs_face = d.vs_face.as<int>(), g.sg_ptr = d.sg_ptr.as<int>(), g.sg_face = d.sg_face.as<int>();
@@ -438,7 +531,7 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
     const double2 *line = d.line.as<double2>();
     const Freq *freqs = d.freqs.as<Freq>();
     const double2 *tab = d.tab.as<double2>();
-    const int npts = (int)P.quad.size(), nline = (int)P.line.size();
+    const int npts = P.npts, nline = (int)P.line.size();
 
     if (!p->galerkin) {
         const int nmat = (wantG ? 1 : 0) + (wantH ? 1 : 0);
@@ -662,7 +755,7 @@ int cqbem_laplace(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params
         const double4 *quad = d.quad.as<double4>();
         const double2 *line = d.line.as<double2>();
         const Freq *freqs = d.freqs.as<Freq>();
-        const int npts = (int)P.quad.size(), nline = (int)P.line.size();
+        const int npts = P.npts, nline = (int)P.line.size();
         if (!params->galerkin) {
             if (wG || wH) {
                 HIPCHK(launch_gh(d, P, freqs, l, 1, 0, nv, dU.as<cd>(), dP.as<cd>(), n3, n3 * n3, st));

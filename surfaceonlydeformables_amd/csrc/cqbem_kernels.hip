@@ -64,8 +64,8 @@ __global__ void __launch_bounds__(64) k_colloc_gh(Geom g, const double4 *__restr
             const double r0 = Q.x * y00 + Q.y * y10 + Q.z * y20 - x0;
             const double r1 = Q.x * y01 + Q.y * y11 + Q.z * y21 - x1;
             const double r2 = Q.x * y02 + Q.y * y12 + Q.z * y22 - x2;
-            const double r = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
-            const double ir = 1.0 / r, ir2 = ir * ir;
+            const double d2 = r0 * r0 + r1 * r1 + r2 * r2;
+            const double ir = rsqrt_nr(d2), r = d2 * ir, ir2 = ir * ir;
             const double h0 = r0 * ir, h1 = r1 * ir, h2 = r2 * ir;
             const double bc = c == 0 ? Q.x : (c == 1 ? Q.y : Q.z);
             const double wL = Q.w * ar * bc;
@@ -125,6 +125,211 @@ __global__ void __launch_bounds__(64) k_colloc_gh(Geom g, const double4 *__restr
         }
 }
 
+
+// ------------------------------------------------------------------------
+// G and H, regular faces, pair form.  Column vertices are grouped into
+// Morton-ordered patches of kPairPV vertices; a block owns one (patch,
+// frequency, row vertex) and evaluates every face touching the patch once.
+// Two adjacent lanes share a face: per round each evaluates one quadrature
+// point, forms its 3x3 G and H blocks K (7 + 9 complex), and the two swap K
+// with a DPP quad permutation, so both see both points.  Lane A (even)
+// accumulates corner 0 in full and the G part of corner 1; lane B (odd)
+// corner 2 in full and the H part of corner 1 -- each point is evaluated
+// once for all three corner basis functions.  The face sums are added into
+// per-vertex LDS accumulators in sub-steps that the host schedules so that
+// no vertex is updated twice in one sub-step, in face order: deterministic,
+// no atomics.  The block then writes each (row, column) block once.
+// ------------------------------------------------------------------------
+constexpr int kPairPV = 256;                   // vertex slots per patch (64 KB of LDS accumulators)
+constexpr int kPairThreads = 256;
+constexpr int kPairFaces = kPairThreads / 2;  // faces per pass
+constexpr int kPairWords = 32;                // doubles per slot: G (6 sym + beta) then H (9), complex
+constexpr int kFaceWords = 13;                // per face of a pass: 3 corners, normal, area (16 KB per block)
+
+// Swap a double with the neighbouring lane of the pair (DPP quad_perm [1,0,3,2]).
+__device__ inline double pair_swap(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0xB1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+template <bool DG, bool DH>
+__global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, const double4 *__restrict__ quad, int nrounds,
+                                                                  const Freq *__restrict__ freqs, int l0, Mat m, int row0,
+                                                                  const int *__restrict__ patch_pass,
+                                                                  const int2 *__restrict__ pass_info,
+                                                                  const int4 *__restrict__ pass_faces,
+                                                                  const int *__restrict__ patch_verts, cd *__restrict__ UG,
+                                                                  cd *__restrict__ UH, long long ld, long long lstride) {
+    __shared__ double acc[kPairWords * kPairPV];  // [word][slot]
+    __shared__ double fgeo[kFaceWords * kPairFaces];  // [word][face of pass]: re-read every round, not held in VGPRs
+    const int pc = blockIdx.x, l = blockIdx.y, il = blockIdx.z;
+    const int i = row0 + il;
+    const int tid = threadIdx.x, pr = tid >> 1, side = tid & 1;
+    const Freq f = freqs[l0 + l];
+    const double x0 = g.V[3 * i], x1 = g.V[3 * i + 1], x2 = g.V[3 * i + 2];
+    for (int k = tid; k < kPairWords * kPairPV; k += kPairThreads) acc[k] = 0.0;
+
+    for (int pass = patch_pass[pc]; pass < patch_pass[pc + 1]; ++pass) {
+        const int2 info = pass_info[pass];
+        const int nfaces = info.y & 0xffff, nsub = info.y >> 16;
+        // pair-uniform: both lanes of a pair take the same branches around the swaps
+        int4 fe = make_int4(-1, -1, -1, -1);
+        bool active = pr < nfaces;
+        if (active) {
+            fe = pass_faces[info.x + pr];
+            const int a0 = g.F[3 * fe.x], a1 = g.F[3 * fe.x + 1], a2 = g.F[3 * fe.x + 2];
+            active = !(same_vertex(g, i, a0) || same_vertex(g, i, a1) || same_vertex(g, i, a2));
+        }
+        cd accF[16], accXG[7], accXH[9];
+#pragma unroll
+        for (int k = 0; k < 16; k++) accF[k] = cd{0, 0};
+#pragma unroll
+        for (int k = 0; k < 7; k++) accXG[k] = cd{0, 0};
+#pragma unroll
+        for (int k = 0; k < 9; k++) accXH[k] = cd{0, 0};
+        if (active) {
+            const int fc = fe.x;
+            if (side == 0) {
+                const int a0 = g.F[3 * fc], a1 = g.F[3 * fc + 1], a2 = g.F[3 * fc + 2];
+                double *dst = fgeo + pr;
+                for (int k = 0; k < 3; k++) {
+                    dst[k * kPairFaces] = g.V[3 * a0 + k];
+                    dst[(3 + k) * kPairFaces] = g.V[3 * a1 + k];
+                    dst[(6 + k) * kPairFaces] = g.V[3 * a2 + k];
+                    dst[(9 + k) * kPairFaces] = g.N[3 * fc + k];
+                }
+                dst[12 * kPairFaces] = g.area[fc];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (int rd = 0; rd < nrounds; ++rd) {
+                // the face slot is made opaque per round so that the loads below stay in the loop (LDS reads
+                // instead of 13 loop-invariant values held in VGPRs)
+                int slot = pr;
+                asm volatile("" : "+v"(slot));
+                const double *fg = fgeo + slot;
+                const double y00 = fg[0], y01 = fg[kPairFaces], y02 = fg[2 * kPairFaces];
+                const double y10 = fg[3 * kPairFaces], y11 = fg[4 * kPairFaces], y12 = fg[5 * kPairFaces];
+                const double y20 = fg[6 * kPairFaces], y21 = fg[7 * kPairFaces], y22 = fg[8 * kPairFaces];
+                const double ar = fg[12 * kPairFaces];
+                const double4 Q0 = quad[2 * rd], Q1 = quad[2 * rd + 1];
+                const double4 Qo = side ? Q1 : Q0, Qp = side ? Q0 : Q1;
+                const double r0 = Qo.x * y00 + Qo.y * y10 + Qo.z * y20 - x0;
+                const double r1 = Qo.x * y01 + Qo.y * y11 + Qo.z * y21 - x1;
+                const double r2 = Qo.x * y02 + Qo.y * y12 + Qo.z * y22 - x2;
+                const double d2 = r0 * r0 + r1 * r1 + r2 * r2;
+                const double ir = rsqrt_nr(d2), r = d2 * ir, ir2 = ir * ir;
+                const double h[3] = {r0 * ir, r1 * ir, r2 * ir};
+                const double wL = Qo.w * ar;
+                const Expo E = expo(r, f);
+                cd al, be, c1, c2, c4;
+                if (DG) g_coef(E, ir, ir2, m, al, be);
+                if (DH) h_coef(E, ir, ir2, f, m, c1, c2, c4);
+                // corner weights of the own and the partner point: the full corner (0 on lane A, 2 on lane B) and
+                // corner 1, whose G part lane A and H part lane B accumulate (zero weight on the other lane keeps
+                // the code uniform, so each K entry is swapped and consumed at once)
+                const double fo = side ? Qo.z : Qo.x, fp = side ? Qp.z : Qp.x;
+                const double xgo = side ? 0.0 : Qo.y, xgp = side ? 0.0 : Qp.y;
+                const double xho = side ? Qo.y : 0.0, xhp = side ? Qp.y : 0.0;
+                auto consume = [&](const cd &Kk, cd &aF, cd &aX, double xo, double xp) {
+                    const cd Kq{pair_swap(Kk.x), pair_swap(Kk.y)};
+                    fma_cr(aF, Kk, fo);
+                    fma_cr(aF, Kq, fp);
+                    fma_cr(aX, Kk, xo);
+                    fma_cr(aX, Kq, xp);
+                };
+                if (DG) {  // G block: alpha rh rh^T + beta I, packed 00 11 22 01 02 12 then beta
+                    const double wr = wL * ir;
+                    al = al * wr;
+#pragma unroll
+                    for (int a = 0; a < 3; a++)
+#pragma unroll
+                        for (int b = a; b < 3; b++) consume(al * (h[a] * h[b]), accF[sidx(a, b)], accXG[sidx(a, b)], xgo, xgp);
+                    consume(be * wr, accF[6], accXG[6], xgo, xgp);
+                }
+                if (DH) {  // H block (without 1/4pi): rh_a u_b + n_a v_b + d I with u = c4 rn rh + c1 n, v = c2 rh, d = c2 rn
+                    const double n[3] = {fg[9 * kPairFaces], fg[10 * kPairFaces], fg[11 * kPairFaces]};
+                    const double rn = h[0] * n[0] + h[1] * n[1] + h[2] * n[2];
+                    c1 = c1 * wL;
+                    c2 = c2 * wL;
+                    c4 = c4 * (wL * rn);
+                    const cd d = c2 * rn;
+                    cd u[3], v[3];
+#pragma unroll
+                    for (int b = 0; b < 3; b++) {
+                        u[b] = c4 * h[b];
+                        fma_cr(u[b], c1, n[b]);
+                        v[b] = c2 * h[b];
+                    }
+#pragma unroll
+                    for (int a = 0; a < 3; a++)
+#pragma unroll
+                        for (int b = 0; b < 3; b++) {
+                            cd w = u[b] * h[a];
+                            fma_cr(w, v[b], n[a]);
+                            if (a == b) w += d;
+                            consume(w, accF[7 + a * 3 + b], accXH[a * 3 + b], xho, xhp);
+                        }
+                }
+            }
+        }
+        // scheduled updates of the vertex accumulators; corner code: slot | sub-step << 10, -1 = outside the patch
+        const int cF = side ? fe.w : fe.y, cX = fe.z;
+        for (int sub = 0; sub < nsub; ++sub) {
+            __syncthreads();
+            if (!active) continue;
+            if (cF >= 0 && (cF >> 10) == sub) {
+                const int sl = cF & 0x3ff;
+#pragma unroll
+                for (int k = DG ? 0 : 7; k < (DH ? 16 : 7); k++) {
+                    acc[(2 * k) * kPairPV + sl] += accF[k].x;
+                    acc[(2 * k + 1) * kPairPV + sl] += accF[k].y;
+                }
+            }
+            if (cX >= 0 && (cX >> 10) == sub) {
+                const int sl = cX & 0x3ff;
+                if (side == 0) {
+                    if (DG) {
+#pragma unroll
+                        for (int k = 0; k < 7; k++) {
+                            acc[(2 * k) * kPairPV + sl] += accXG[k].x;
+                            acc[(2 * k + 1) * kPairPV + sl] += accXG[k].y;
+                        }
+                    }
+                } else if (DH) {
+#pragma unroll
+                    for (int k = 0; k < 9; k++) {
+                        acc[(14 + 2 * k) * kPairPV + sl] += accXH[k].x;
+                        acc[(15 + 2 * k) * kPairPV + sl] += accXH[k].y;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // one thread per vertex slot writes its (row, column) G and H blocks
+    for (int sl = tid; sl < kPairPV; sl += kPairThreads) {
+        const int j = patch_verts[pc * kPairPV + sl];
+        if (j < 0) continue;
+        auto a = [&](int e) { return acc[e * kPairPV + sl]; };
+        const long long obase = (long long)l * lstride + (long long)(3 * il) * ld + 3 * (long long)j;
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                const int e = sidx(p, b);
+                if (DG) {
+                    cd v{a(2 * e), a(2 * e + 1)};
+                    if (p == b) v += cd{a(12), a(13)};
+                    UG[obase + p * ld + b] = v;
+                }
+                if (DH) UH[obase + p * ld + b] = cd{a(14 + 2 * (p * 3 + b)), a(15 + 2 * (p * 3 + b))} * kInv4Pi;
+            }
+    }
+}
 
 // ------------------------------------------------------------------------
 // G and H, regular faces, patch form.  Column vertices are grouped into
@@ -203,8 +408,8 @@ __global__ void __launch_bounds__(kPatchThreads) k_colloc_gh_patch(Geom g, const
                 const double r0 = Q.x * y00 + Q.y * y10 + Q.z * y20 - x0;
                 const double r1 = Q.x * y01 + Q.y * y11 + Q.z * y21 - x1;
                 const double r2 = Q.x * y02 + Q.y * y12 + Q.z * y22 - x2;
-                const double r = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
-                const double ir = 1.0 / r, ir2 = ir * ir;
+                const double d2 = r0 * r0 + r1 * r1 + r2 * r2;
+                const double ir = rsqrt_nr(d2), r = d2 * ir, ir2 = ir * ir;
                 const double h[3] = {r0 * ir, r1 * ir, r2 * ir};
                 const double w = Q.w * ar;
                 const double wc[3] = {w * Q.x, w * Q.y, w * Q.z};
@@ -300,30 +505,24 @@ __global__ void __launch_bounds__(kPatchThreads) k_colloc_gh_patch(Geom g, const
         }
 }
 
-// Sum of a double over a 256-thread block.
-__device__ inline double block_sum256(double v, double *red) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    const int w = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[w] = v;
-    __syncthreads();
-    return red[0] + red[1] + red[2] + red[3];
-}
-
 // ------------------------------------------------------------------------
 // B_trans and B_angular, regular faces.  Grid: (nl, rows), 256 threads; lanes
-// stride over the faces, then a block reduction.
+// stride over the faces, then a wave shuffle reduction and one LDS pass.
 // ------------------------------------------------------------------------
+constexpr int kBWords = 48;  // B_trans (6 sym complex) + B_angular (18 complex), as doubles
+
 __global__ void __launch_bounds__(256) k_colloc_b(Geom g, const double4 *__restrict__ quad, int npts,
                                                    const Freq *__restrict__ freqs, int l0, Mat m, int row0,
                                                    cd *__restrict__ Bt, cd *__restrict__ Ba, long long nv3) {
-    __shared__ double red[4];
+    __shared__ double red[4][kBWords];
     const int l = blockIdx.x;
     const int i = row0 + blockIdx.y;
     const Freq f = freqs[l0 + l];
     const double x0 = g.V[3 * i], x1 = g.V[3 * i + 1], x2 = g.V[3 * i + 2];
     cd bt[6], ba[18];
+#pragma unroll
     for (int k = 0; k < 6; k++) bt[k] = cd{0, 0};
+#pragma unroll
     for (int k = 0; k < 18; k++) ba[k] = cd{0, 0};
     for (int fc = threadIdx.x; fc < g.nf; fc += 256) {
         const int a0 = g.F[3 * fc], a1 = g.F[3 * fc + 1], a2 = g.F[3 * fc + 2];
@@ -332,14 +531,15 @@ __global__ void __launch_bounds__(256) k_colloc_b(Geom g, const double4 *__restr
         const double n[3] = {g.N[3 * fc], g.N[3 * fc + 1], g.N[3 * fc + 2]};
         const double ar = g.area[fc];
         cd mt[3] = {{0, 0}, {0, 0}, {0, 0}}, m2 = {0, 0}, M[6], S = {0, 0}, S3 = {0, 0};
+#pragma unroll
         for (int k = 0; k < 6; k++) M[k] = cd{0, 0};
         for (int q = 0; q < npts; ++q) {
             const double4 Q = quad[q];
             const double r0 = Q.x * p0[0] + Q.y * p1[0] + Q.z * p2[0] - x0;
             const double r1 = Q.x * p0[1] + Q.y * p1[1] + Q.z * p2[1] - x1;
             const double r2 = Q.x * p0[2] + Q.y * p1[2] + Q.z * p2[2] - x2;
-            const double r = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
-            const double ir = 1.0 / r, ir2 = ir * ir;
+            const double d2 = r0 * r0 + r1 * r1 + r2 * r2;
+            const double ir = rsqrt_nr(d2), r = d2 * ir, ir2 = ir * ir;
             const double h[3] = {r0 * ir, r1 * ir, r2 * ir};
             const double w = Q.w * ar;
             const Expo E = expo(r, f);
@@ -348,20 +548,26 @@ __global__ void __launch_bounds__(256) k_colloc_b(Geom g, const double4 *__restr
             const double rn = h[0] * n[0] + h[1] * n[1] + h[2] * n[2];
             t1 = t1 * w;
             u1 = u1 * w;
+#pragma unroll
             for (int a = 0; a < 3; a++) fma_cr(mt[a], t1, h[a]);
             fma_cr(m2, t2, w * rn);
+#pragma unroll
             for (int a = 0; a < 3; a++)
+#pragma unroll
                 for (int b = a; b < 3; b++) fma_cr(M[sidx(a, b)], u1, h[a] * h[b]);
             fma_cr(S, u12, w);
             fma_cr(S3, u3, w);
         }
+#pragma unroll
         for (int a = 0; a < 3; a++)
+#pragma unroll
             for (int b = a; b < 3; b++) {
                 cd v = {0, 0};
                 fma_cr(v, mt[a], n[b]);
                 fma_cr(v, mt[b], n[a]);
                 if (a == b) v += m2;
                 bt[sidx(a, b)] += v;
+#pragma unroll
                 for (int k = 0; k < 3; k++) {
                     cd u = {0, 0};
                     fma_cr(u, M[sidx(a, k)], n[b]);
@@ -372,18 +578,34 @@ __global__ void __launch_bounds__(256) k_colloc_b(Geom g, const double4 *__restr
                 }
             }
     }
-    // block reduction and symmetric expansion
-    cd rbt[6], rba[18];
-    for (int k = 0; k < 6; k++) rbt[k] = cd{block_sum256(bt[k].x, red), block_sum256(bt[k].y, red)};
-    for (int k = 0; k < 18; k++) rba[k] = cd{block_sum256(ba[k].x, red), block_sum256(ba[k].y, red)};
-    if (threadIdx.x == 0) {
-        cd *bt_out = Bt + (long long)l * nv3 * 3 + (long long)(3 * i) * 3;
-        cd *ba_out = Ba + (long long)l * nv3 * 9 + (long long)(3 * i) * 9;
-        for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++) {
-                bt_out[a * 3 + b] = rbt[sidx(a, b)];
-                for (int k = 0; k < 3; k++) ba_out[a * 9 + b * 3 + k] = rba[sidx(a, b) * 3 + k];
-            }
+    // wave reduction of the 48 partial sums, then the 4 waves through LDS (fixed order)
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < kBWords; k++) {
+        double v = k < 12 ? (k & 1 ? bt[k >> 1].y : bt[k >> 1].x) : (k & 1 ? ba[(k - 12) >> 1].y : ba[(k - 12) >> 1].x);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) red[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kBWords) {
+        const int k = threadIdx.x;
+        const double v = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+        // scatter the packed symmetric entries to their full positions
+        const int c = k & 1;
+        if (k < 12) {
+            const int e = k >> 1;
+            double *bt_out = reinterpret_cast<double *>(Bt + (long long)l * nv3 * 3 + (long long)(3 * i) * 3);
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++)
+                    if (sidx(a, b) == e) bt_out[2 * (a * 3 + b) + c] = v;
+        } else {
+            const int e = (k - 12) >> 1, sab = e / 3, kk = e % 3;
+            double *ba_out = reinterpret_cast<double *>(Ba + (long long)l * nv3 * 9 + (long long)(3 * i) * 9);
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++)
+                    if (sidx(a, b) == sab) ba_out[2 * (a * 9 + b * 3 + kk) + c] = v;
+        }
     }
 }
 
@@ -689,6 +911,26 @@ hipError_t launch_colloc_gh_patch(const Geom &g, const double4 *quad, int npts, 
                            patch_ptr, patch_faces, patch_verts, pass_ptr, gather_ptr, gather_items, UG, UH, ld, lstride);
     return hipGetLastError();
 }
+
+hipError_t launch_colloc_gh_pair(const Geom &g, const double4 *quad, int npts_even, const Freq *freqs, int l0, int nl, const Mat &m,
+                                 int row0, int nrows, const int *patch_pass, const int2 *pass_info, const int4 *pass_faces,
+                                 const int *patch_verts, int npatches, cd *UG, cd *UH, long long ld, long long lstride,
+                                 hipStream_t st) {
+    if ((!UG && !UH) || (npts_even & 1)) return hipErrorInvalidValue;
+    dim3 grid(npatches, nl, nrows);
+    if (UG && UH)
+        hipLaunchKernelGGL((k_colloc_gh_pair<true, true>), grid, dim3(kPairThreads), 0, st, g, quad, npts_even / 2, freqs, l0, m,
+                           row0, patch_pass, pass_info, pass_faces, patch_verts, UG, UH, ld, lstride);
+    else if (UG)
+        hipLaunchKernelGGL((k_colloc_gh_pair<true, false>), grid, dim3(kPairThreads), 0, st, g, quad, npts_even / 2, freqs, l0,
+                           m, row0, patch_pass, pass_info, pass_faces, patch_verts, UG, UH, ld, lstride);
+    else
+        hipLaunchKernelGGL((k_colloc_gh_pair<false, true>), grid, dim3(kPairThreads), 0, st, g, quad, npts_even / 2, freqs, l0,
+                           m, row0, patch_pass, pass_info, pass_faces, patch_verts, UG, UH, ld, lstride);
+    return hipGetLastError();
+}
+int pair_patch_size() { return kPairPV; }
+int pair_pass_faces() { return kPairFaces; }
 
 int patch_size() { return kPatch; }
 int patch_pass_faces() { return kPassFaces; }
