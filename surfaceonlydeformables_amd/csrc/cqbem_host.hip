@@ -289,8 +289,9 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
     }
     // Pair-form patches: vertices in the same Morton order, grown into a patch while it stays within
     // pair_patch_size() vertices and its face list within kPairTargetPasses full passes (a patch lists every
-    // face with a corner in it, in passes of pair_pass_faces() faces).  Within a pass every in-patch corner
-    // gets a sub-step: its rank among the pass's updates of that vertex, in face order.
+    // face with a corner in it, ordered by a vertex-disjoint colouring and cut into passes of pair_pass_faces()
+    // faces).  Within a pass every in-patch corner gets a sub-step: its rank among the pass's updates of that
+    // vertex, in face order.
     {
         constexpr int kPairTargetPasses = 4;
         const int C = pair_patch_size(), PF = pair_pass_faces();
@@ -319,9 +320,30 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
         for (int v = 0; v < P.nv; v++) P.pair_patch_verts[(size_t)patch[v] * C + slot[v]] = v;
         P.pair_patch_pass.assign(1, 0);
         std::vector<int> cnt(C);
+        std::vector<std::vector<int>> used;  // per colour, per slot: taken
         for (int pc = 0; pc < P.pair_npatches; pc++) {
             auto &l = lists[pc];
             std::sort(l.begin(), l.end());
+            // greedy colouring: faces of one colour share no in-patch vertex; ordering the passes by colour keeps
+            // the updates per vertex and pass (the sub-steps) near 2-3 instead of the full valence
+            used.clear();
+            std::vector<std::pair<int, int>> keyed;
+            keyed.reserve(l.size());
+            for (int f : l) {
+                int colour = 0;
+                for (;; colour++) {
+                    if (colour == (int)used.size()) used.emplace_back(C, 0);
+                    bool ok = true;
+                    for (int c = 0; c < 3; c++)
+                        if (patch[F[3 * f + c]] == pc && used[colour][slot[F[3 * f + c]]]) ok = false;
+                    if (ok) break;
+                }
+                for (int c = 0; c < 3; c++)
+                    if (patch[F[3 * f + c]] == pc) used[colour][slot[F[3 * f + c]]] = 1;
+                keyed.emplace_back(colour, f);
+            }
+            std::sort(keyed.begin(), keyed.end());
+            for (size_t k = 0; k < l.size(); k++) l[k] = keyed[k].second;
             for (size_t b = 0; b < l.size(); b += PF) {
                 const int nf = (int)std::min<size_t>(PF, l.size() - b);
                 std::fill(cnt.begin(), cnt.end(), 0);

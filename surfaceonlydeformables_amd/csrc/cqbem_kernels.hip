@@ -278,33 +278,29 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
         }
         // scheduled updates of the vertex accumulators; corner code: slot | sub-step << 10, -1 = outside the patch
         const int cF = side ? fe.w : fe.y, cX = fe.z;
+        // read-modify-write of n consecutive complex words of slot sl: all reads issued before the adds
+        auto rmw = [&](int sl, int w0, const cd *v, int n) {
+            double t[32];
+#pragma unroll
+            for (int k = 0; k < 16; k++)
+                if (k < n) t[2 * k] = acc[(w0 + 2 * k) * kPairPV + sl], t[2 * k + 1] = acc[(w0 + 2 * k + 1) * kPairPV + sl];
+#pragma unroll
+            for (int k = 0; k < 16; k++)
+                if (k < n) acc[(w0 + 2 * k) * kPairPV + sl] = t[2 * k] + v[k].x, acc[(w0 + 2 * k + 1) * kPairPV + sl] = t[2 * k + 1] + v[k].y;
+        };
         for (int sub = 0; sub < nsub; ++sub) {
             __syncthreads();
             if (!active) continue;
             if (cF >= 0 && (cF >> 10) == sub) {
-                const int sl = cF & 0x3ff;
-#pragma unroll
-                for (int k = DG ? 0 : 7; k < (DH ? 16 : 7); k++) {
-                    acc[(2 * k) * kPairPV + sl] += accF[k].x;
-                    acc[(2 * k + 1) * kPairPV + sl] += accF[k].y;
-                }
+                if (DG && DH) rmw(cF & 0x3ff, 0, accF, 16);
+                else if (DG) rmw(cF & 0x3ff, 0, accF, 7);
+                else rmw(cF & 0x3ff, 14, accF + 7, 9);
             }
             if (cX >= 0 && (cX >> 10) == sub) {
-                const int sl = cX & 0x3ff;
                 if (side == 0) {
-                    if (DG) {
-#pragma unroll
-                        for (int k = 0; k < 7; k++) {
-                            acc[(2 * k) * kPairPV + sl] += accXG[k].x;
-                            acc[(2 * k + 1) * kPairPV + sl] += accXG[k].y;
-                        }
-                    }
+                    if (DG) rmw(cX & 0x3ff, 0, accXG, 7);
                 } else if (DH) {
-#pragma unroll
-                    for (int k = 0; k < 9; k++) {
-                        acc[(14 + 2 * k) * kPairPV + sl] += accXH[k].x;
-                        acc[(15 + 2 * k) * kPairPV + sl] += accXH[k].y;
-                    }
+                    rmw(cX & 0x3ff, 14, accXH, 9);
                 }
             }
         }
@@ -816,10 +812,52 @@ __global__ void __launch_bounds__(64) k_colloc_singular(Geom g, const double2 *_
 // T[n][l] = R^-n / L * mult_l * zeta^(-n l) (G.cu:166-179).  One lane per
 // entry, NC weights per pass; the table index is uniform (scalar loads).
 // ------------------------------------------------------------------------
-template <int NC>
+// Inverse CQ transform of one chunk of NC weights n0 .. n0+NC-1: out[n][e] = sum_l Re(X_l(e) T(n, l)), with
+// T(n, l) = (2 or 1) R^-n / L exp(-2 pi i n l / L) (the half-spectrum table).  One lane per entry e; the
+// chunk's table lives in LDS (read as broadcasts) and the spectrum is read LC frequencies at a time so
+// that LC loads per lane are in flight.
+constexpr int kTransformNC = 32, kTransformLC = 8, kTransformMaxLf = 64;
+
+template <int NC, int LC>
 __global__ void __launch_bounds__(256) k_transform(const cd *__restrict__ spec, long long E, int Lf,
                                                     const double2 *__restrict__ tab, int L, int n0,
                                                     double *__restrict__ out, long long out_nstride) {
+    __shared__ double2 T[NC * kTransformMaxLf];  // [l][k]
+    for (int t = threadIdx.x; t < NC * Lf; t += 256) {
+        const int l = t / NC, k = t % NC;
+        T[t] = n0 + k < L ? tab[(long long)(n0 + k) * Lf + l] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const long long stride = (long long)gridDim.x * 256;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < E; e += stride) {
+        double acc[NC];
+#pragma unroll
+        for (int k = 0; k < NC; k++) acc[k] = 0.0;
+        for (int l0 = 0; l0 < Lf; l0 += LC) {
+            cd X[LC];
+#pragma unroll
+            for (int c = 0; c < LC; c++) X[c] = l0 + c < Lf ? spec[(long long)(l0 + c) * E + e] : cd{0.0, 0.0};
+#pragma unroll
+            for (int c = 0; c < LC; c++) {
+                if (l0 + c >= Lf) break;
+                const double2 *Tl = T + (l0 + c) * NC;
+#pragma unroll
+                for (int k = 0; k < NC; k++) acc[k] = fma(X[c].x, Tl[k].x, fma(-X[c].y, Tl[k].y, acc[k]));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NC; k++) {
+            const int n = n0 + k;
+            if (n < L) out[(long long)n * out_nstride + e] = acc[k];
+        }
+    }
+}
+
+// Same for Lf > kTransformMaxLf, with the table read from global memory.
+template <int NC>
+__global__ void __launch_bounds__(256) k_transform_wide(const cd *__restrict__ spec, long long E, int Lf,
+                                                         const double2 *__restrict__ tab, int L, int n0,
+                                                         double *__restrict__ out, long long out_nstride) {
     const long long stride = (long long)gridDim.x * 256;
     for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < E; e += stride) {
         double acc[NC];
@@ -831,8 +869,8 @@ __global__ void __launch_bounds__(256) k_transform(const cd *__restrict__ spec, 
             for (int k = 0; k < NC; k++) {
                 const int n = n0 + k;
                 if (n < L) {
-                    const double2 T = tab[(long long)n * Lf + l];
-                    acc[k] = fma(X.x, T.x, fma(-X.y, T.y, acc[k]));
+                    const double2 Tv = tab[(long long)n * Lf + l];
+                    acc[k] = fma(X.x, Tv.x, fma(-X.y, Tv.y, acc[k]));
                 }
             }
         }
@@ -962,12 +1000,16 @@ hipError_t launch_colloc_singular(int kind, const Geom &g, const double2 *line, 
 hipError_t launch_transform(const cd *spec, long long E, int Lf, const double2 *tab, int L, double *out, long long out_nstride,
                             int num_cus, hipStream_t st) {
     long long blocks = (E + 255) / 256;
-    const long long cap = (long long)num_cus * 16;
+    const long long cap = (long long)num_cus * 64;
     if (blocks > cap) blocks = cap;
     if (blocks < 1) blocks = 1;
-    for (int n0 = 0; n0 < L; n0 += 32) {
-        hipLaunchKernelGGL(k_transform<32>, dim3((unsigned)blocks), dim3(256), 0, st, spec, E, Lf, tab, L, n0, out,
-                           out_nstride);
+    for (int n0 = 0; n0 < L; n0 += kTransformNC) {
+        if (Lf <= kTransformMaxLf)
+            hipLaunchKernelGGL((k_transform<kTransformNC, kTransformLC>), dim3((unsigned)blocks), dim3(256), 0, st, spec, E, Lf,
+                               tab, L, n0, out, out_nstride);
+        else
+            hipLaunchKernelGGL((k_transform_wide<kTransformNC>), dim3((unsigned)blocks), dim3(256), 0, st, spec, E, Lf, tab, L,
+                               n0, out, out_nstride);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
