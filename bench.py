@@ -144,7 +144,7 @@ def main():
             "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
             "launch_ms": gh_ms / max(gh_launches, 1),
             "flops_per_launch": FLOPS_PER_EVAL_GH * gh_evals / max(gh_launches, 1)}
-    prof = os.path.join(ROOT, "profiles", "k_colloc_gh_icosphere_l4_traffic.json")
+    prof = os.path.join(ROOT, "profiles", "k_colloc_gh_pair_icosphere_l4_traffic.json")
     if os.path.exists(prof):
         with open(prof) as fh:
             roof["traffic"] = json.load(fh).get("hbm_bytes_per_launch")
