@@ -158,6 +158,7 @@ struct Prepared {
     int pair_npatches = 0;
     int npts = 0;               // points of the composite triangle rule
     std::vector<double4> quad;  // the rule, padded to an even count with a zero-weight point
+    std::vector<double4> pair_pts;  // per padded point: (b0, b1, b2, w), (phi_F own, phi_F partner, 0, 0)
     std::vector<double2> line;
     std::vector<Freq> freqs;
     std::vector<double2> tab;
@@ -373,6 +374,12 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
     for (const TriPoint &t : composite_tri_rule(p->order, p->subdivision)) P.quad.push_back(make_double4(t.b[0], t.b[1], t.b[2], t.w));
     P.npts = (int)P.quad.size();
     if (P.quad.size() & 1) P.quad.push_back(make_double4(1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 0.0));
+    // pair-kernel point table: even points go to lane A (corner 0), odd points to lane B (corner 2)
+    for (size_t q = 0; q < P.quad.size(); q++) {
+        const double4 &Q = P.quad[q], &Qp = P.quad[q ^ 1];
+        P.pair_pts.push_back(Q);
+        P.pair_pts.push_back((q & 1) ? make_double4(Q.z, Qp.z, 0.0, 0.0) : make_double4(Q.x, Qp.x, 0.0, 0.0));
+    }
     for (auto &xw : composite_line01(p->order, p->subdivision)) P.line.push_back(make_double2(xw.first, xw.second));
     const auto s = cq_frequencies(p->L, p->R, p->dt, p->bdf);
     for (auto sl : s) {
@@ -395,7 +402,7 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
 struct DevMesh {
     DBuf V, F, N, area, vs_ptr, vs_face, sg_ptr, sg_face, vmap, quad, line, freqs, tab, patch_ptr, patch_faces, patch_verts;
     DBuf pass_ptr, gather_ptr, gather_items;
-    DBuf pair_patch_pass, pair_pass_info, pair_pass_faces, pair_patch_verts;
+    DBuf pair_patch_pass, pair_pass_info, pair_pass_faces, pair_patch_verts, pair_pts;
     Geom g;
 };
 
@@ -419,7 +426,7 @@ hipError_t launch_gh(const DevMesh &d, const Prepared &P, const Freq *freqs, int
     const double4 *quad = d.quad.as<double4>();
     const int npts = P.npts;
     if (var == kVarPair)
-        return launch_colloc_gh_pair(d.g, quad, (int)P.quad.size(), freqs, l0, nl, P.mat, row0, nrows,
+        return launch_colloc_gh_pair(d.g, d.pair_pts.as<double4>(), (int)P.quad.size(), freqs, l0, nl, P.mat, row0, nrows,
                                      d.pair_patch_pass.as<int>(), d.pair_pass_info.as<int2>(), d.pair_pass_faces.as<int4>(),
                                      d.pair_patch_verts.as<int>(), P.pair_npatches, UG, UH, ld, lstride, st);
     if (var == kVarStar) return launch_colloc_gh(d.g, quad, npts, freqs, l0, nl, P.mat, row0, nrows, UG, UH, ld, lstride, st);
@@ -454,6 +461,7 @@ void upload_mesh(DevMesh &d, const cqbem_mesh *mesh, const Prepared &P, hipStrea
     upload(d.pass_ptr, P.pass_ptr, st), upload(d.gather_ptr, P.gather_ptr, st), upload(d.gather_items, P.gather_items, st);
     upload(d.pair_patch_pass, P.pair_patch_pass, st), upload(d.pair_pass_info, P.pair_pass_info, st);
     upload(d.pair_pass_faces, P.pair_pass_faces, st), upload(d.pair_patch_verts, P.pair_patch_verts, st);
+    upload(d.pair_pts, P.pair_pts, st);
     Geom &g = d.g;
     g.V = d.V.as<double>(), g.F = d.F.as<int>(), g.N = d.N.as<double>(), g.area = d.area.as<double>();
     g.vs_ptr = d.vs_ptr.as<int>(), g.vs_face = d.vs_face.as<int>(), g.sg_ptr = d.sg_ptr.as<int>(), g.sg_face = d.sg_face.as<int>();

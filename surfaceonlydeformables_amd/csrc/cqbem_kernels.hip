@@ -133,9 +133,10 @@ __global__ void __launch_bounds__(64) k_colloc_gh(Geom g, const double4 *__restr
 // Two adjacent lanes share a face: per round each evaluates one quadrature
 // point, forms its 3x3 G and H blocks K (7 + 9 complex), and the two swap K
 // with a DPP quad permutation, so both see both points.  Lane A (even)
-// accumulates corner 0 in full and the G part of corner 1; lane B (odd)
-// corner 2 in full and the H part of corner 1 -- each point is evaluated
-// once for all three corner basis functions.  The face sums are added into
+// accumulates corner 0 and lane B (odd) corner 2 over both points; corner 1
+// is summed over each lane's own points and the two halves are added after
+// the face (lane A keeps its G part, lane B its H part) -- each point is
+// evaluated once for all three corner basis functions.  The face sums are added into
 // per-vertex LDS accumulators in sub-steps that the host schedules so that
 // no vertex is updated twice in one sub-step, in face order: deterministic,
 // no atomics.  The block then writes each (row, column) block once.
@@ -155,7 +156,7 @@ __device__ inline double pair_swap(double v) {
 }
 
 template <bool DG, bool DH>
-__global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, const double4 *__restrict__ quad, int nrounds,
+__global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, const double4 *__restrict__ pts, int nrounds,
                                                                   const Freq *__restrict__ freqs, int l0, Mat m, int row0,
                                                                   const int *__restrict__ patch_pass,
                                                                   const int2 *__restrict__ pass_info,
@@ -182,13 +183,9 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
             const int a0 = g.F[3 * fe.x], a1 = g.F[3 * fe.x + 1], a2 = g.F[3 * fe.x + 2];
             active = !(same_vertex(g, i, a0) || same_vertex(g, i, a1) || same_vertex(g, i, a2));
         }
-        cd accF[16], accXG[7], accXH[9];
+        cd accF[16], accX[16];
 #pragma unroll
-        for (int k = 0; k < 16; k++) accF[k] = cd{0, 0};
-#pragma unroll
-        for (int k = 0; k < 7; k++) accXG[k] = cd{0, 0};
-#pragma unroll
-        for (int k = 0; k < 9; k++) accXH[k] = cd{0, 0};
+        for (int k = 0; k < 16; k++) accF[k] = cd{0, 0}, accX[k] = cd{0, 0};
         if (active) {
             const int fc = fe.x;
             if (side == 0) {
@@ -205,6 +202,7 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            double4 qcur = pts[2 * side], wcur = pts[2 * side + 1];  // next round's point, prefetched
             for (int rd = 0; rd < nrounds; ++rd) {
                 // the face slot is made opaque per round so that the loads below stay in the loop (LDS reads
                 // instead of 13 loop-invariant values held in VGPRs)
@@ -215,8 +213,8 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
                 const double y10 = fg[3 * kPairFaces], y11 = fg[4 * kPairFaces], y12 = fg[5 * kPairFaces];
                 const double y20 = fg[6 * kPairFaces], y21 = fg[7 * kPairFaces], y22 = fg[8 * kPairFaces];
                 const double ar = fg[12 * kPairFaces];
-                const double4 Q0 = quad[2 * rd], Q1 = quad[2 * rd + 1];
-                const double4 Qo = side ? Q1 : Q0, Qp = side ? Q0 : Q1;
+                const double4 Qo = qcur, Wt = wcur;  // this lane's point and its corner weights
+                if (rd + 1 < nrounds) qcur = pts[4 * (rd + 1) + 2 * side], wcur = pts[4 * (rd + 1) + 2 * side + 1];
                 const double r0 = Qo.x * y00 + Qo.y * y10 + Qo.z * y20 - x0;
                 const double r1 = Qo.x * y01 + Qo.y * y11 + Qo.z * y21 - x1;
                 const double r2 = Qo.x * y02 + Qo.y * y12 + Qo.z * y22 - x2;
@@ -228,18 +226,14 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
                 cd al, be, c1, c2, c4;
                 if (DG) g_coef(E, ir, ir2, m, al, be);
                 if (DH) h_coef(E, ir, ir2, f, m, c1, c2, c4);
-                // corner weights of the own and the partner point: the full corner (0 on lane A, 2 on lane B) and
-                // corner 1, whose G part lane A and H part lane B accumulate (zero weight on the other lane keeps
-                // the code uniform, so each K entry is swapped and consumed at once)
-                const double fo = side ? Qo.z : Qo.x, fp = side ? Qp.z : Qp.x;
-                const double xgo = side ? 0.0 : Qo.y, xgp = side ? 0.0 : Qp.y;
-                const double xho = side ? Qo.y : 0.0, xhp = side ? Qp.y : 0.0;
-                auto consume = [&](const cd &Kk, cd &aF, cd &aX, double xo, double xp) {
+                // corner weights: the full corner (0 on lane A, 2 on lane B) takes the own and the partner point;
+                // corner 1 takes the own point only, and the pair adds its two halves after the face
+                const double fo = Wt.x, fp = Wt.y, xo = Qo.y;
+                auto consume = [&](const cd &Kk, cd &aF, cd &aX) {
                     const cd Kq{pair_swap(Kk.x), pair_swap(Kk.y)};
                     fma_cr(aF, Kk, fo);
                     fma_cr(aF, Kq, fp);
                     fma_cr(aX, Kk, xo);
-                    fma_cr(aX, Kq, xp);
                 };
                 if (DG) {  // G block: alpha rh rh^T + beta I, packed 00 11 22 01 02 12 then beta
                     const double wr = wL * ir;
@@ -247,8 +241,8 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
 #pragma unroll
                     for (int a = 0; a < 3; a++)
 #pragma unroll
-                        for (int b = a; b < 3; b++) consume(al * (h[a] * h[b]), accF[sidx(a, b)], accXG[sidx(a, b)], xgo, xgp);
-                    consume(be * wr, accF[6], accXG[6], xgo, xgp);
+                        for (int b = a; b < 3; b++) consume(al * (h[a] * h[b]), accF[sidx(a, b)], accX[sidx(a, b)]);
+                    consume(be * wr, accF[6], accX[6]);
                 }
                 if (DH) {  // H block (without 1/4pi): rh_a u_b + n_a v_b + d I with u = c4 rn rh + c1 n, v = c2 rh, d = c2 rn
                     const double n[3] = {fg[9 * kPairFaces], fg[10 * kPairFaces], fg[11 * kPairFaces]};
@@ -271,10 +265,12 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
                             cd w = u[b] * h[a];
                             fma_cr(w, v[b], n[a]);
                             if (a == b) w += d;
-                            consume(w, accF[7 + a * 3 + b], accXH[a * 3 + b], xho, xhp);
+                            consume(w, accF[7 + a * 3 + b], accX[7 + a * 3 + b]);
                         }
                 }
             }
+#pragma unroll
+            for (int k = 0; k < 16; k++) accX[k] += cd{pair_swap(accX[k].x), pair_swap(accX[k].y)};
         }
         // scheduled updates of the vertex accumulators; corner code: slot | sub-step << 10, -1 = outside the patch
         const int cF = side ? fe.w : fe.y, cX = fe.z;
@@ -298,9 +294,9 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
             }
             if (cX >= 0 && (cX >> 10) == sub) {
                 if (side == 0) {
-                    if (DG) rmw(cX & 0x3ff, 0, accXG, 7);
+                    if (DG) rmw(cX & 0x3ff, 0, accX, 7);
                 } else if (DH) {
-                    rmw(cX & 0x3ff, 14, accXH, 9);
+                    rmw(cX & 0x3ff, 14, accX + 7, 9);
                 }
             }
         }
@@ -950,20 +946,20 @@ hipError_t launch_colloc_gh_patch(const Geom &g, const double4 *quad, int npts, 
     return hipGetLastError();
 }
 
-hipError_t launch_colloc_gh_pair(const Geom &g, const double4 *quad, int npts_even, const Freq *freqs, int l0, int nl, const Mat &m,
+hipError_t launch_colloc_gh_pair(const Geom &g, const double4 *pts, int npts_even, const Freq *freqs, int l0, int nl, const Mat &m,
                                  int row0, int nrows, const int *patch_pass, const int2 *pass_info, const int4 *pass_faces,
                                  const int *patch_verts, int npatches, cd *UG, cd *UH, long long ld, long long lstride,
                                  hipStream_t st) {
     if ((!UG && !UH) || (npts_even & 1)) return hipErrorInvalidValue;
     dim3 grid(npatches, nl, nrows);
     if (UG && UH)
-        hipLaunchKernelGGL((k_colloc_gh_pair<true, true>), grid, dim3(kPairThreads), 0, st, g, quad, npts_even / 2, freqs, l0, m,
+        hipLaunchKernelGGL((k_colloc_gh_pair<true, true>), grid, dim3(kPairThreads), 0, st, g, pts, npts_even / 2, freqs, l0, m,
                            row0, patch_pass, pass_info, pass_faces, patch_verts, UG, UH, ld, lstride);
     else if (UG)
-        hipLaunchKernelGGL((k_colloc_gh_pair<true, false>), grid, dim3(kPairThreads), 0, st, g, quad, npts_even / 2, freqs, l0,
+        hipLaunchKernelGGL((k_colloc_gh_pair<true, false>), grid, dim3(kPairThreads), 0, st, g, pts, npts_even / 2, freqs, l0,
                            m, row0, patch_pass, pass_info, pass_faces, patch_verts, UG, UH, ld, lstride);
     else
-        hipLaunchKernelGGL((k_colloc_gh_pair<false, true>), grid, dim3(kPairThreads), 0, st, g, quad, npts_even / 2, freqs, l0,
+        hipLaunchKernelGGL((k_colloc_gh_pair<false, true>), grid, dim3(kPairThreads), 0, st, g, pts, npts_even / 2, freqs, l0,
                            m, row0, patch_pass, pass_info, pass_faces, patch_verts, UG, UH, ld, lstride);
     return hipGetLastError();
 }

@@ -29,8 +29,9 @@ hipError_t launch_colloc_gh_patch(const Geom &g, const double4 *quad, int npts, 
 // Pair form of the G/H regular-face kernel (both matrices): per patch a range of passes
 // [patch_pass[p], patch_pass[p+1]); per pass (first face entry, nfaces | nsub << 16); per face entry
 // (face, corner 0, corner 1, corner 2) with corner = slot | sub-step << 10 or -1 outside the patch.
-// The quadrature table holds npts_even points (zero-weight padding).
-hipError_t launch_colloc_gh_pair(const Geom &g, const double4 *quad, int npts_even, const Freq *freqs, int l0, int nl, const Mat &m,
+// `pts` holds, for each point q of the (even-padded) rule, (b0, b1, b2, w) and (phi_F(q), phi_F(partner), 0, 0)
+// with phi_F the barycentric of corner 0 for even q (lane A) and of corner 2 for odd q (lane B), partner = q ^ 1.
+hipError_t launch_colloc_gh_pair(const Geom &g, const double4 *pts, int npts_even, const Freq *freqs, int l0, int nl, const Mat &m,
                                  int row0, int nrows, const int *patch_pass, const int2 *pass_info, const int4 *pass_faces,
                                  const int *patch_verts, int npatches, cd *UG, cd *UH, long long ld, long long lstride,
                                  hipStream_t st);
