@@ -146,11 +146,6 @@ struct Prepared {
     int nv, nf, L, Lf;
     std::vector<double> N, area;
     std::vector<int> vs_ptr, vs_face, sg_ptr, sg_face;
-    std::vector<int> patch_ptr, patch_verts;  // Morton-ordered column-vertex patches
-    std::vector<int2> patch_faces;            // (face, packed in-patch corner slots, 0xff = outside)
-    std::vector<int> pass_ptr, gather_ptr;    // per patch: first pass; per (pass, slot): CSR into gather_items
-    std::vector<unsigned char> gather_items;  // 3 * face-in-pass + corner, in face order
-    int npatches = 0;
     // pair form: patches of pair_patch_size() vertices, passes of pair_pass_faces() faces
     std::vector<int> pair_patch_pass, pair_patch_verts;
     std::vector<int2> pair_pass_info;   // (first face entry, nfaces | nsub << 16)
@@ -213,10 +208,9 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
         for (int i = 0; i < P.nv; i++) P.sg_ptr[i + 1] = P.sg_ptr[i] + (int)lists[i].size();
         for (auto &l : lists) P.sg_face.insert(P.sg_face.end(), l.begin(), l.end());
     }
-    // Column-vertex patches: vertices sorted by the Morton code of their position, cut into runs of
-    // patch_size(); each patch lists every face with a corner in it.
+    // Column vertices in the Morton order of their positions (21 bits per axis over the bounding box).
+    std::vector<int> order(P.nv);
     {
-        const int C = patch_size();
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
         for (int v = 0; v < P.nv; v++)
             for (int k = 0; k < 3; k++) lo[k] = std::min(lo[k], V[3 * v + k]), hi[k] = std::max(hi[k], V[3 * v + k]);
@@ -237,56 +231,7 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
             key[v] = {spread(c[0]) | spread(c[1]) << 1 | spread(c[2]) << 2, v};
         }
         std::sort(key.begin(), key.end());
-        P.npatches = (P.nv + C - 1) / C;
-        P.patch_verts.assign((size_t)P.npatches * C, -1);
-        std::vector<int> slot(P.nv), patch(P.nv);
-        for (int r = 0; r < P.nv; r++) {
-            const int v = key[r].second;
-            P.patch_verts[r] = v, patch[v] = r / C, slot[v] = r % C;
-        }
-        std::vector<std::vector<int2>> lists(P.npatches);
-        for (int f = 0; f < P.nf; f++) {
-            int ps[3] = {patch[F[3 * f]], patch[F[3 * f + 1]], patch[F[3 * f + 2]]};
-            for (int k = 0; k < 3; k++) {
-                bool seen = false;
-                for (int k2 = 0; k2 < k; k2++) seen |= ps[k2] == ps[k];
-                if (seen) continue;
-                int packed = 0;
-                for (int c = 0; c < 3; c++) packed |= (ps[c] == ps[k] ? slot[F[3 * f + c]] : 0xff) << (8 * c);
-                lists[ps[k]].push_back(make_int2(f, packed));
-            }
-        }
-        P.patch_ptr.assign(1, 0);
-        for (auto &l : lists) {
-            P.patch_faces.insert(P.patch_faces.end(), l.begin(), l.end());
-            P.patch_ptr.push_back((int)P.patch_faces.size());
-        }
-        // gather lists: for each pass of patch_pass_faces() faces, one row of C+1 ints (row start, then
-        // per-slot ends) into gather_items, which names each slot's staged (face, corner) entries in face order
-        const int PF = patch_pass_faces();
-        int npass = 0;
-        for (int pc = 0; pc < P.npatches; pc++) {
-            P.pass_ptr.push_back(npass);
-            const int nfc = P.patch_ptr[pc + 1] - P.patch_ptr[pc];
-            for (int b = 0; b < nfc; b += PF, npass++) {
-                std::vector<std::vector<unsigned char>> per(C);
-                for (int fi = 0; fi < PF && b + fi < nfc; fi++) {
-                    const int packed = P.patch_faces[P.patch_ptr[pc] + b + fi].y;
-                    for (int k = 0; k < 3; k++) {
-                        const int sl = (packed >> (8 * k)) & 0xff;
-                        if (sl != 0xff) per[sl].push_back((unsigned char)(3 * fi + k));
-                    }
-                }
-                P.gather_ptr.push_back((int)P.gather_items.size());
-                for (int sl = 0; sl < C; sl++) {
-                    P.gather_items.insert(P.gather_items.end(), per[sl].begin(), per[sl].end());
-                    P.gather_ptr.push_back((int)P.gather_items.size());
-                }
-            }
-        }
-        P.pass_ptr.push_back(npass);
-        if (P.gather_ptr.empty()) P.gather_ptr.assign(C + 1, 0);
-        if (P.gather_items.empty()) P.gather_items.push_back(0);
+        for (int r = 0; r < P.nv; r++) order[r] = key[r].second;
     }
     // Pair-form patches: vertices in the same Morton order, grown into a patch while it stays within
     // pair_patch_size() vertices and its face list within kPairTargetPasses full passes (a patch lists every
@@ -296,8 +241,6 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
     {
         constexpr int kPairTargetPasses = 4;
         const int C = pair_patch_size(), PF = pair_pass_faces();
-        std::vector<int> order(P.patch_verts.begin(), P.patch_verts.end());
-        order.erase(std::remove(order.begin(), order.end(), -1), order.end());
         std::vector<int> slot(P.nv), patch(P.nv), mark(P.nf, -1);
         std::vector<std::vector<int>> lists;
         for (int r = 0; r < P.nv;) {
@@ -400,49 +343,26 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
 }
 
 struct DevMesh {
-    DBuf V, F, N, area, vs_ptr, vs_face, sg_ptr, sg_face, vmap, quad, line, freqs, tab, patch_ptr, patch_faces, patch_verts;
-    DBuf pass_ptr, gather_ptr, gather_items;
+    DBuf V, F, N, area, vs_ptr, vs_face, sg_ptr, sg_face, vmap, quad, line, freqs, tab;
     DBuf pair_patch_pass, pair_pass_info, pair_pass_faces, pair_patch_verts, pair_pts;
     Geom g;
 };
 
-// Collocation G/H regular-face kernel variant: "pair" (default: Morton patches, each face evaluated
-// once per patch by a lane pair), "star" (one lane per column vertex over its vertex star), "patch" or
-// "patch_split" (Morton patches, two lanes per face holding all three corners); CQBEM_COLLOC_KERNEL
-// overrides.
-enum { kVarPatch = 0, kVarPatchSplit = 1, kVarStar = 2, kVarPair = 3 };
-int colloc_variant() {
+// Collocation G/H regular-face kernel variant: "pair" (default: Morton patches, each face evaluated once
+// per patch by a lane pair) or "star" (one lane per column vertex over its vertex star, each face
+// evaluated once per corner); CQBEM_COLLOC_KERNEL=star selects the latter.
+bool colloc_star() {
     const char *e = std::getenv("CQBEM_COLLOC_KERNEL");
-    if (!e) return kVarPair;
-    if (!std::strcmp(e, "patch")) return kVarPatch;
-    if (!std::strcmp(e, "patch_split")) return kVarPatchSplit;
-    if (!std::strcmp(e, "star")) return kVarStar;
-    return kVarPair;
+    return e && !std::strcmp(e, "star");
 }
 
 hipError_t launch_gh(const DevMesh &d, const Prepared &P, const Freq *freqs, int l0, int nl, int row0, int nrows, cd *UG,
                      cd *UH, long long ld, long long lstride, hipStream_t st) {
-    int var = colloc_variant();
-    const double4 *quad = d.quad.as<double4>();
-    const int npts = P.npts;
-    if (var == kVarPair)
-        return launch_colloc_gh_pair(d.g, d.pair_pts.as<double4>(), (int)P.quad.size(), freqs, l0, nl, P.mat, row0, nrows,
-                                     d.pair_patch_pass.as<int>(), d.pair_pass_info.as<int2>(), d.pair_pass_faces.as<int4>(),
-                                     d.pair_patch_verts.as<int>(), P.pair_npatches, UG, UH, ld, lstride, st);
-    if (var == kVarStar) return launch_colloc_gh(d.g, quad, npts, freqs, l0, nl, P.mat, row0, nrows, UG, UH, ld, lstride, st);
-    const int *pp = d.patch_ptr.as<int>(), *pv = d.patch_verts.as<int>();
-    const int2 *pf = d.patch_faces.as<int2>();
-    const int *psp = d.pass_ptr.as<int>(), *gp = d.gather_ptr.as<int>();
-    const unsigned char *gi = d.gather_items.as<unsigned char>();
-    if (var == kVarPatchSplit && UG && UH) {
-        hipError_t e = launch_colloc_gh_patch(d.g, quad, npts, freqs, l0, nl, P.mat, row0, nrows, pp, pf, pv, psp, gp, gi, P.npatches, UG,
-                                              nullptr, ld, lstride, st);
-        if (e != hipSuccess) return e;
-        return launch_colloc_gh_patch(d.g, quad, npts, freqs, l0, nl, P.mat, row0, nrows, pp, pf, pv, psp, gp, gi, P.npatches, nullptr, UH,
-                                      ld, lstride, st);
-    }
-    return launch_colloc_gh_patch(d.g, quad, npts, freqs, l0, nl, P.mat, row0, nrows, pp, pf, pv, psp, gp, gi, P.npatches, UG, UH, ld,
-                                  lstride, st);
+    if (colloc_star())
+        return launch_colloc_gh(d.g, d.quad.as<double4>(), P.npts, freqs, l0, nl, P.mat, row0, nrows, UG, UH, ld, lstride, st);
+    return launch_colloc_gh_pair(d.g, d.pair_pts.as<double4>(), (int)P.quad.size(), freqs, l0, nl, P.mat, row0, nrows,
+                                 d.pair_patch_pass.as<int>(), d.pair_pass_info.as<int2>(), d.pair_pass_faces.as<int4>(),
+                                 d.pair_patch_verts.as<int>(), P.pair_npatches, UG, UH, ld, lstride, st);
 }
 
 void upload_mesh(DevMesh &d, const cqbem_mesh *mesh, const Prepared &P, hipStream_t st) {
@@ -457,8 +377,6 @@ void upload_mesh(DevMesh &d, const cqbem_mesh *mesh, const Prepared &P, hipStrea
     upload(d.N, P.N, st), upload(d.area, P.area, st), upload(d.vs_ptr, P.vs_ptr, st), upload(d.vs_face, P.vs_face, st);
     upload(d.sg_ptr, P.sg_ptr, st), upload(d.sg_face, P.sg_face, st), upload(d.quad, P.quad, st), upload(d.line, P.line, st);
     upload(d.freqs, P.freqs, st), upload(d.tab, P.tab, st);
-    upload(d.patch_ptr, P.patch_ptr, st), upload(d.patch_faces, P.patch_faces, st), upload(d.patch_verts, P.patch_verts, st);
-    upload(d.pass_ptr, P.pass_ptr, st), upload(d.gather_ptr, P.gather_ptr, st), upload(d.gather_items, P.gather_items, st);
     upload(d.pair_patch_pass, P.pair_patch_pass, st), upload(d.pair_pass_info, P.pair_pass_info, st);
     upload(d.pair_pass_faces, P.pair_pass_faces, st), upload(d.pair_patch_verts, P.pair_patch_verts, st);
     upload(d.pair_pts, P.pair_pts, st);
@@ -505,9 +423,28 @@ uint64_t footprint(const cqbem_mesh *mesh, const cqbem_params *p, int outputs_on
     const int nmat = ((p->which & CQBEM_G) ? 1 : 0) + ((p->which & CQBEM_H) ? 1 : 0);
     uint64_t bytes = 0;
     if (nmat) {
-        const int rows = slab_rows(nv, Lf, nmat, slab_budget);
-        bytes += (uint64_t)nmat * Lf * 3 * (uint64_t)rows * 3 * nv * sizeof(cd);
+        if (p->galerkin) {  // full-matrix spectra
+            bytes += (uint64_t)nmat * Lf * 9 * (uint64_t)nv * nv * sizeof(cd);
+        } else {
+            const int rows = slab_rows(nv, Lf, nmat, slab_budget);
+            bytes += (uint64_t)nmat * Lf * 3 * (uint64_t)rows * 3 * nv * sizeof(cd);
+        }
         if (!outputs_on_device) bytes += (uint64_t)nmat * matrix_bytes(nv, L);
+    }
+    if (p->galerkin) {
+        // touching-pair scratch: one 270-complex block per ordered pair of faces sharing a vertex and frequency
+        std::vector<std::vector<int>> star(nv);
+        for (int64_t f = 0; f < mesh->nf; f++)
+            for (int c = 0; c < 3; c++) star[mesh->F[3 * f + c]].push_back((int)f);
+        uint64_t npairs = 0;
+        std::vector<int> cand;
+        for (int64_t f = 0; f < mesh->nf; f++) {
+            cand.clear();
+            for (int c = 0; c < 3; c++) cand.insert(cand.end(), star[mesh->F[3 * f + c]].begin(), star[mesh->F[3 * f + c]].end());
+            std::sort(cand.begin(), cand.end());
+            npairs += (uint64_t)(std::unique(cand.begin(), cand.end()) - cand.begin());
+        }
+        bytes += npairs * ((uint64_t)Lf * 270 * sizeof(cd) + 256);
     }
     bytes += (uint64_t)Lf * 3 * nv * 12 * sizeof(cd) + (uint64_t)L * 3 * nv * 18 * sizeof(double);
     bytes += (uint64_t)mesh->nf * 64 + (uint64_t)nv * 64 + (1 << 20);

@@ -324,180 +324,6 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
 }
 
 // ------------------------------------------------------------------------
-// G and H, regular faces, patch form.  Column vertices are grouped into
-// Morton-ordered patches of kPatch vertices; a 128-lane block owns one
-// (patch, frequency, row vertex) and evaluates every face touching the patch
-// once (faces on a patch boundary are evaluated by each patch they touch,
-// ~1.26x on an icosphere, instead of once per corner as in k_colloc_gh).  Two
-// lanes share a face (alternate quadrature points) and combine their
-// per-corner sums with one shuffle; each pass of 64 faces stages its corner
-// sums in LDS and every vertex slot's lane adds the entries that name it, in
-// face order -- deterministic, no atomics.  The block then writes each
-// (row, column) block once.
-// ------------------------------------------------------------------------
-// 120 slots keep acc + stage within 80 KB, so two blocks (8 waves) share a CU.
-constexpr int kPatch = 120;
-constexpr int kPatchThreads = 128;
-constexpr int kPassFaces = kPatchThreads / 2;
-constexpr int kSlotWords = 32;  // 16 complex per vertex slot: G (6 sym + beta), H (9)
-
-template <bool DG, bool DH>
-__global__ void __launch_bounds__(kPatchThreads) k_colloc_gh_patch(Geom g, const double4 *__restrict__ quad, int npts,
-                                                                    const Freq *__restrict__ freqs, int l0, Mat m, int row0,
-                                                                    const int *__restrict__ patch_ptr,
-                                                                    const int2 *__restrict__ patch_faces,
-                                                                    const int *__restrict__ patch_verts,
-                                                                    const int *__restrict__ pass_ptr,
-                                                                    const int *__restrict__ gather_ptr,
-                                                                    const unsigned char *__restrict__ gather_items,
-                                                                    cd *__restrict__ UG, cd *__restrict__ UH, long long ld,
-                                                                    long long lstride) {
-    __shared__ double acc[kPatch * kSlotWords];
-    __shared__ double stage[kPassFaces * 3 * kSlotWords];
-    const int pc = blockIdx.x, l = blockIdx.y, il = blockIdx.z;
-    const int i = row0 + il;
-    const int tid = threadIdx.x;
-    const Freq f = freqs[l0 + l];
-    const double x0 = g.V[3 * i], x1 = g.V[3 * i + 1], x2 = g.V[3 * i + 2];
-    for (int k = tid; k < kPatch * kSlotWords; k += kPatchThreads) acc[k] = 0.0;
-    __syncthreads();
-    const int e0 = patch_ptr[pc], nitems = 2 * (patch_ptr[pc + 1] - e0);
-    int pass = pass_ptr[pc];
-    for (int base = 0; base < nitems; base += kPatchThreads, pass++) {
-        const int it = base + tid;
-        const bool valid = it < nitems;
-        const int part = it & 1;
-        int2 ent = make_int2(0, 0x00ffffff);
-        bool live = false;
-        if (valid) {
-            ent = patch_faces[e0 + (it >> 1)];
-            const int a0 = g.F[3 * ent.x], a1 = g.F[3 * ent.x + 1], a2 = g.F[3 * ent.x + 2];
-            live = !(same_vertex(g, i, a0) || same_vertex(g, i, a1) || same_vertex(g, i, a2));
-        }
-        cd ga[3][6], gb[3], hh[3][9];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            gb[k] = cd{0, 0};
-#pragma unroll
-            for (int e = 0; e < 6; e++) ga[k][e] = cd{0, 0};
-#pragma unroll
-            for (int e = 0; e < 9; e++) hh[k][e] = cd{0, 0};
-        }
-        if (live) {
-            const int fc = ent.x;
-            const double *p0 = g.V + 3 * g.F[3 * fc], *p1 = g.V + 3 * g.F[3 * fc + 1], *p2 = g.V + 3 * g.F[3 * fc + 2];
-            const double y00 = p0[0], y01 = p0[1], y02 = p0[2];
-            const double y10 = p1[0], y11 = p1[1], y12 = p1[2];
-            const double y20 = p2[0], y21 = p2[1], y22 = p2[2];
-            const double n[3] = {g.N[3 * fc], g.N[3 * fc + 1], g.N[3 * fc + 2]};
-            const double ar = g.area[fc];
-            for (int q2 = 0; q2 < npts; q2 += 2) {
-                // the two lanes of a face take points q2 and q2+1; both loads are uniform (scalar)
-                const double4 Qa = quad[q2];
-                const double4 Qb = quad[q2 + 1 < npts ? q2 + 1 : q2];
-                if (part == 1 && q2 + 1 >= npts) break;
-                const double4 Q = part ? Qb : Qa;
-                const double r0 = Q.x * y00 + Q.y * y10 + Q.z * y20 - x0;
-                const double r1 = Q.x * y01 + Q.y * y11 + Q.z * y21 - x1;
-                const double r2 = Q.x * y02 + Q.y * y12 + Q.z * y22 - x2;
-                const double d2 = r0 * r0 + r1 * r1 + r2 * r2;
-                const double ir = rsqrt_nr(d2), r = d2 * ir, ir2 = ir * ir;
-                const double h[3] = {r0 * ir, r1 * ir, r2 * ir};
-                const double w = Q.w * ar;
-                const double wc[3] = {w * Q.x, w * Q.y, w * Q.z};
-                const Expo E = expo(r, f);
-                if (DG) {
-                    cd al, be;
-                    g_coef(E, ir, ir2, m, al, be);
-                    al = al * ir;
-                    be = be * ir;
-                    const double hh6[6] = {h[0] * h[0], h[1] * h[1], h[2] * h[2], h[0] * h[1], h[0] * h[2], h[1] * h[2]};
-#pragma unroll
-                    for (int k = 0; k < 3; k++) {
-                        const cd ak = al * wc[k];
-                        fma_cr(gb[k], be, wc[k]);
-#pragma unroll
-                        for (int e = 0; e < 6; e++) fma_cr(ga[k][e], ak, hh6[e]);
-                    }
-                }
-                if (DH) {
-                    cd c1, c2, c4;
-                    h_coef(E, ir, ir2, f, m, c1, c2, c4);
-                    const double rn = h[0] * n[0] + h[1] * n[1] + h[2] * n[2];
-                    cd pv[9];
-#pragma unroll
-                    for (int a = 0; a < 3; a++)
-#pragma unroll
-                        for (int b = 0; b < 3; b++) {
-                            cd v = c1 * (h[a] * n[b]);
-                            fma_cr(v, c2, n[a] * h[b] + (a == b ? rn : 0.0));
-                            fma_cr(v, c4, rn * h[a] * h[b]);
-                            pv[a * 3 + b] = v;
-                        }
-#pragma unroll
-                    for (int k = 0; k < 3; k++)
-#pragma unroll
-                        for (int e = 0; e < 9; e++) fma_cr(hh[k][e], pv[e], wc[k]);
-                }
-            }
-        }
-        // combine the two lanes of a face; the even lane stages the three corner sums
-        const int fs = tid >> 1;
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            double v[kSlotWords];
-#pragma unroll
-            for (int e = 0; e < 6; e++) v[2 * e] = ga[k][e].x, v[2 * e + 1] = ga[k][e].y;
-            v[12] = gb[k].x, v[13] = gb[k].y;
-#pragma unroll
-            for (int e = 0; e < 9; e++) v[14 + 2 * e] = hh[k][e].x, v[15 + 2 * e] = hh[k][e].y;
-#pragma unroll
-            for (int e = 0; e < kSlotWords; e++) v[e] += __shfl_xor(v[e], 1, 64);
-            if (part == 0) {  // stage layout [corner][word][face]: consecutive faces are consecutive words
-#pragma unroll
-                for (int e = 0; e < kSlotWords; e++) stage[(k * kSlotWords + e) * kPassFaces + fs] = v[e];
-            }
-        }
-        __syncthreads();
-        // vertex slot `tid` adds, in face order, the staged corners that belong to it
-        if (tid < kPatch) {  // acc layout [word][slot]; the host lists, per slot, its (face, corner) entries in face order
-            const int g0 = gather_ptr[pass * (kPatch + 1) + tid], g1 = gather_ptr[pass * (kPatch + 1) + tid + 1];
-            double sum[kSlotWords];
-#pragma unroll
-            for (int e = 0; e < kSlotWords; e++) sum[e] = acc[e * kPatch + tid];
-            for (int t = g0; t < g1; t++) {
-                const int fk = gather_items[t];
-                const int fi = fk / 3, k = fk - 3 * fi;
-                const double *src = stage + k * kSlotWords * kPassFaces + fi;
-#pragma unroll
-                for (int e = 0; e < kSlotWords; e++) sum[e] += src[e * kPassFaces];
-            }
-#pragma unroll
-            for (int e = 0; e < kSlotWords; e++) acc[e * kPatch + tid] = sum[e];
-        }
-        __syncthreads();
-    }
-    const int sl = tid;  // one slot per lane
-    if (sl >= kPatch) return;
-    const int j = patch_verts[pc * kPatch + sl];
-    if (j < 0) return;
-    auto a = [&](int e) { return acc[e * kPatch + sl]; };
-    const long long obase = (long long)l * lstride + (long long)(3 * il) * ld + 3 * (long long)j;
-#pragma unroll
-    for (int p = 0; p < 3; p++)
-#pragma unroll
-        for (int b = 0; b < 3; b++) {
-            if (DG) {
-                const int e = sidx(p, b);
-                cd v{a(2 * e), a(2 * e + 1)};
-                if (p == b) v += cd{a(12), a(13)};
-                UG[obase + p * ld + b] = v;
-            }
-            if (DH) UH[obase + p * ld + b] = cd{a(14 + 2 * (p * 3 + b)), a(15 + 2 * (p * 3 + b))} * kInv4Pi;
-        }
-}
-
-// ------------------------------------------------------------------------
 // B_trans and B_angular, regular faces.  Grid: (nl, rows), 256 threads; lanes
 // stride over the faces, then a wave shuffle reduction and one LDS pass.
 // ------------------------------------------------------------------------
@@ -929,23 +755,6 @@ hipError_t launch_colloc_gh(const Geom &g, const double4 *quad, int npts, const 
 }
 
 
-hipError_t launch_colloc_gh_patch(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
-                                  int row0, int nrows, const int *patch_ptr, const int2 *patch_faces, const int *patch_verts,
-                                  const int *pass_ptr, const int *gather_ptr, const unsigned char *gather_items,
-                                  int npatches, cd *UG, cd *UH, long long ld, long long lstride, hipStream_t st) {
-    dim3 grid(npatches, nl, nrows);
-    if (UG && UH)
-        hipLaunchKernelGGL((k_colloc_gh_patch<true, true>), grid, dim3(kPatchThreads), 0, st, g, quad, npts, freqs, l0, m, row0,
-                           patch_ptr, patch_faces, patch_verts, pass_ptr, gather_ptr, gather_items, UG, UH, ld, lstride);
-    else if (UG)
-        hipLaunchKernelGGL((k_colloc_gh_patch<true, false>), grid, dim3(kPatchThreads), 0, st, g, quad, npts, freqs, l0, m, row0,
-                           patch_ptr, patch_faces, patch_verts, pass_ptr, gather_ptr, gather_items, UG, UH, ld, lstride);
-    else if (UH)
-        hipLaunchKernelGGL((k_colloc_gh_patch<false, true>), grid, dim3(kPatchThreads), 0, st, g, quad, npts, freqs, l0, m, row0,
-                           patch_ptr, patch_faces, patch_verts, pass_ptr, gather_ptr, gather_items, UG, UH, ld, lstride);
-    return hipGetLastError();
-}
-
 hipError_t launch_colloc_gh_pair(const Geom &g, const double4 *pts, int npts_even, const Freq *freqs, int l0, int nl, const Mat &m,
                                  int row0, int nrows, const int *patch_pass, const int2 *pass_info, const int4 *pass_faces,
                                  const int *patch_verts, int npatches, cd *UG, cd *UH, long long ld, long long lstride,
@@ -965,9 +774,6 @@ hipError_t launch_colloc_gh_pair(const Geom &g, const double4 *pts, int npts_eve
 }
 int pair_patch_size() { return kPairPV; }
 int pair_pass_faces() { return kPairFaces; }
-
-int patch_size() { return kPatch; }
-int patch_pass_faces() { return kPassFaces; }
 
 hipError_t launch_colloc_b(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
                            int row0, int nrows, cd *Bt, cd *Ba, hipStream_t st) {

@@ -22,10 +22,6 @@ struct Geom {
 
 hipError_t launch_colloc_gh(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
                             int row0, int nrows, cd *UG, cd *UH, long long ld, long long lstride, hipStream_t st);
-hipError_t launch_colloc_gh_patch(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
-                                  int row0, int nrows, const int *patch_ptr, const int2 *patch_faces, const int *patch_verts,
-                                  const int *pass_ptr, const int *gather_ptr, const unsigned char *gather_items,
-                                  int npatches, cd *UG, cd *UH, long long ld, long long lstride, hipStream_t st);
 // Pair form of the G/H regular-face kernel (both matrices): per patch a range of passes
 // [patch_pass[p], patch_pass[p+1]); per pass (first face entry, nfaces | nsub << 16); per face entry
 // (face, corner 0, corner 1, corner 2) with corner = slot | sub-step << 10 or -1 outside the patch.
@@ -37,8 +33,6 @@ hipError_t launch_colloc_gh_pair(const Geom &g, const double4 *pts, int npts_eve
                                  hipStream_t st);
 int pair_patch_size();
 int pair_pass_faces();
-int patch_size();
-int patch_pass_faces();
 hipError_t launch_colloc_b(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
                            int row0, int nrows, cd *Bt, cd *Ba, hipStream_t st);
 hipError_t launch_colloc_singular(int kind, const Geom &g, const double2 *line, int nline, const Freq *freqs, int l0, int nl,

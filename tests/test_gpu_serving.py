@@ -1,0 +1,103 @@
+"""Serving path on the GPU: concurrent jobs through cqbem_submit / cqbem_poll / cqbem_wait.
+
+A mixed batch of objects is submitted (the multi-object workload: icospheres and cubes of different
+sizes and materials, L in {8, 16, 32}, collocation and Galerkin).  The batch runs on the context's
+stream slots under its HBM budget; BUSY answers are retried after a running job finishes.  Every
+result must equal, bit for bit, the same job assembled alone with cqbem_assemble, and the budget
+reservation must return to zero.  A job larger than the budget is refused with CQBEM_ERR_NOMEM."""
+import numpy as np
+import pytest
+
+from surfaceonlydeformables_amd import abi
+from surfaceonlydeformables_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+RHO = 1000.0
+
+
+def job_mix():
+    rng = np.random.default_rng(0)
+    jobs = []
+    shapes = [("ico", 1), ("cube", 3), ("ico", 2), ("cube", 2), ("ico", 1), ("cube", 4)]
+    for k, (kind, size) in enumerate(shapes):
+        radius = float(np.exp(rng.uniform(np.log(0.05), np.log(0.3))))
+        if kind == "ico":
+            V, F = M.icosphere(size, radius)
+        else:
+            V, F = M.cube(size, radius)
+        V = M.center(M.jitter(V, F, 0.01, k), F)
+        E = float(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
+        nu = float(rng.uniform(0.2, 0.45))
+        L = int(rng.choice([8, 16, 32]))
+        _, c1, c2 = M.material_speeds(E, nu, RHO)
+        R = 1e-10 ** (1.0 / (2 * L))
+        galerkin = k == 4
+        order, sub = (2, 1) if galerkin else (5, 2)
+        params = abi.make_params(c1, c2, RHO, nu, 1e-3, R, L, bdf=abi.BDF2 if k % 2 == 0 else abi.BDF1,
+                                 order=order, subdivision=sub, galerkin=galerkin)
+        jobs.append((V, F, params))
+    return jobs
+
+
+def device_outputs(torch, nv, L):
+    n3 = 3 * nv
+    dev = torch.device("cuda:0")
+    shapes = {"G": (L, n3, n3), "H": (L, n3, n3), "B_trans": (L, n3, 3), "B_angular": (L, n3, 3, 3), "B": (L, n3, 6)}
+    return {k: torch.full(s, float("nan"), dtype=torch.float64, device=dev) for k, s in shapes.items()}
+
+
+def test_concurrent_jobs_match_isolated_runs():
+    import torch
+    ctx = abi.Context(0)
+    try:
+        jobs = job_mix()
+        meshes = [abi.make_mesh(V, F) for V, F, _ in jobs]
+        served = [device_outputs(torch, len(V), p.L) for V, F, p in jobs]
+        pending, ids = list(range(len(jobs))), {}
+        while pending or ids:
+            while pending:
+                k = pending[0]
+                jid = ctx.submit(meshes[k], jobs[k][2], {n: t.data_ptr() for n, t in served[k].items()})
+                if jid < 0:
+                    break
+                ids[jid] = k
+                pending.pop(0)
+                print(f"submitted job {k} as {jid}", flush=True)
+            done = [j for j in ids if ctx.poll(j)]
+            if not done and ids:
+                done = [next(iter(ids))]
+            for j in done:
+                ctx.wait(j)
+                print(f"job {ids[j]} finished", flush=True)
+                del ids[j]
+        torch.cuda.synchronize()
+        assert ctx.reserved_bytes() == 0
+        for k, (V, F, p) in enumerate(jobs):
+            alone = device_outputs(torch, len(V), p.L)
+            ctx.assemble_device(meshes[k], p, {n: t.data_ptr() for n, t in alone.items()})
+            torch.cuda.synchronize()
+            for n in alone:
+                a, b = served[k][n].cpu().numpy(), alone[n].cpu().numpy()
+                assert np.isfinite(a).all(), (k, n)
+                assert np.array_equal(a, b), (k, n)
+    finally:
+        ctx.close()
+
+
+def test_job_larger_than_budget_is_refused():
+    import torch
+    V, F = M.icosphere(2, 0.2)
+    _, c1, c2 = M.material_speeds(5e5, 0.3, RHO)
+    p = abi.make_params(c1, c2, RHO, 0.3, 1e-3, 1e-10 ** (1 / 32), 16)
+    mesh = abi.make_mesh(V, F)
+    need = abi.job_footprint(mesh, p)
+    ctx = abi.Context(0, hbm_budget_bytes=need // 2)
+    try:
+        outs = device_outputs(torch, len(V), p.L)
+        with pytest.raises(abi.CQBEMError) as err:
+            ctx.submit(mesh, p, {n: t.data_ptr() for n, t in outs.items()})
+        assert err.value.code == abi.ERR_NOMEM
+        assert ctx.reserved_bytes() == 0
+    finally:
+        ctx.close()
