@@ -8,7 +8,8 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(PKG, "libcqbem.so")
-SOURCES = ["cqbem_kernels.hip", "cqbem_galerkin.hip", "cqbem_host.hip", "elastodynamics_cqbem_matrix_collection.cpp"]
+SOURCES = ["cqbem_kernels.hip", "cqbem_galerkin.hip", "cqbem_host.hip", "cqbem_dense.cpp",
+           "elastodynamics_cqbem_matrix_collection.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed-FP32 instructions stay disabled in all device code
 DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -41,7 +42,7 @@ def build(verbose: bool = False) -> str:
             sys.stderr.write(text)
     if failed:
         raise RuntimeError("hipcc failed")
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", OUT + ".tmp", *objs]
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", OUT + ".tmp", *objs, "-ldl"]
     subprocess.run(cmd, check=True)
     os.replace(OUT + ".tmp", OUT)
     # command-line driver of the C++ collection facade (links the library in-tree, rpath $ORIGIN)

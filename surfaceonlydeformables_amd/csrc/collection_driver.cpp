@@ -1,6 +1,7 @@
 // Command-line driver of the C++ collection facade: reads a scene JSON (mesh file, material,
 // simulation keys as in the reference's data/*.json), runs init(), and writes G, H (negated, as the
-// object reads them) and B as raw little-endian doubles to <out>_{G,H,B}.bin, plus L and nv.
+// object reads them) and B as raw little-endian doubles to <out>_{G,H,B}.bin, the matrices init()
+// derives for the time stepping to <out>_post_<name>.bin, and a JSON line with L, nv and timings.
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -10,7 +11,7 @@
 
 int main(int argc, char **argv) {
     if (argc != 3) {
-        std::cerr << "usage: collection_driver scene.json out_prefix" << std::endl;
+        std::cerr << "usage: collection_driver scene.json out_prefix   (out_prefix '-': no files, timings only)" << std::endl;
         return 2;
     }
     std::ifstream in(argv[1]);
@@ -31,12 +32,31 @@ int main(int argc, char **argv) {
         return 1;
     }
     const std::string pre = argv[2];
+    if (pre == "-") {
+        std::printf("{\"L\": %zu, \"nv\": %lld, \"mass\": %.17g, \"ms\": %.3f, \"factor_ms\": %.3f}\n", col.max_time_history + 1,
+                    (long long)col.num_vertices, col.mass, col.last_assembly_ms, col.last_factor_ms);
+        return 0;
+    }
     auto dump = [&](const std::string &name, const std::vector<cqbem::Dense> &mats) {
         std::ofstream o(pre + "_" + name + ".bin", std::ios::binary);
         for (auto &m : mats) o.write(reinterpret_cast<const char *>(m.data.data()), m.data.size() * sizeof(double));
     };
     dump("G", col.G_matrices), dump("H", col.H_matrices), dump("B", col.B_matrices);
-    std::printf("{\"L\": %zu, \"nv\": %lld, \"R\": %.17g, \"c1\": %.17g, \"c2\": %.17g, \"ms\": %.3f}\n",
-                col.max_time_history + 1, (long long)col.num_vertices, col.R, col.c1, col.c2, col.last_assembly_ms);
+    auto dump1 = [&](const std::string &name, const std::vector<double> &v) {
+        std::ofstream o(pre + "_post_" + name + ".bin", std::ios::binary);
+        o.write(reinterpret_cast<const char *>(v.data()), v.size() * sizeof(double));
+    };
+    dump1("u_p", col.u_p.data), dump1("u_p_diag", col.u_p_diag.data);
+    dump1("H_inv_B_trans", col.H_inv_B_trans.data), dump1("H_inv_B_euler", col.H_inv_B_euler.data);
+    dump1("B_trans_inv_H", col.B_trans_inv_H.data), dump1("B_euler_inv_H", col.B_euler_inv_H.data);
+    dump1("B_trans_inv_G_prime", col.B_trans_inv_G_prime.data), dump1("B_euler_inv_G_prime", col.B_euler_inv_G_prime.data);
+    dump1("B_euler_inv_B_trans", col.B_euler_inv_B_trans.data);
+    dump1("translational_acc_mat", col.translational_acc_mat.data), dump1("rotational_acc_mat", col.rotational_acc_mat.data);
+    dump1("rotation_removal_weights", col.rotation_removal_weights);
+    dump1("I", std::vector<double>(col.I, col.I + 9));
+    std::printf("{\"L\": %zu, \"nv\": %lld, \"R\": %.17g, \"c1\": %.17g, \"c2\": %.17g, \"mass\": %.17g, "
+                "\"ms\": %.3f, \"factor_ms\": %.3f}\n",
+                col.max_time_history + 1, (long long)col.num_vertices, col.R, col.c1, col.c2, col.mass, col.last_assembly_ms,
+                col.last_factor_ms);
     return 0;
 }

@@ -3,6 +3,8 @@
 // src/elastics_bem_matrix_collection.cpp:16-41).
 #include "elastodynamics_cqbem_matrix_collection.hpp"
 
+#include "cqbem_rules.hpp"
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -259,6 +261,7 @@ bool ElastodynamicsCQBEMMatrixCollection::run(const Json &config) {
     else
         return fail("Unsupported multistep method detected. Valid options are \"BDF1\"/\"BDF2\"");
     use_galerkin = sim.contains("use_galerkin") && sim["use_galerkin"].boolean();
+    update_local_frame = !sim.contains("update_local_frame") || sim["update_local_frame"].boolean();
     const int order = (int)sim["gaussian_quadrature_order"].number();
     const int sub = (int)sim["quadrature_subdivision"].number();
     // material constants and history length (matrix_collection.cpp:59-69)
@@ -300,6 +303,133 @@ bool ElastodynamicsCQBEMMatrixCollection::run(const Json &config) {
         std::copy(b.begin() + n * n3 * 6, b.begin() + (n + 1) * n3 * 6, B_matrices[n].data.begin());
     }
     (void)cuda_thread_per_block_;  // launch shapes are chosen by the library
+    const auto t1 = std::chrono::steady_clock::now();
+    if (!post_process()) return false;
+    last_factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+    return true;
+}
+
+// What init() derives from G_0, H_0 and B_0 for the time stepping: the external acceleration matrices
+// (physics_base_matrix_collection.cpp:40-50,374-436), G' and the least-squares / LU solves
+// (matrix_collection.cpp:174-215), the rotation-removal weights (:307-322) and u_p's diagonal blocks
+// (:217-227).  The LU of H_0 and its solves run on the GPU.
+bool ElastodynamicsCQBEMMatrixCollection::post_process() {
+    const int64_t nv = num_vertices, n3 = 3 * nv, nf = (int64_t)(F.size() / 3);
+    auto vtx = [&](int64_t v, int k) { return V[3 * v + k]; };
+    std::vector<double> A2(nf), N(3 * nf);  // doubled areas, unit normals
+    for (int64_t f = 0; f < nf; f++) {
+        const int32_t *t = &F[3 * f];
+        double e1[3], e2[3], c[3];
+        for (int k = 0; k < 3; k++) e1[k] = vtx(t[1], k) - vtx(t[0], k), e2[k] = vtx(t[2], k) - vtx(t[0], k);
+        c[0] = e1[1] * e2[2] - e1[2] * e2[1], c[1] = e1[2] * e2[0] - e1[0] * e2[2], c[2] = e1[0] * e2[1] - e1[1] * e2[0];
+        A2[f] = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        for (int k = 0; k < 3; k++) N[3 * f + k] = c[k] / A2[f];
+    }
+    // inertia about the volume centroid of the centred mesh, order-4 triangle rule
+    double cmc[3] = {0, 0, 0}, vol = 0;
+    for (int64_t f = 0; f < nf; f++) {
+        const double *a = &V[3 * F[3 * f]], *b = &V[3 * F[3 * f + 1]], *d = &V[3 * F[3 * f + 2]];
+        const double tv = (a[0] * (b[1] * d[2] - b[2] * d[1]) - a[1] * (b[0] * d[2] - b[2] * d[0]) +
+                           a[2] * (b[0] * d[1] - b[1] * d[0])) / 6.0;
+        vol += tv;
+        for (int k = 0; k < 3; k++) cmc[k] += tv * (a[k] + b[k] + d[k]) / 4.0;
+    }
+    for (int k = 0; k < 3; k++) cmc[k] /= vol;
+    std::fill(I, I + 9, 0.0);
+    const auto rule = tri_rule(4);
+    for (int64_t f = 0; f < nf; f++) {
+        const double *n = &N[3 * f];
+        for (const TriPoint &q : rule) {
+            double r[3];
+            for (int k = 0; k < 3; k++)
+                r[k] = q.b[0] * vtx(F[3 * f], k) + q.b[1] * vtx(F[3 * f + 1], k) + q.b[2] * vtx(F[3 * f + 2], k) - cmc[k];
+            const double w = q.w * 0.5 * A2[f];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+                    const double v = i == j ? (rho / 3.) * (r[i1] * r[i1] * r[i1] * n[i1] + r[i2] * r[i2] * r[i2] * n[i2])
+                                            : (-rho / 4.) * (r[j] * r[i] * r[i] * n[i] + r[i] * r[j] * r[j] * n[j]);
+                    I[3 * i + j] += w * v;
+                }
+        }
+    }
+    // external acceleration matrices (3 x 3 nv)
+    translational_acc_mat = Dense(3, n3);
+    rotational_acc_mat = Dense(3, n3);
+    Dense rot_raw(3, n3);
+    for (int64_t f = 0; f < nf; f++) {
+        double sum[3];
+        for (int k = 0; k < 3; k++) sum[k] = vtx(F[3 * f], k) + vtx(F[3 * f + 1], k) + vtx(F[3 * f + 2], k);
+        for (int c = 0; c < 3; c++) {
+            const int64_t j = F[3 * f + c];
+            for (int k = 0; k < 3; k++) translational_acc_mat(k, 3 * j + k) += A2[f] / 6.;
+            double v[3];
+            for (int k = 0; k < 3; k++) v[k] = A2[f] / 6. * 0.25 * (sum[k] + vtx(j, k));
+            // cross-product matrix [v]_x
+            const double X[3][3] = {{0, -v[2], v[1]}, {v[2], 0, -v[0]}, {-v[1], v[0], 0}};
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) rot_raw(a, 3 * j + b) += X[a][b];
+        }
+    }
+    for (double &x : translational_acc_mat.data) x *= 1. / mass;
+    {
+        // I^-1 by the adjugate
+        const double *m = I;
+        const double det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) + m[2] * (m[3] * m[7] - m[4] * m[6]);
+        if (!(std::fabs(det) > 0)) return fail("singular inertia tensor");
+        const double inv[9] = {(m[4] * m[8] - m[5] * m[7]) / det, (m[2] * m[7] - m[1] * m[8]) / det, (m[1] * m[5] - m[2] * m[4]) / det,
+                               (m[5] * m[6] - m[3] * m[8]) / det, (m[0] * m[8] - m[2] * m[6]) / det, (m[2] * m[3] - m[0] * m[5]) / det,
+                               (m[3] * m[7] - m[4] * m[6]) / det, (m[1] * m[6] - m[0] * m[7]) / det, (m[0] * m[4] - m[1] * m[3]) / det};
+        for (int a = 0; a < 3; a++)
+            for (int64_t c = 0; c < n3; c++)
+                rotational_acc_mat(a, c) = inv[3 * a] * rot_raw(0, c) + inv[3 * a + 1] * rot_raw(1, c) + inv[3 * a + 2] * rot_raw(2, c);
+    }
+    // B_0 halves, H_0 (un-negated) and G'
+    std::vector<double> Bt((size_t)(n3 * 3)), Be((size_t)(n3 * 3));
+    for (int64_t r = 0; r < n3; r++)
+        for (int k = 0; k < 3; k++) Bt[3 * r + k] = B_matrices[0](r, k), Be[3 * r + k] = B_matrices[0](r, 3 + k);
+    std::vector<double> H0(H_matrices[0].data.size());
+    for (size_t e = 0; e < H0.size(); e++) H0[e] = -H_matrices[0].data[e];
+    u_p = G_matrices[0];  // becomes G', then H_0^-1 G'
+    if (update_local_frame)
+        for (int64_t r = 0; r < n3; r++)
+            for (int64_t c = 0; c < n3; c++) {
+                double v = 0;
+                for (int k = 0; k < 3; k++) v += Bt[3 * r + k] * translational_acc_mat(k, c) + Be[3 * r + k] * rotational_acc_mat(k, c);
+                u_p(r, c) += v;
+            }
+    std::string err;
+    if (!B_trans_inv.compute(Bt, n3, 3, err) || !B_euler_inv.compute(Be, n3, 3, err)) return fail(err);
+    auto as3 = [&](std::vector<double> x, int64_t cols) {
+        Dense d(3, cols);
+        d.data = std::move(x);
+        return d;
+    };
+    B_trans_inv_H = as3(B_trans_inv.solve(H0.data(), n3), n3);
+    B_euler_inv_H = as3(B_euler_inv.solve(H0.data(), n3), n3);
+    B_trans_inv_G_prime = as3(B_trans_inv.solve(u_p.data.data(), n3), n3);
+    B_euler_inv_G_prime = as3(B_euler_inv.solve(u_p.data.data(), n3), n3);
+    B_euler_inv_B_trans = as3(B_euler_inv.solve(Bt.data(), 3), 3);
+    // rotation-removal weights
+    rotation_removal_weights.assign(nv, 0.0);
+    for (int64_t f = 0; f < nf; f++) {
+        double yn = 0;
+        for (int k = 0; k < 3; k++) yn += vtx(F[3 * f], k) * N[3 * f + k];
+        const double val = (1. / 6.) * A2[f] * yn;
+        for (int c = 0; c < 3; c++) rotation_removal_weights[F[3 * f + c]] += val;
+    }
+    // LU of H_0 on the GPU and its solves
+    if (!H_inv.factor(n3, H0.data(), err)) return fail(err);
+    H0.clear(), H0.shrink_to_fit();
+    H_inv_B_trans = Dense(n3, 3), H_inv_B_euler = Dense(n3, 3);
+    H_inv_B_trans.data = Bt, H_inv_B_euler.data = Be;
+    if (!H_inv.solve(u_p.data.data(), n3, err) || !H_inv.solve(H_inv_B_trans.data.data(), 3, err) ||
+        !H_inv.solve(H_inv_B_euler.data.data(), 3, err))
+        return fail(err);
+    u_p_diag = Dense(nv, 9);
+    for (int64_t v = 0; v < nv; v++)
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) u_p_diag(v, 3 * a + b) = u_p(3 * v + a, 3 * v + b);
     return true;
 }
 

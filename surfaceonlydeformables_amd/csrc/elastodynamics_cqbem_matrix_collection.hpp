@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/cqbem.h"
+#include "cqbem_dense.hpp"
 
 namespace cqbem {
 
@@ -65,14 +66,28 @@ class ElastodynamicsCQBEMMatrixCollection {
     std::vector<int32_t> F;
     double cm[3] = {0, 0, 0};
     double mass = 0.0;
+    double I[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // inertia tensor about the centroid, row-major
     double c1 = 0, c2 = 0, rho = 0, mu = 0, nu = 0, R = 0;
     bool use_galerkin = false;
+    bool update_local_frame = true;
     MultistepMethod multistep_method = BDF2;
-    double last_assembly_ms = 0.0;
+    double last_assembly_ms = 0.0, last_factor_ms = 0.0;
+
+    // --- after the assembly (matrix_collection.cpp:174-215, physics_base_matrix_collection.cpp:40-50) ---
+    Dense translational_acc_mat, rotational_acc_mat;  // 3 x 3 nv
+    DeviceLU H_inv;                                   // LU of H_0 (un-negated, with C), on the GPU
+    ThinLeastSquares B_trans_inv, B_euler_inv;        // of B_0's two 3 nv x 3 halves
+    Dense B_trans_inv_H, B_euler_inv_H, B_trans_inv_G_prime, B_euler_inv_G_prime;  // 3 x 3 nv
+    Dense B_euler_inv_B_trans;                                                    // 3 x 3
+    Dense H_inv_B_trans, H_inv_B_euler;                                           // 3 nv x 3
+    Dense u_p;       // H_0^-1 G' (3 nv x 3 nv), G' = G_0 (+ B_trans T + B_euler R with the local frame update)
+    Dense u_p_diag;  // nv x 9: the 3 x 3 diagonal blocks of u_p
+    std::vector<double> rotation_removal_weights;  // nv
 
   private:
     bool fail(const std::string &msg);
     bool run(const Json &config);
+    bool post_process();
 
     double dt_;
     bool enable_cuda_;

@@ -76,3 +76,88 @@ def test_facade_matches_oracle(tmp_path, oracle):
     assert rel(G, ref["G"]) < 1e-9
     assert rel(-H, ref["H"]) < 1e-8
     assert rel(B, ref["B6"]) < 1e-9
+
+
+def _tetra_inertia(V, F, rho):
+    """Exact inertia tensor of the closed polyhedron about its centroid (signed tetrahedra from the origin)."""
+    vol, first, second = 0.0, np.zeros(3), np.zeros((3, 3))
+    for f in F:
+        a, b, c = V[f[0]], V[f[1]], V[f[2]]
+        det = np.dot(a, np.cross(b, c))
+        vol += det / 6
+        first += det / 24 * (a + b + c)
+        s = a + b + c
+        second += det / 120 * (np.outer(a, a) + np.outer(b, b) + np.outer(c, c) + np.outer(s, s))
+    cm = first / vol
+    C = second - vol * np.outer(cm, cm)
+    return rho * (np.trace(C) * np.eye(3) - C), vol
+
+
+def _cross(v):
+    return np.array([[0, -v[2], v[1]], [v[2], 0, -v[0]], [-v[1], v[0], 0]])
+
+
+@pytest.mark.gpu
+def test_facade_time_stepping_matrices(tmp_path):
+    """init() also derives what the time stepping reads (matrix_collection.cpp:174-227): the LU of H_0 on the
+    GPU and its solves, the least-squares B inverses, G', the external acceleration matrices and the
+    rotation-removal weights.  Each is recomputed here with numpy from the dumped G, H, B and the mesh; the
+    inertia tensor is pinned by the exact polyhedral (tetrahedra) formula, which the order-4 surface rule
+    reproduces because its integrands are cubic."""
+    V, F = M.icosphere(1, 0.1)
+    V = M.jitter(V, F, 0.01, 5)
+    path, cfg = scene(tmp_path, V, F)
+    print("collection facade post-processing ico1", flush=True)
+    r = run(path, tmp_path / "out")
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    Vc = M.center(V, F)
+    nv, L = len(V), info["L"]
+    n3 = 3 * nv
+    rho = 1000.0
+    G = np.fromfile(tmp_path / "out_G.bin").reshape(L, n3, n3)
+    H = np.fromfile(tmp_path / "out_H.bin").reshape(L, n3, n3)
+    B = np.fromfile(tmp_path / "out_B.bin").reshape(L, n3, 6)
+    post = lambda name, shape: np.fromfile(tmp_path / f"out_post_{name}.bin").reshape(shape)
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+
+    I_exact, vol = _tetra_inertia(Vc, F, rho)
+    assert abs(info["mass"] - rho * vol) < 1e-12 * rho * vol
+    assert rel(post("I", (3, 3)), I_exact) < 1e-12
+
+    A2 = np.linalg.norm(np.cross(Vc[F[:, 1]] - Vc[F[:, 0]], Vc[F[:, 2]] - Vc[F[:, 0]]), axis=1)
+    Nf = np.cross(Vc[F[:, 1]] - Vc[F[:, 0]], Vc[F[:, 2]] - Vc[F[:, 0]]) / A2[:, None]
+    T = np.zeros((3, n3))
+    Rr = np.zeros((3, n3))
+    for f, tri in enumerate(F):
+        s = Vc[tri].sum(0)
+        for j in tri:
+            T[:, 3 * j:3 * j + 3] += np.eye(3) * A2[f] / 6
+            Rr[:, 3 * j:3 * j + 3] += A2[f] / 6 * 0.25 * _cross(s + Vc[j])
+    T /= rho * vol
+    Rm = np.linalg.solve(post("I", (3, 3)), Rr)
+    assert rel(post("translational_acc_mat", (3, n3)), T) < 1e-13
+    assert rel(post("rotational_acc_mat", (3, n3)), Rm) < 1e-12
+
+    w = np.zeros(nv)
+    for f, tri in enumerate(F):
+        w[tri] += A2[f] / 6 * np.dot(Vc[tri[0]], Nf[f])
+    rw = post("rotation_removal_weights", (nv,))
+    assert rel(rw, w) < 1e-13
+    assert abs(rw.sum() - 3 * vol) < 1e-12 * vol  # divergence theorem: sum = int y.n dA = 3 V
+
+    H0 = -H[0]
+    Bt, Be = B[0][:, :3], B[0][:, 3:]
+    Gp = G[0] + Bt @ T + Be @ Rm  # update_local_frame defaults to true
+    assert rel(post("u_p", (n3, n3)), np.linalg.solve(H0, Gp)) < 1e-10
+    assert rel(post("H_inv_B_trans", (n3, 3)), np.linalg.solve(H0, Bt)) < 1e-10
+    assert rel(post("H_inv_B_euler", (n3, 3)), np.linalg.solve(H0, Be)) < 1e-10
+    u_p = post("u_p", (n3, n3))
+    diag = np.stack([u_p[3 * v:3 * v + 3, 3 * v:3 * v + 3].ravel() for v in range(nv)])
+    assert np.array_equal(post("u_p_diag", (nv, 9)), diag)
+    lsq = lambda A, Y: np.linalg.lstsq(A, Y, rcond=None)[0]
+    assert rel(post("B_trans_inv_H", (3, n3)), lsq(Bt, H0)) < 1e-10
+    assert rel(post("B_euler_inv_H", (3, n3)), lsq(Be, H0)) < 1e-10
+    assert rel(post("B_trans_inv_G_prime", (3, n3)), lsq(Bt, Gp)) < 1e-10
+    assert rel(post("B_euler_inv_G_prime", (3, n3)), lsq(Be, Gp)) < 1e-10
+    assert rel(post("B_euler_inv_B_trans", (3, 3)), lsq(Be, Bt)) < 1e-10
