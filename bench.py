@@ -77,6 +77,121 @@ def cpu_baseline(V, F, params, seconds_target=15.0):
                        f"{M_sample} entries ({t_tr:.1f} s), extrapolated to the full job: {full:.0f} s")}
 
 
+def scene_jobs(n_objects=16, seed=0):
+    """The seeded multi-object scene: icospheres of level 1-4 or k x k-per-face cubes (k in [4, 16]),
+    radius / half-size log-uniform in [0.05, 0.3] m, E log-uniform in [1e4, 1e6] Pa, nu in [0.2, 0.45],
+    rho = 1000, L in {16, 32, 64}, dt = 1e-3 s, BDF2, collocation order 5 x subdivision 2."""
+    from surfaceonlydeformables_amd import abi
+    from surfaceonlydeformables_amd import mesh as M
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for k in range(n_objects):
+        size = float(np.exp(rng.uniform(np.log(0.05), np.log(0.3))))
+        if rng.uniform() < 0.5:
+            V, F = M.icosphere(int(rng.integers(1, 5)), size)
+        else:
+            V, F = M.cube(int(rng.integers(4, 17)), size)
+        V = M.center(M.jitter(V, F, 0.01, seed * 1000 + k), F)
+        E = float(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
+        nu = float(rng.uniform(0.2, 0.45))
+        L = int(rng.choice([16, 32, 64]))
+        _, c1, c2 = M.material_speeds(E, nu, 1000.0)
+        params = abi.make_params(c1, c2, 1000.0, nu, 1e-3, 1e-10 ** (1.0 / (2 * L)), L, bdf=abi.BDF2, order=5,
+                                 subdivision=2)
+        jobs.append((V, F, params))
+    return jobs
+
+
+def run_jobs(ctx, jobs, arrivals, torch):
+    """Feeds jobs to the serving API at the given arrival times (seconds from start; BUSY answers wait
+    for a running job), keeps each job's outputs until it completes, and returns per-job latencies
+    (completion - arrival) and the makespan."""
+    from surfaceonlydeformables_amd import abi
+    dev = torch.device("cuda:0")
+    queue, running, lat = [], {}, [None] * len(jobs)
+    meshes = [abi.make_mesh(V, F) for V, F, _ in jobs]
+    t0 = time.perf_counter()
+    nxt = 0
+    while nxt < len(jobs) or queue or running:
+        now = time.perf_counter() - t0
+        while nxt < len(jobs) and arrivals[nxt] <= now:
+            queue.append(nxt)
+            nxt += 1
+        while queue:
+            k = queue[0]
+            V, F, p = jobs[k]
+            n3 = 3 * len(V)
+            outs = {"G": torch.empty((p.L, n3, n3), dtype=torch.float64, device=dev),
+                    "H": torch.empty((p.L, n3, n3), dtype=torch.float64, device=dev),
+                    "B": torch.empty((p.L, n3, 6), dtype=torch.float64, device=dev)}
+            jid = ctx.submit(meshes[k], p, {n: t.data_ptr() for n, t in outs.items()})
+            if jid < 0:
+                del outs
+                break
+            running[jid] = (k, outs)
+            queue.pop(0)
+        for jid in [j for j in running if ctx.poll(j)]:
+            ctx.wait(jid)
+            k, _ = running.pop(jid)
+            lat[k] = time.perf_counter() - t0 - arrivals[k]
+        if not queue and nxt < len(jobs) and not running:
+            time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
+        else:
+            time.sleep(0.0005)
+    torch.cuda.synchronize()
+    return lat, time.perf_counter() - t0
+
+
+def scene_main(args):
+    """Multi-object scene: all objects submitted at t = 0, overlapping on the context's stream slots."""
+    import torch
+    from surfaceonlydeformables_amd import abi
+    jobs = scene_jobs()
+    ctx = abi.Context(0)
+    entries = sum(2.0 * (3 * len(V)) ** 2 * p.L for V, F, p in jobs)
+    run_jobs(ctx, jobs[:2], [0.0, 0.0], torch)  # warm-up
+    lat, makespan = run_jobs(ctx, jobs, [0.0] * len(jobs), torch)
+    # the same jobs one after another, each alone on the device
+    serial = 0.0
+    dev = torch.device("cuda:0")
+    for V, F, p in jobs:
+        n3 = 3 * len(V)
+        outs = {"G": torch.empty((p.L, n3, n3), dtype=torch.float64, device=dev),
+                "H": torch.empty((p.L, n3, n3), dtype=torch.float64, device=dev),
+                "B": torch.empty((p.L, n3, 6), dtype=torch.float64, device=dev)}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.assemble_device(abi.make_mesh(V, F), p, {n: t.data_ptr() for n, t in outs.items()})
+        serial += time.perf_counter() - t0
+        del outs
+    ctx.close()
+    print(json.dumps({"metric": "multi-object scene: G+H entries/s (all objects submitted at once)",
+                      "value": entries / makespan, "unit": "entries/s", "objects": len(jobs),
+                      "makespan_ms": makespan * 1e3, "serial_sum_ms": serial * 1e3,
+                      "latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3, "p99": float(np.percentile(lat, 99)) * 1e3},
+                      "scene": [{"nv": len(V), "L": p.L} for V, F, p in jobs], "dtype": "f64", "data": "synthetic"}),
+          flush=True)
+
+
+def serving_main(args):
+    """Poisson stream of scene-distribution jobs at `rate` jobs/s for `duration` seconds."""
+    import torch
+    from surfaceonlydeformables_amd import abi
+    rng = np.random.default_rng(1)
+    n = max(1, int(args.serving * args.duration))
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serving, n)).tolist()
+    jobs = scene_jobs(n, seed=2)
+    ctx = abi.Context(0)
+    run_jobs(ctx, jobs[:2], [0.0, 0.0], torch)  # warm-up
+    lat, makespan = run_jobs(ctx, jobs, arrivals, torch)
+    ctx.close()
+    entries = sum(2.0 * (3 * len(V)) ** 2 * p.L for V, F, p in jobs)
+    print(json.dumps({"metric": "serving: Poisson job stream, G+H entries/s and job latency",
+                      "rate_jobs_per_s": args.serving, "jobs": n, "value": entries / makespan, "unit": "entries/s",
+                      "latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3, "p99": float(np.percentile(lat, 99)) * 1e3},
+                      "makespan_s": makespan, "dtype": "f64", "data": "synthetic"}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,11 +202,19 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one untimed assembly followed by a 1 GiB device copy, for HBM counter runs "
                          "(the copy calibrates FETCH_SIZE / WRITE_SIZE); prints no JSON line")
+    ap.add_argument("--scene", action="store_true", help="multi-object scene throughput instead of the single-object line")
+    ap.add_argument("--serving", type=float, default=0.0, help="Poisson serving stream at this many jobs/s")
+    ap.add_argument("--duration", type=float, default=10.0, help="serving stream length in seconds")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the CQBEM precompute of one object runs on one MI355X "
               "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.scene:
+        return scene_main(args)
+    if args.serving > 0:
+        return serving_main(args)
 
     import torch
     from surfaceonlydeformables_amd import abi
@@ -149,6 +272,24 @@ def main():
         with open(prof) as fh:
             roof["traffic"] = json.load(fh).get("hbm_bytes_per_launch")
 
+    # time stepping on the resident matrices: the history product of one step with a full history
+    nh = L - 1
+    hp = torch.randn((nh, n3), dtype=torch.float64, device=dev)
+    hu = torch.randn((nh, n3), dtype=torch.float64, device=dev)
+    ha = torch.randn((nh, 6), dtype=torch.float64, device=dev)
+    hb = torch.empty(n3, dtype=torch.float64, device=dev)
+    hist_args = (n3, L, ptrs["G"], ptrs["H"], ptrs["B"], hp.data_ptr(), hu.data_ptr(), ha.data_ptr(), nh, hb.data_ptr())
+    ctx.history(*hist_args)
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.history(*hist_args)
+    hist_ms = (time.perf_counter() - t0) * 1e3 / reps
+    hist_bytes = 2.0 * nh * n3 * n3 * 8 + nh * n3 * 6 * 8
+    history = {"kernel": "k_history", "ms": hist_ms, "matrices_read_GB": hist_bytes / 1e9,
+               "achieved_TBps": hist_bytes / (hist_ms * 1e-3) / 1e12, "peak_TBps": 8.0,
+               "note": "b = sum_i G_i p + H_i u + B_i acc over the 31 resident history weights (one CQ time step)"}
+
     line = {"metric": METRIC, "value": value, "unit": "entries/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f64+f32",
@@ -157,7 +298,7 @@ def main():
                        "entries_per_step": entries},
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
             "phases_ms_last_step": {k: last[k] for k in ("regular_gh", "regular_b", "singular", "transform", "host")},
-            "roofline": roof}
+            "roofline": roof, "time_step_history": history}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(V, F, params)
     ctx.close()

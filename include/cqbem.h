@@ -118,6 +118,17 @@ int cqbem_wait(cqbem_context *ctx, uint64_t job_id);
 /* Bytes currently reserved by in-flight jobs. */
 int cqbem_reserved_bytes(const cqbem_context *ctx, uint64_t *bytes);
 
+/* ---- time stepping on the resident matrices ----
+ * History term of one CQ time step (reference: ElastodynamicsCQBEMObject::estimate_next_state,
+ * src/elastodynamics_cqbem_object.cpp:42-49, whose H_matrices are stored negated):
+ *   b = sum_{i=1..m} ( G_i p[nhist-i] - H_i u[nhist-i] + B_i acc[nhist-i] ),   m = min(nhist, L-1),
+ * with G, H (L x n3 x n3) and B (L x n3 x 6) as cqbem_assemble writes them, and the histories
+ * p_hist, u_hist (nhist x n3) and acc_hist (nhist x 6: translational then rotational acceleration),
+ * oldest first.  All pointers are device pointers on the context's GPU; any of G, H, B may be NULL
+ * (its term is skipped).  b (n3) is complete when the call returns. */
+int cqbem_history(cqbem_context *ctx, int64_t n3, int32_t L, const double *G, const double *H, const double *B,
+                  const double *p_hist, const double *u_hist, const double *acc_hist, int32_t nhist, double *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,7 +22,7 @@ G, H, B_TRANS, B_ANGULAR, ALL = 1, 2, 4, 8, 15
 EXPORTED = [
     "cqbem_create", "cqbem_destroy", "cqbem_last_error", "cqbem_assemble", "cqbem_job_footprint",
     "cqbem_laplace", "cqbem_frequencies", "cqbem_last_timings", "cqbem_submit", "cqbem_poll",
-    "cqbem_wait", "cqbem_reserved_bytes",
+    "cqbem_wait", "cqbem_reserved_bytes", "cqbem_history",
 ]
 
 
@@ -80,6 +80,7 @@ def lib():
         L.cqbem_poll.argtypes = [C.c_void_p, C.c_uint64, P(C.c_int32)]
         L.cqbem_wait.argtypes = [C.c_void_p, C.c_uint64]
         L.cqbem_reserved_bytes.argtypes = [C.c_void_p, P(C.c_uint64)]
+        L.cqbem_history.argtypes = [C.c_void_p, C.c_int64, C.c_int32] + [C.c_void_p] * 6 + [C.c_int32, C.c_void_p]
         _lib = L
     return _lib
 
@@ -209,6 +210,11 @@ class Context:
         b = C.c_uint64()
         self._check(lib().cqbem_reserved_bytes(self._h, C.byref(b)))
         return b.value
+
+    def history(self, n3: int, L: int, G, H, B, p_hist, u_hist, acc_hist, nhist: int, b):
+        """History term of one CQ step on device buffers (device pointers as ints; G, H, B may be 0)."""
+        self._check(lib().cqbem_history(self._h, n3, L, G or None, H or None, B or None, p_hist or None,
+                                        u_hist or None, acc_hist or None, nhist, b))
 
 
 def job_footprint(mesh, params, outputs_on_device=True) -> int:

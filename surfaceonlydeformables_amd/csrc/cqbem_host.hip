@@ -654,6 +654,29 @@ int cqbem_frequencies(int32_t L, double R, double dt, int32_t bdf, double *s_out
     return (int)s.size();
 }
 
+int cqbem_history(cqbem_context *ctx, int64_t n3, int32_t L, const double *G, const double *H, const double *B,
+                  const double *p_hist, const double *u_hist, const double *acc_hist, int32_t nhist, double *b) {
+    if (!ctx) return CQBEM_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    try {
+        if (n3 <= 0 || L < 1 || nhist < 0 || !b) throw Err{CQBEM_ERR_ARG, "history: bad sizes or null output"};
+        const int m = std::min<int>(nhist, L - 1);
+        if (m > 0 && (((G && !p_hist) || (H && !u_hist) || (B && !acc_hist))))
+            throw Err{CQBEM_ERR_ARG, "history: a matrix without its history"};
+        HIPCHK(hipSetDevice(ctx->device));
+        if (m == 0 || (!G && !H && !B)) {
+            HIPCHK(hipMemsetAsync(b, 0, sizeof(double) * n3, ctx->stream));
+        } else {
+            HIPCHK(launch_history(G, H, B, p_hist, u_hist, acc_hist, n3, m, nhist, b, ctx->stream));
+        }
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    } catch (const Err &e) {
+        ctx->last_error = e.msg;
+        return e.code;
+    }
+    return CQBEM_OK;
+}
+
 int cqbem_job_footprint(const cqbem_mesh *mesh, const cqbem_params *params, int outputs_on_device, uint64_t *bytes) {
     try {
         check_inputs(mesh, params);

@@ -704,6 +704,62 @@ __global__ void __launch_bounds__(256) k_transform_wide(const cd *__restrict__ s
     }
 }
 
+// ------------------------------------------------------------------------
+// History term of one CQ time step on the resident matrices:
+//   b = sum_{i=1..m} (G_i p[nh-i] - H_i u[nh-i] + B_i acc[nh-i]).
+// One wavefront owns kHistRows consecutive rows; lanes stride the columns, so every history
+// vector element is loaded once per kHistRows matrix rows.  Fixed-order wave reduction.
+// ------------------------------------------------------------------------
+constexpr int kHistRows = 4;
+
+__global__ void __launch_bounds__(256) k_history(const double *__restrict__ G, const double *__restrict__ H,
+                                                  const double *__restrict__ B, const double *__restrict__ p,
+                                                  const double *__restrict__ u, const double *__restrict__ acc,
+                                                  long long n3, int m, int nh, double *__restrict__ b) {
+    const long long r0 = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * kHistRows;
+    const int lane = threadIdx.x & 63;
+    if (r0 >= n3) return;
+    const int nr = (int)(n3 - r0 < kHistRows ? n3 - r0 : kHistRows);
+    double s[kHistRows] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 1; i <= m; i++) {
+        const long long mo = (long long)i * n3 * n3 + r0 * n3;
+        const double *pv = p + (long long)(nh - i) * n3, *uv = u + (long long)(nh - i) * n3;
+        if (G) {
+            for (long long c = lane; c < n3; c += 64) {
+                const double x = pv[c];
+#pragma unroll
+                for (int k = 0; k < kHistRows; k++)
+                    if (k < nr) s[k] = fma(G[mo + k * n3 + c], x, s[k]);
+            }
+        }
+        if (H) {
+            for (long long c = lane; c < n3; c += 64) {
+                const double x = uv[c];
+#pragma unroll
+                for (int k = 0; k < kHistRows; k++)
+                    if (k < nr) s[k] = fma(-H[mo + k * n3 + c], x, s[k]);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kHistRows; k++)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s[k] += __shfl_xor(s[k], o, 64);
+    if (lane < nr) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < kHistRows; k++)
+            if (k == lane) v = s[k];
+        if (B)
+            for (int i = 1; i <= m; i++) {
+                const double *br = B + ((long long)i * n3 + r0 + lane) * 6, *av = acc + (long long)(nh - i) * 6;
+#pragma unroll
+                for (int k = 0; k < 6; k++) v = fma(br[k], av[k], v);
+            }
+        b[r0 + lane] = v;
+    }
+}
+
 // H_0 += C on the 3x3 diagonal blocks (collocation, H.cu:186-191).
 __global__ void k_add_c_colloc(double *H0, const double *C, int nv) {
     const int t = blockIdx.x * 256 + threadIdx.x;
@@ -816,6 +872,13 @@ hipError_t launch_transform(const cd *spec, long long E, int Lf, const double2 *
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+hipError_t launch_history(const double *G, const double *H, const double *B, const double *p, const double *u,
+                          const double *acc, long long n3, int m, int nh, double *b, hipStream_t st) {
+    const long long waves = (n3 + kHistRows - 1) / kHistRows;
+    hipLaunchKernelGGL(k_history, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, G, H, B, p, u, acc, n3, m, nh, b);
+    return hipGetLastError();
 }
 
 hipError_t launch_add_c_colloc(double *H0, const double *C, int nv, hipStream_t st) {

@@ -1,0 +1,80 @@
+"""History term of a CQ time step on the resident matrices (cqbem_history), against numpy.
+
+b = sum_{i=1..m} (G_i p[nh-i] - H_i u[nh-i] + B_i acc[nh-i]), m = min(nh, L-1): the sum
+ElastodynamicsCQBEMObject::estimate_next_state forms each step (elastodynamics_cqbem_object.cpp:42-49),
+with H not negated here.  Floating-point sums in a different order: relative error <= 1e-13."""
+import numpy as np
+import pytest
+
+from surfaceonlydeformables_amd import abi
+from surfaceonlydeformables_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+
+def reference(G, H, B, p, u, acc, L):
+    nh = p.shape[0]
+    m = min(nh, L - 1)
+    b = np.zeros(G.shape[1])
+    for i in range(1, m + 1):
+        b += G[i] @ p[nh - i] - H[i] @ u[nh - i] + B[i] @ acc[nh - i]
+    return b
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = abi.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("n3,L,nh", [(9, 4, 2), (27, 8, 7), (129, 6, 11), (387, 16, 15), (390, 5, 0)])
+def test_history_random(ctx, n3, L, nh):
+    import torch
+    rng = np.random.default_rng(n3)
+    G, H = rng.standard_normal((L, n3, n3)), rng.standard_normal((L, n3, n3))
+    B = rng.standard_normal((L, n3, 6))
+    p, u, acc = rng.standard_normal((nh, n3)), rng.standard_normal((nh, n3)), rng.standard_normal((nh, 6))
+    dev = torch.device("cuda:0")
+    t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in
+         dict(G=G, H=H, B=B, p=p if nh else np.zeros((1, n3)), u=u if nh else np.zeros((1, n3)),
+              acc=acc if nh else np.zeros((1, 6))).items()}
+    b = torch.full((n3,), float("nan"), dtype=torch.float64, device=dev)
+    print(f"history n3={n3} L={L} nh={nh}", flush=True)
+    ctx.history(n3, L, t["G"].data_ptr(), t["H"].data_ptr(), t["B"].data_ptr(), t["p"].data_ptr(),
+                t["u"].data_ptr(), t["acc"].data_ptr(), nh, b.data_ptr())
+    got = b.cpu().numpy()
+    want = reference(G, H, B, p, u, acc, L) if nh else np.zeros(n3)
+    if nh:
+        assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want)
+    else:
+        assert np.array_equal(got, want)
+
+
+def test_history_on_assembled_matrices(ctx):
+    """The matrices cqbem_assemble leaves in HBM feed the history product directly; a NULL G skips its term."""
+    import torch
+    V, F = M.icosphere(1, 0.2)
+    V = M.center(M.jitter(V, F, 0.01, 1), F)
+    _, c1, c2 = M.material_speeds(5e5, 0.3, 1000.0)
+    L = 16
+    params = abi.make_params(c1, c2, 1000.0, 0.3, 1e-3, 1e-10 ** (1 / (2 * L)), L)
+    n3 = 3 * len(V)
+    dev = torch.device("cuda:0")
+    outs = {"G": torch.empty((L, n3, n3), dtype=torch.float64, device=dev),
+            "H": torch.empty((L, n3, n3), dtype=torch.float64, device=dev),
+            "B": torch.empty((L, n3, 6), dtype=torch.float64, device=dev)}
+    ctx.assemble_device(abi.make_mesh(V, F), params, {k: v.data_ptr() for k, v in outs.items()})
+    rng = np.random.default_rng(7)
+    nh = 20
+    p, u, acc = rng.standard_normal((nh, n3)), rng.standard_normal((nh, n3)), rng.standard_normal((nh, 6))
+    tp, tu, ta = (torch.from_numpy(x).to(dev) for x in (p, u, acc))
+    b = torch.empty(n3, dtype=torch.float64, device=dev)
+    G, H, B = (outs[k].cpu().numpy() for k in ("G", "H", "B"))
+    ctx.history(n3, L, outs["G"].data_ptr(), outs["H"].data_ptr(), outs["B"].data_ptr(), tp.data_ptr(), tu.data_ptr(),
+                ta.data_ptr(), nh, b.data_ptr())
+    want = reference(G, H, B, p, u, acc, L)
+    assert np.linalg.norm(b.cpu().numpy() - want) <= 1e-13 * np.linalg.norm(want)
+    ctx.history(n3, L, 0, outs["H"].data_ptr(), outs["B"].data_ptr(), 0, tu.data_ptr(), ta.data_ptr(), nh, b.data_ptr())
+    want = reference(np.zeros_like(G), H, B, p, u, acc, L)
+    assert np.linalg.norm(b.cpu().numpy() - want) <= 1e-13 * np.linalg.norm(want)
