@@ -707,56 +707,59 @@ __global__ void __launch_bounds__(256) k_transform_wide(const cd *__restrict__ s
 // ------------------------------------------------------------------------
 // History term of one CQ time step on the resident matrices:
 //   b = sum_{i=1..m} (G_i p[nh-i] - H_i u[nh-i] + B_i acc[nh-i]).
-// One wavefront owns kHistRows consecutive rows; lanes stride the columns, so every history
-// vector element is loaded once per kHistRows matrix rows.  Fixed-order wave reduction.
+// One wavefront per row; lanes stride the columns with kHistUnroll independent loads per matrix in
+// flight (the 2 (L-1) row streams are read once, HBM-bound); fixed-order wave reduction.
 // ------------------------------------------------------------------------
-constexpr int kHistRows = 4;
+constexpr int kHistUnroll = 4;
 
+template <bool HG, bool HH>
+__device__ inline double history_row(const double *__restrict__ gr, const double *__restrict__ hr, const double *__restrict__ pv,
+                                     const double *__restrict__ uv, long long n3, int lane, double s) {
+    long long c = lane;
+    for (; c + 64 * (kHistUnroll - 1) < n3; c += 64 * kHistUnroll) {
+        double g[kHistUnroll], h[kHistUnroll], x[kHistUnroll], y[kHistUnroll];
+#pragma unroll
+        for (int k = 0; k < kHistUnroll; k++) {
+            if (HG) g[k] = gr[c + 64 * k], x[k] = pv[c + 64 * k];
+            if (HH) h[k] = hr[c + 64 * k], y[k] = uv[c + 64 * k];
+        }
+#pragma unroll
+        for (int k = 0; k < kHistUnroll; k++) {
+            if (HG) s = fma(g[k], x[k], s);
+            if (HH) s = fma(-h[k], y[k], s);
+        }
+    }
+    for (; c < n3; c += 64) {
+        if (HG) s = fma(gr[c], pv[c], s);
+        if (HH) s = fma(-hr[c], uv[c], s);
+    }
+    return s;
+}
+
+template <bool HG, bool HH>
 __global__ void __launch_bounds__(256) k_history(const double *__restrict__ G, const double *__restrict__ H,
                                                   const double *__restrict__ B, const double *__restrict__ p,
                                                   const double *__restrict__ u, const double *__restrict__ acc,
                                                   long long n3, int m, int nh, double *__restrict__ b) {
-    const long long r0 = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * kHistRows;
+    const long long r = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (r0 >= n3) return;
-    const int nr = (int)(n3 - r0 < kHistRows ? n3 - r0 : kHistRows);
-    double s[kHistRows] = {0.0, 0.0, 0.0, 0.0};
+    if (r >= n3) return;
+    double s = 0.0;
     for (int i = 1; i <= m; i++) {
-        const long long mo = (long long)i * n3 * n3 + r0 * n3;
-        const double *pv = p + (long long)(nh - i) * n3, *uv = u + (long long)(nh - i) * n3;
-        if (G) {
-            for (long long c = lane; c < n3; c += 64) {
-                const double x = pv[c];
-#pragma unroll
-                for (int k = 0; k < kHistRows; k++)
-                    if (k < nr) s[k] = fma(G[mo + k * n3 + c], x, s[k]);
-            }
-        }
-        if (H) {
-            for (long long c = lane; c < n3; c += 64) {
-                const double x = uv[c];
-#pragma unroll
-                for (int k = 0; k < kHistRows; k++)
-                    if (k < nr) s[k] = fma(-H[mo + k * n3 + c], x, s[k]);
-            }
-        }
+        const long long ro = (long long)i * n3 * n3 + r * n3;
+        s = history_row<HG, HH>(HG ? G + ro : nullptr, HH ? H + ro : nullptr, HG ? p + (long long)(nh - i) * n3 : nullptr,
+                                HH ? u + (long long)(nh - i) * n3 : nullptr, n3, lane, s);
     }
 #pragma unroll
-    for (int k = 0; k < kHistRows; k++)
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) s[k] += __shfl_xor(s[k], o, 64);
-    if (lane < nr) {
-        double v = 0.0;
-#pragma unroll
-        for (int k = 0; k < kHistRows; k++)
-            if (k == lane) v = s[k];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) {
         if (B)
             for (int i = 1; i <= m; i++) {
-                const double *br = B + ((long long)i * n3 + r0 + lane) * 6, *av = acc + (long long)(nh - i) * 6;
+                const double *br = B + ((long long)i * n3 + r) * 6, *av = acc + (long long)(nh - i) * 6;
 #pragma unroll
-                for (int k = 0; k < 6; k++) v = fma(br[k], av[k], v);
+                for (int k = 0; k < 6; k++) s = fma(br[k], av[k], s);
             }
-        b[r0 + lane] = v;
+        b[r] = s;
     }
 }
 
@@ -876,8 +879,15 @@ hipError_t launch_transform(const cd *spec, long long E, int Lf, const double2 *
 
 hipError_t launch_history(const double *G, const double *H, const double *B, const double *p, const double *u,
                           const double *acc, long long n3, int m, int nh, double *b, hipStream_t st) {
-    const long long waves = (n3 + kHistRows - 1) / kHistRows;
-    hipLaunchKernelGGL(k_history, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, G, H, B, p, u, acc, n3, m, nh, b);
+    const dim3 grid((unsigned)((n3 + 3) / 4));
+    if (G && H)
+        hipLaunchKernelGGL((k_history<true, true>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, m, nh, b);
+    else if (G)
+        hipLaunchKernelGGL((k_history<true, false>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, m, nh, b);
+    else if (H)
+        hipLaunchKernelGGL((k_history<false, true>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, m, nh, b);
+    else
+        hipLaunchKernelGGL((k_history<false, false>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, m, nh, b);
     return hipGetLastError();
 }
 
