@@ -528,8 +528,10 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
                 {
                     Timer t(st, &tm[4]);
                     const long long E = 3LL * nr * n3;
-                    if (wantG) HIPCHK(launch_transform(specG.as<cd>(), E, Lf, tab, L, G + 3LL * r0 * n3, n3 * n3, ctx->num_cus, st));
-                    if (wantH) HIPCHK(launch_transform(specH.as<cd>(), E, Lf, tab, L, H + 3LL * r0 * n3, n3 * n3, ctx->num_cus, st));
+                    if (wantG)
+                        HIPCHK(launch_transform(specG.as<cd>(), E, lstride, Lf, tab, L, G + 3LL * r0 * n3, n3 * n3, ctx->num_cus, st));
+                    if (wantH)
+                        HIPCHK(launch_transform(specH.as<cd>(), E, lstride, Lf, tab, L, H + 3LL * r0 * n3, n3 * n3, ctx->num_cus, st));
                 }
             }
         }
@@ -561,8 +563,8 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
             run_galerkin(gj, P.vs_ptr, P.vs_face, mesh->F, specG.as<cd>(), specH.as<cd>(), wantB ? specB.as<cd>() : nullptr,
                          wantB ? specBa.as<cd>() : nullptr, tm, st);
             Timer t(st, &tm[4]);
-            if (wantG) HIPCHK(launch_transform(specG.as<cd>(), n3 * n3, Lf, tab, L, G, n3 * n3, ctx->num_cus, st));
-            if (wantH) HIPCHK(launch_transform(specH.as<cd>(), n3 * n3, Lf, tab, L, H, n3 * n3, ctx->num_cus, st));
+            if (wantG) HIPCHK(launch_transform(specG.as<cd>(), n3 * n3, n3 * n3, Lf, tab, L, G, n3 * n3, ctx->num_cus, st));
+            if (wantH) HIPCHK(launch_transform(specH.as<cd>(), n3 * n3, n3 * n3, Lf, tab, L, H, n3 * n3, ctx->num_cus, st));
         }
         if (wantH) {
             // 1/2 P1 mass matrix per component (elastics_bem_matrix_collection.cpp:447-467), summed per entry on the host
@@ -583,8 +585,8 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
     }
     if (wantB) {
         Timer t(st, &tm[4]);
-        HIPCHK(launch_transform(specB.as<cd>(), n3 * 3, Lf, tab, L, Wbt, n3 * 3, ctx->num_cus, st));
-        HIPCHK(launch_transform(specBa.as<cd>(), n3 * 9, Lf, tab, L, Wba, n3 * 9, ctx->num_cus, st));
+        HIPCHK(launch_transform(specB.as<cd>(), n3 * 3, n3 * 3, Lf, tab, L, Wbt, n3 * 3, ctx->num_cus, st));
+        HIPCHK(launch_transform(specBa.as<cd>(), n3 * 9, n3 * 9, Lf, tab, L, Wba, n3 * 9, ctx->num_cus, st));
         HIPCHK(launch_b_post(Wbt, Wba, W6, g.V, nv, L, st));
     }
     if (!out->on_device && sync_outputs) {

@@ -641,7 +641,7 @@ __global__ void __launch_bounds__(64) k_colloc_singular(Geom g, const double2 *_
 constexpr int kTransformNC = 32, kTransformLC = 8, kTransformMaxLf = 64;
 
 template <int NC, int LC>
-__global__ void __launch_bounds__(256) k_transform(const cd *__restrict__ spec, long long E, int Lf,
+__global__ void __launch_bounds__(256) k_transform(const cd *__restrict__ spec, long long E, long long spec_ls, int Lf,
                                                     const double2 *__restrict__ tab, int L, int n0,
                                                     double *__restrict__ out, long long out_nstride) {
     __shared__ double2 T[NC * kTransformMaxLf];  // [l][k]
@@ -658,7 +658,7 @@ __global__ void __launch_bounds__(256) k_transform(const cd *__restrict__ spec, 
         for (int l0 = 0; l0 < Lf; l0 += LC) {
             cd X[LC];
 #pragma unroll
-            for (int c = 0; c < LC; c++) X[c] = l0 + c < Lf ? spec[(long long)(l0 + c) * E + e] : cd{0.0, 0.0};
+            for (int c = 0; c < LC; c++) X[c] = l0 + c < Lf ? spec[(long long)(l0 + c) * spec_ls + e] : cd{0.0, 0.0};
 #pragma unroll
             for (int c = 0; c < LC; c++) {
                 if (l0 + c >= Lf) break;
@@ -677,7 +677,7 @@ __global__ void __launch_bounds__(256) k_transform(const cd *__restrict__ spec, 
 
 // Same for Lf > kTransformMaxLf, with the table read from global memory.
 template <int NC>
-__global__ void __launch_bounds__(256) k_transform_wide(const cd *__restrict__ spec, long long E, int Lf,
+__global__ void __launch_bounds__(256) k_transform_wide(const cd *__restrict__ spec, long long E, long long spec_ls, int Lf,
                                                          const double2 *__restrict__ tab, int L, int n0,
                                                          double *__restrict__ out, long long out_nstride) {
     const long long stride = (long long)gridDim.x * 256;
@@ -686,7 +686,7 @@ __global__ void __launch_bounds__(256) k_transform_wide(const cd *__restrict__ s
 #pragma unroll
         for (int k = 0; k < NC; k++) acc[k] = 0.0;
         for (int l = 0; l < Lf; l++) {
-            const cd X = spec[(long long)l * E + e];
+            const cd X = spec[(long long)l * spec_ls + e];
 #pragma unroll
             for (int k = 0; k < NC; k++) {
                 const int n = n0 + k;
@@ -858,7 +858,8 @@ hipError_t launch_colloc_singular(int kind, const Geom &g, const double2 *line, 
     return hipGetLastError();
 }
 
-hipError_t launch_transform(const cd *spec, long long E, int Lf, const double2 *tab, int L, double *out, long long out_nstride,
+hipError_t launch_transform(const cd *spec, long long E, long long spec_ls, int Lf, const double2 *tab, int L, double *out,
+                            long long out_nstride,
                             int num_cus, hipStream_t st) {
     long long blocks = (E + 255) / 256;
     const long long cap = (long long)num_cus * 64;
@@ -866,11 +867,11 @@ hipError_t launch_transform(const cd *spec, long long E, int Lf, const double2 *
     if (blocks < 1) blocks = 1;
     for (int n0 = 0; n0 < L; n0 += kTransformNC) {
         if (Lf <= kTransformMaxLf)
-            hipLaunchKernelGGL((k_transform<kTransformNC, kTransformLC>), dim3((unsigned)blocks), dim3(256), 0, st, spec, E, Lf,
-                               tab, L, n0, out, out_nstride);
+            hipLaunchKernelGGL((k_transform<kTransformNC, kTransformLC>), dim3((unsigned)blocks), dim3(256), 0, st, spec, E,
+                               spec_ls, Lf, tab, L, n0, out, out_nstride);
         else
-            hipLaunchKernelGGL((k_transform_wide<kTransformNC>), dim3((unsigned)blocks), dim3(256), 0, st, spec, E, Lf, tab, L,
-                               n0, out, out_nstride);
+            hipLaunchKernelGGL((k_transform_wide<kTransformNC>), dim3((unsigned)blocks), dim3(256), 0, st, spec, E, spec_ls, Lf,
+                               tab, L, n0, out, out_nstride);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

@@ -38,7 +38,10 @@ hipError_t launch_colloc_b(const Geom &g, const double4 *quad, int npts, const F
 hipError_t launch_colloc_singular(int kind, const Geom &g, const double2 *line, int nline, const Freq *freqs, int l0, int nl,
                                   const Mat &m, int row0, int nrows, cd *U, long long ld, long long lstride, cd *Bt,
                                   cd *Ba, hipStream_t st);
-hipError_t launch_transform(const cd *spec, long long E, int Lf, const double2 *tab, int L, double *out, long long out_nstride,
+// out[n * out_nstride + e] = sum_l Re(spec[l * spec_ls + e] T(n, l)) for e < E (spec_ls >= E: a slab buffer may be
+// sized for more rows than the slab holds).
+hipError_t launch_transform(const cd *spec, long long E, long long spec_ls, int Lf, const double2 *tab, int L, double *out,
+                            long long out_nstride,
                             int num_cus, hipStream_t st);
 hipError_t launch_add_c_colloc(double *H0, const double *C, int nv, hipStream_t st);
 // b = sum_{i=1..m} (G_i p[nh-i] - H_i u[nh-i] + B_i acc[nh-i]); any of G, H, B may be null.

@@ -182,3 +182,34 @@ def test_dimensional_scaling_full_size(ctx, which_mesh):
             assert d < 1e-8, k
         else:
             assert np.array_equal(x, y), k
+
+
+def test_row_slabs_are_bitwise_identical():
+    """A small HBM budget splits the collocation rows into several slabs (the path large meshes take); every
+    row's quadrature and transform are independent of the slab it falls in, so the result is bit-identical."""
+    import torch
+    V, F = M.icosphere(2, 0.2)
+    V = M.center(M.jitter(V, F, 0.01, 4), F)
+    c1, c2 = material(5e5, 0.3)
+    L = 16
+    params = abi.make_params(c1, c2, RHO, 0.3, 1e-3, 1e-10 ** (1 / (2 * L)), L)
+    mesh = abi.make_mesh(V, F)
+    n3 = 3 * len(V)
+    dev = torch.device("cuda:0")
+    shapes = {"G": (L, n3, n3), "H": (L, n3, n3), "B": (L, n3, 6)}
+    full = abi.Context(0)
+    per_row = 2 * (L // 2 + 1) * 9 * len(V) * 16  # G and H spectra of one collocation row
+    fixed = abi.job_footprint(mesh, params, outputs_on_device=True) - len(V) * per_row  # all rows fit 8 GiB
+    small = abi.Context(0, hbm_budget_bytes=fixed + 21 * per_row)  # 20-row slabs: 9 slabs
+    try:
+        res = []
+        for ctx in (full, small):
+            outs = {k: torch.full(s, float("nan"), dtype=torch.float64, device=dev) for k, s in shapes.items()}
+            ctx.assemble_device(mesh, params, {k: t.data_ptr() for k, t in outs.items()})
+            res.append({k: t.cpu().numpy() for k, t in outs.items()})
+        for k in shapes:
+            assert np.isfinite(res[1][k]).all(), k
+            assert np.array_equal(res[0][k], res[1][k]), k
+    finally:
+        full.close()
+        small.close()
