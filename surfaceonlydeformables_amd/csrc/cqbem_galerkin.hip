@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -542,6 +544,54 @@ __global__ void k_gal_gather_b(const int *__restrict__ vptr, const int *__restri
     for (int e = 0; e < 27; e++) ao[e] += ba[e];
 }
 
+// Test points of the faces faces[0..nfl) in that order: tpts[fi * npts + a] = (x_a, face) with x_a the a-th rule
+// point of the face, and psi[3 (fi * npts + a) + k] = w_a area phi_k(x_a), the weight of x_a for test corner k.
+__global__ void k_gal_points(Geom g, const double4 *__restrict__ quad, int npts, const int *__restrict__ faces, int nfl,
+                             double4 *__restrict__ tpts, double *__restrict__ psi) {
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long long)nfl * npts) return;
+    const int fi = (int)(t / npts), a = (int)(t % npts);
+    const int f1 = faces[fi];
+    const double *X0 = g.V + 3 * g.F[3 * f1], *X1 = g.V + 3 * g.F[3 * f1 + 1], *X2 = g.V + 3 * g.F[3 * f1 + 2];
+    const double4 Qa = quad[a];
+    const double x0 = Qa.x * X0[0] + Qa.y * X1[0] + Qa.z * X2[0];
+    const double x1 = Qa.x * X0[1] + Qa.y * X1[1] + Qa.z * X2[1];
+    const double x2 = Qa.x * X0[2] + Qa.y * X1[2] + Qa.z * X2[2];
+    tpts[t] = make_double4(x0, x1, x2, (double)f1);
+    const double wa = Qa.w * g.area[f1];
+    psi[3 * t] = wa * Qa.x, psi[3 * t + 1] = wa * Qa.y, psi[3 * t + 2] = wa * Qa.z;
+}
+
+// Projection of per-point rows onto the test basis.  src holds, per frequency (stride src_ls), rows 3p..3p+2 of
+// ew complex entries for the local test points p = fl * npts + a of the chunk faces faces[0..nfc); the faces of a
+// chunk share no vertex, so dst rows 3 F[f][k] + r are owned by one thread each:
+//   dst[l][3 F[f][k] + r][c] += sum_a psi[k of point row0 + p] src[l][3 p + r][c].
+__global__ void k_gal_project(const cd *__restrict__ src, long long ew, long long src_ls, const double *__restrict__ psi,
+                              const int *__restrict__ faces, int nfc, int npts, long long row0, const int *__restrict__ F,
+                              cd *__restrict__ dst, long long dst_ls) {
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long long)nfc * 3 * ew) return;
+    const int l = blockIdx.y;
+    const int fl = (int)(t / (3 * ew));
+    const long long rem = t - (long long)fl * 3 * ew;
+    const int r = (int)(rem / ew);
+    const long long c = rem - r * ew;
+    const cd *s = src + (long long)l * src_ls + (3LL * fl * npts + r) * ew + c;
+    const double *w = psi + 3 * (row0 + (long long)fl * npts);
+    cd a0 = {0, 0}, a1 = {0, 0}, a2 = {0, 0};
+    for (int a = 0; a < npts; a++) {
+        const cd v = s[3LL * a * ew];
+        fma_cr(a0, v, w[3 * a]);
+        fma_cr(a1, v, w[3 * a + 1]);
+        fma_cr(a2, v, w[3 * a + 2]);
+    }
+    const int f = faces[fl];
+    cd *d = dst + (long long)l * dst_ls + r * ew + c;
+    d[3LL * F[3 * f] * ew] += a0;
+    d[3LL * F[3 * f + 1] * ew] += a1;
+    d[3LL * F[3 * f + 2] * ew] += a2;
+}
+
 __global__ void k_fill_zero(double2 *p, long long n) {
     const long long stride = (long long)gridDim.x * 256;
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) p[i] = make_double2(0.0, 0.0);
@@ -587,6 +637,21 @@ struct EvTimer {
 
 }  // namespace
 
+int galerkin_chunk_faces(int nv, int nf, int npts, int nl, bool G, bool H, bool B, size_t budget) {
+    const size_t per_face = (size_t)npts * 3 * nl * ((G ? 3ULL * nv : 0) + (H ? 3ULL * nv : 0) + (B ? 12 : 0)) * sizeof(cd);
+    size_t cap = per_face ? budget / per_face : (size_t)nf;
+    cap = std::max<size_t>(cap, 1);
+    cap = std::min<size_t>(cap, (size_t)(65535 / std::max(npts, 1)));
+    return (int)std::min<size_t>(cap, (size_t)std::max(nf, 1));
+}
+
+size_t galerkin_scratch_bytes(int nv, int nf, int npts, int nl, bool G, bool H, bool B, size_t budget) {
+    const size_t cap = (size_t)galerkin_chunk_faces(nv, nf, npts, nl, G, H, B, budget);
+    const size_t rows = 3 * cap * npts;
+    return (size_t)nf * npts * (sizeof(double4) + 3 * sizeof(double)) +
+           (size_t)nl * rows * ((G ? 3ULL * nv : 0) + (H ? 3ULL * nv : 0) + (B ? 12 : 0)) * sizeof(cd);
+}
+
 void run_galerkin(const GalerkinJob &job, const std::vector<int> &vs_ptr, const std::vector<int> &vs_face, const int *F,
                   cd *U, cd *P, cd *Bt, cd *Ba, double *tm, hipStream_t st) {
     const Geom &g = job.g;
@@ -621,7 +686,60 @@ void run_galerkin(const GalerkinJob &job, const std::vector<int> &vs_ptr, const 
     DevVec dfaces;
     const int *faces = dfaces.up(flat, st);
 
-    if (U || P || Bt || Ba) {
+    const char *variant = std::getenv("CQBEM_GALERKIN_KERNEL");
+    if ((U || P || Bt || Ba) && !(variant && !std::strcmp(variant, "star"))) {
+        // Test-point rows: the collocation kernels evaluate, for every rule point x_a of the test faces of a
+        // chunk, the row sum over the source faces not touching x_a's face; k_gal_project then weights the
+        // rows by the test basis.  Chunks are runs of one colour class, so their faces share no vertex.
+        EvTimer t(st, &tm[1]);
+        const int npts = job.npts;
+        const long long ntp = (long long)nf * npts;
+        DevVec dtp, dpsi, dsG, dsH, dsBt, dsBa;
+        check(hipMalloc(&dtp.p, (size_t)ntp * sizeof(double4)), "hipMalloc test points");
+        check(hipMalloc(&dpsi.p, (size_t)ntp * 3 * sizeof(double)), "hipMalloc test weights");
+        double4 *tpts = (double4 *)dtp.p;
+        double *psi = (double *)dpsi.p;
+        hipLaunchKernelGGL(k_gal_points, dim3((unsigned)((ntp + 255) / 256)), dim3(256), 0, st, g, job.quad, npts, faces, nf,
+                           tpts, psi);
+        check(hipGetLastError(), "k_gal_points");
+        const int cap = galerkin_chunk_faces(nv, nf, npts, nl, U != nullptr, P != nullptr, Bt && Ba, job.scratch_bytes);
+        const long long cap_rows = 3LL * cap * npts, ls = cap_rows * n3;
+        if (U) check(hipMalloc(&dsG.p, (size_t)nl * ls * sizeof(cd)), "hipMalloc point rows G");
+        if (P) check(hipMalloc(&dsH.p, (size_t)nl * ls * sizeof(cd)), "hipMalloc point rows H");
+        if (Bt && Ba) {
+            check(hipMalloc(&dsBt.p, (size_t)nl * cap_rows * 3 * sizeof(cd)), "hipMalloc point rows Bt");
+            check(hipMalloc(&dsBa.p, (size_t)nl * cap_rows * 9 * sizeof(cd)), "hipMalloc point rows Ba");
+        }
+        cd *sG = (cd *)dsG.p, *sH = (cd *)dsH.p, *sBt = (cd *)dsBt.p, *sBa = (cd *)dsBa.p;
+        auto project = [&](const cd *src, long long ew, long long src_ls, const int *cf, int nfc, long long row0, cd *dst,
+                           long long dst_ls) {
+            const long long n = (long long)nfc * 3 * ew;
+            hipLaunchKernelGGL(k_gal_project, dim3((unsigned)((n + 255) / 256), nl), dim3(256), 0, st, src, ew, src_ls, psi, cf,
+                               nfc, npts, row0, g.F, dst, dst_ls);
+            check(hipGetLastError(), "k_gal_project");
+        };
+        for (size_t c = 0; c < classes.size(); c++) {
+            const int cnt = cls_off[c + 1] - cls_off[c];
+            for (int b = 0; b < cnt; b += cap) {
+                const int nfc = std::min(cap, cnt - b), fi0 = cls_off[c] + b;
+                const int row0 = fi0 * npts, nrows = nfc * npts;
+                if (U || P) {
+                    check(launch_colloc_gh_pair(g, job.pair_pts, job.npts_even, job.freqs, 0, nl, job.mat, row0, nrows, tpts,
+                                                job.patch_pass, job.pass_info, job.pass_faces, job.patch_verts, job.npatches,
+                                                sG, sH, n3, ls, st),
+                          "k_colloc_gh_pair (test points)");
+                    if (U) project(sG, n3, ls, faces + fi0, nfc, row0, U, n3 * n3);
+                    if (P) project(sH, n3, ls, faces + fi0, nfc, row0, P, n3 * n3);
+                }
+                if (Bt && Ba) {
+                    check(launch_points_b(g, job.quad, npts, job.freqs, 0, nl, job.mat, tpts, row0, nrows, sBt, sBa, cap_rows, st),
+                          "k_colloc_b (test points)");
+                    project(sBt, 3, cap_rows * 3, faces + fi0, nfc, row0, Bt, n3 * 3);
+                    project(sBa, 9, cap_rows * 9, faces + fi0, nfc, row0, Ba, n3 * 9);
+                }
+            }
+        }
+    } else if (U || P || Bt || Ba) {
         EvTimer t(st, &tm[1]);
         for (size_t c = 0; c < classes.size(); c++) {
             const int cnt = cls_off[c + 1] - cls_off[c];

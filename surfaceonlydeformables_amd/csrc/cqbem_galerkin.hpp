@@ -18,7 +18,21 @@ struct GalerkinJob {
     int nl;               // number of frequencies
     Mat mat;
     int num_cus;
+    // pair-form tables of the collocation G/H kernel (column patches; see launch_colloc_gh_pair)
+    const double4 *pair_pts;
+    int npts_even;
+    const int *patch_pass;
+    const int2 *pass_info;
+    const int4 *pass_faces;
+    const int *patch_verts;
+    int npatches;
+    size_t scratch_bytes;  // bound of the per-test-point row scratch (at least one face is always processed)
 };
+
+// Test faces per chunk of the per-test-point row scratch, and the device bytes the scratch takes
+// (test points, their basis weights, and one chunk of G/H/B rows for nl frequencies).
+int galerkin_chunk_faces(int nv, int nf, int npts, int nl, bool G, bool H, bool B, size_t budget);
+size_t galerkin_scratch_bytes(int nv, int nf, int npts, int nl, bool G, bool H, bool B, size_t budget);
 
 // Accumulates the Laplace-domain spectra of all frequencies of `job` into
 // U, P (nl x 3nv x 3nv), Bt (nl x 3nv x 3), Ba (nl x 3nv x 9); any may be null.

@@ -361,7 +361,7 @@ hipError_t launch_gh(const DevMesh &d, const Prepared &P, const Freq *freqs, int
     if (colloc_star())
         return launch_colloc_gh(d.g, d.quad.as<double4>(), P.npts, freqs, l0, nl, P.mat, row0, nrows, UG, UH, ld, lstride, st);
     return launch_colloc_gh_pair(d.g, d.pair_pts.as<double4>(), (int)P.quad.size(), freqs, l0, nl, P.mat, row0, nrows,
-                                 d.pair_patch_pass.as<int>(), d.pair_pass_info.as<int2>(), d.pair_pass_faces.as<int4>(),
+                                 nullptr, d.pair_patch_pass.as<int>(), d.pair_pass_info.as<int2>(), d.pair_pass_faces.as<int4>(),
                                  d.pair_patch_verts.as<int>(), P.pair_npatches, UG, UH, ld, lstride, st);
 }
 
@@ -416,6 +416,10 @@ struct Timer {
     }
 };
 
+// Bound of the Galerkin per-test-point row scratch within a job's scratch budget: larger chunks only save
+// launches, the traffic per face is the same.
+size_t galerkin_budget(size_t slab_budget) { return std::min<size_t>(slab_budget, (size_t)8 << 30); }
+
 // Device footprint of a job: scratch spectra (one slab), B spectra/outputs and, when
 // outputs are not caller-provided device buffers, the staged outputs.
 uint64_t footprint(const cqbem_mesh *mesh, const cqbem_params *p, int outputs_on_device, size_t slab_budget) {
@@ -445,6 +449,10 @@ uint64_t footprint(const cqbem_mesh *mesh, const cqbem_params *p, int outputs_on
             npairs += (uint64_t)(std::unique(cand.begin(), cand.end()) - cand.begin());
         }
         bytes += npairs * ((uint64_t)Lf * 270 * sizeof(cd) + 256);
+        const int npts = (int)composite_tri_rule(p->order, p->subdivision).size();
+        const bool B = (p->which & (CQBEM_B_TRANS | CQBEM_B_ANGULAR)) != 0;
+        bytes += galerkin_scratch_bytes(nv, (int)mesh->nf, npts, Lf, (p->which & CQBEM_G) != 0, (p->which & CQBEM_H) != 0, B,
+                                        galerkin_budget(slab_budget));
     }
     bytes += (uint64_t)Lf * 3 * nv * 12 * sizeof(cd) + (uint64_t)L * 3 * nv * 18 * sizeof(double);
     bytes += (uint64_t)mesh->nf * 64 + (uint64_t)nv * 64 + (1 << 20);
@@ -552,7 +560,9 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
             HIPCHK(launch_add_c_colloc(H, dC.as<double>(), nv, st));
         }
     } else {
-        GalerkinJob gj{g, quad, npts, line, nline, freqs, Lf, P.mat, ctx->num_cus};
+        GalerkinJob gj{g, quad, npts, line, nline, freqs, Lf, P.mat, ctx->num_cus, d.pair_pts.as<double4>(), (int)P.quad.size(),
+                       d.pair_patch_pass.as<int>(), d.pair_pass_info.as<int2>(), d.pair_pass_faces.as<int4>(),
+                       d.pair_patch_verts.as<int>(), P.pair_npatches, galerkin_budget(slab_budget)};
         const int nmat = (wantG ? 1 : 0) + (wantH ? 1 : 0);
         if (nmat || wantB) {
             // Galerkin spectra are assembled for the full matrix (face pairs scatter to any row)
@@ -760,7 +770,9 @@ int cqbem_laplace(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params
                                               dBa.as<cd>(), st));
             }
         } else {
-            GalerkinJob gj{d.g, quad, npts, line, nline, freqs + l, 1, P.mat, ctx->num_cus};
+            GalerkinJob gj{d.g, quad, npts, line, nline, freqs + l, 1, P.mat, ctx->num_cus, d.pair_pts.as<double4>(),
+                           (int)P.quad.size(), d.pair_patch_pass.as<int>(), d.pair_pass_info.as<int2>(),
+                           d.pair_pass_faces.as<int4>(), d.pair_patch_verts.as<int>(), P.pair_npatches, (size_t)1 << 30};
             double tm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             run_galerkin(gj, P.vs_ptr, P.vs_face, mesh->F, wG ? dU.as<cd>() : nullptr, wH ? dP.as<cd>() : nullptr,
                          wB ? dBt.as<cd>() : nullptr, wB ? dBa.as<cd>() : nullptr, tm, st);

@@ -155,9 +155,10 @@ __device__ inline double pair_swap(double v) {
     return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
-template <bool DG, bool DH>
+template <bool DG, bool DH, bool GAL>
 __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, const double4 *__restrict__ pts, int nrounds,
                                                                   const Freq *__restrict__ freqs, int l0, Mat m, int row0,
+                                                                  const double4 *__restrict__ tpts,
                                                                   const int *__restrict__ patch_pass,
                                                                   const int2 *__restrict__ pass_info,
                                                                   const int4 *__restrict__ pass_faces,
@@ -169,7 +170,17 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
     const int i = row0 + il;
     const int tid = threadIdx.x, pr = tid >> 1, side = tid & 1;
     const Freq f = freqs[l0 + l];
-    const double x0 = g.V[3 * i], x1 = g.V[3 * i + 1], x2 = g.V[3 * i + 2];
+    // the row point: vertex i (collocation), or test point i with its face's corners t[] excluded (Galerkin)
+    double x0, x1, x2;
+    int t[3] = {i, i, i};
+    if (GAL) {
+        const double4 T = tpts[i];
+        x0 = T.x, x1 = T.y, x2 = T.z;
+        const int f1 = (int)T.w;
+        t[0] = g.F[3 * f1], t[1] = g.F[3 * f1 + 1], t[2] = g.F[3 * f1 + 2];
+    } else {
+        x0 = g.V[3 * i], x1 = g.V[3 * i + 1], x2 = g.V[3 * i + 2];
+    }
     for (int k = tid; k < kPairWords * kPairPV; k += kPairThreads) acc[k] = 0.0;
 
     for (int pass = patch_pass[pc]; pass < patch_pass[pc + 1]; ++pass) {
@@ -181,7 +192,8 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
         if (active) {
             fe = pass_faces[info.x + pr];
             const int a0 = g.F[3 * fe.x], a1 = g.F[3 * fe.x + 1], a2 = g.F[3 * fe.x + 2];
-            active = !(same_vertex(g, i, a0) || same_vertex(g, i, a1) || same_vertex(g, i, a2));
+            for (int k = 0; k < (GAL ? 3 : 1); k++)
+                active = active && !(same_vertex(g, t[k], a0) || same_vertex(g, t[k], a1) || same_vertex(g, t[k], a2));
         }
         cd accF[16], accX[16];
 #pragma unroll
@@ -329,14 +341,28 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
 // ------------------------------------------------------------------------
 constexpr int kBWords = 48;  // B_trans (6 sym complex) + B_angular (18 complex), as doubles
 
+// GAL: the rows are test points tpts[row0 + y] (x, y, z, face) whose face's corners are excluded, written to
+// rows 3y..3y+2 of an nv3-row buffer; otherwise collocation vertex i = row0 + y, written to rows 3i..3i+2.
+template <bool GAL>
 __global__ void __launch_bounds__(256) k_colloc_b(Geom g, const double4 *__restrict__ quad, int npts,
                                                    const Freq *__restrict__ freqs, int l0, Mat m, int row0,
-                                                   cd *__restrict__ Bt, cd *__restrict__ Ba, long long nv3) {
+                                                   const double4 *__restrict__ tpts, cd *__restrict__ Bt,
+                                                   cd *__restrict__ Ba, long long nv3) {
     __shared__ double red[4][kBWords];
     const int l = blockIdx.x;
     const int i = row0 + blockIdx.y;
     const Freq f = freqs[l0 + l];
-    const double x0 = g.V[3 * i], x1 = g.V[3 * i + 1], x2 = g.V[3 * i + 2];
+    double x0, x1, x2;
+    int t[3] = {i, i, i};
+    if (GAL) {
+        const double4 T = tpts[i];
+        x0 = T.x, x1 = T.y, x2 = T.z;
+        const int f1 = (int)T.w;
+        t[0] = g.F[3 * f1], t[1] = g.F[3 * f1 + 1], t[2] = g.F[3 * f1 + 2];
+    } else {
+        x0 = g.V[3 * i], x1 = g.V[3 * i + 1], x2 = g.V[3 * i + 2];
+    }
+    const int io = GAL ? (int)blockIdx.y : i;
     cd bt[6], ba[18];
 #pragma unroll
     for (int k = 0; k < 6; k++) bt[k] = cd{0, 0};
@@ -344,7 +370,10 @@ __global__ void __launch_bounds__(256) k_colloc_b(Geom g, const double4 *__restr
     for (int k = 0; k < 18; k++) ba[k] = cd{0, 0};
     for (int fc = threadIdx.x; fc < g.nf; fc += 256) {
         const int a0 = g.F[3 * fc], a1 = g.F[3 * fc + 1], a2 = g.F[3 * fc + 2];
-        if (same_vertex(g, i, a0) || same_vertex(g, i, a1) || same_vertex(g, i, a2)) continue;
+        bool touch = false;
+        for (int k = 0; k < (GAL ? 3 : 1); k++)
+            touch = touch || same_vertex(g, t[k], a0) || same_vertex(g, t[k], a1) || same_vertex(g, t[k], a2);
+        if (touch) continue;
         const double *p0 = g.V + 3 * a0, *p1 = g.V + 3 * a1, *p2 = g.V + 3 * a2;
         const double n[3] = {g.N[3 * fc], g.N[3 * fc + 1], g.N[3 * fc + 2]};
         const double ar = g.area[fc];
@@ -413,13 +442,13 @@ __global__ void __launch_bounds__(256) k_colloc_b(Geom g, const double4 *__restr
         const int c = k & 1;
         if (k < 12) {
             const int e = k >> 1;
-            double *bt_out = reinterpret_cast<double *>(Bt + (long long)l * nv3 * 3 + (long long)(3 * i) * 3);
+            double *bt_out = reinterpret_cast<double *>(Bt + (long long)l * nv3 * 3 + (long long)(3 * io) * 3);
             for (int a = 0; a < 3; a++)
                 for (int b = 0; b < 3; b++)
                     if (sidx(a, b) == e) bt_out[2 * (a * 3 + b) + c] = v;
         } else {
             const int e = (k - 12) >> 1, sab = e / 3, kk = e % 3;
-            double *ba_out = reinterpret_cast<double *>(Ba + (long long)l * nv3 * 9 + (long long)(3 * i) * 9);
+            double *ba_out = reinterpret_cast<double *>(Ba + (long long)l * nv3 * 9 + (long long)(3 * io) * 9);
             for (int a = 0; a < 3; a++)
                 for (int b = 0; b < 3; b++)
                     if (sidx(a, b) == sab) ba_out[2 * (a * 9 + b * 3 + kk) + c] = v;
@@ -815,20 +844,24 @@ hipError_t launch_colloc_gh(const Geom &g, const double4 *quad, int npts, const 
 
 
 hipError_t launch_colloc_gh_pair(const Geom &g, const double4 *pts, int npts_even, const Freq *freqs, int l0, int nl, const Mat &m,
-                                 int row0, int nrows, const int *patch_pass, const int2 *pass_info, const int4 *pass_faces,
-                                 const int *patch_verts, int npatches, cd *UG, cd *UH, long long ld, long long lstride,
-                                 hipStream_t st) {
-    if ((!UG && !UH) || (npts_even & 1)) return hipErrorInvalidValue;
+                                 int row0, int nrows, const double4 *tpts, const int *patch_pass, const int2 *pass_info,
+                                 const int4 *pass_faces, const int *patch_verts, int npatches, cd *UG, cd *UH, long long ld,
+                                 long long lstride, hipStream_t st) {
+    if ((!UG && !UH) || (npts_even & 1) || nrows <= 0 || nrows > 65535) return hipErrorInvalidValue;
     dim3 grid(npatches, nl, nrows);
-    if (UG && UH)
-        hipLaunchKernelGGL((k_colloc_gh_pair<true, true>), grid, dim3(kPairThreads), 0, st, g, pts, npts_even / 2, freqs, l0, m,
-                           row0, patch_pass, pass_info, pass_faces, patch_verts, UG, UH, ld, lstride);
-    else if (UG)
-        hipLaunchKernelGGL((k_colloc_gh_pair<true, false>), grid, dim3(kPairThreads), 0, st, g, pts, npts_even / 2, freqs, l0,
-                           m, row0, patch_pass, pass_info, pass_faces, patch_verts, UG, UH, ld, lstride);
-    else
-        hipLaunchKernelGGL((k_colloc_gh_pair<false, true>), grid, dim3(kPairThreads), 0, st, g, pts, npts_even / 2, freqs, l0,
-                           m, row0, patch_pass, pass_info, pass_faces, patch_verts, UG, UH, ld, lstride);
+#define CQBEM_PAIR(DG, DH, GAL)                                                                                           \
+    hipLaunchKernelGGL((k_colloc_gh_pair<DG, DH, GAL>), grid, dim3(kPairThreads), 0, st, g, pts, npts_even / 2, freqs, l0, m, \
+                       row0, tpts, patch_pass, pass_info, pass_faces, patch_verts, UG, UH, ld, lstride)
+    if (tpts) {
+        if (UG && UH) CQBEM_PAIR(true, true, true);
+        else if (UG) CQBEM_PAIR(true, false, true);
+        else CQBEM_PAIR(false, true, true);
+    } else {
+        if (UG && UH) CQBEM_PAIR(true, true, false);
+        else if (UG) CQBEM_PAIR(true, false, false);
+        else CQBEM_PAIR(false, true, false);
+    }
+#undef CQBEM_PAIR
     return hipGetLastError();
 }
 int pair_patch_size() { return kPairPV; }
@@ -836,8 +869,17 @@ int pair_pass_faces() { return kPairFaces; }
 
 hipError_t launch_colloc_b(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
                            int row0, int nrows, cd *Bt, cd *Ba, hipStream_t st) {
-    dim3 grid(nl, nrows);
-    hipLaunchKernelGGL(k_colloc_b, grid, dim3(256), 0, st, g, quad, npts, freqs, l0, m, row0, Bt, Ba, 3LL * g.nv);
+    if (nrows <= 0 || nrows > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_colloc_b<false>, dim3(nl, nrows), dim3(256), 0, st, g, quad, npts, freqs, l0, m, row0,
+                       (const double4 *)nullptr, Bt, Ba, 3LL * g.nv);
+    return hipGetLastError();
+}
+
+hipError_t launch_points_b(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
+                           const double4 *tpts, int row0, int nrows, cd *Bt, cd *Ba, long long out_rows, hipStream_t st) {
+    if (nrows <= 0 || nrows > 65535 || 3LL * nrows > out_rows) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_colloc_b<true>, dim3(nl, nrows), dim3(256), 0, st, g, quad, npts, freqs, l0, m, row0, tpts, Bt, Ba,
+                       out_rows);
     return hipGetLastError();
 }
 

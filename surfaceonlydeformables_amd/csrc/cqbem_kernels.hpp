@@ -27,14 +27,21 @@ hipError_t launch_colloc_gh(const Geom &g, const double4 *quad, int npts, const 
 // (face, corner 0, corner 1, corner 2) with corner = slot | sub-step << 10 or -1 outside the patch.
 // `pts` holds, for each point q of the (even-padded) rule, (b0, b1, b2, w) and (phi_F(q), phi_F(partner), 0, 0)
 // with phi_F the barycentric of corner 0 for even q (lane A) and of corner 2 for odd q (lane B), partner = q ^ 1.
+// Rows are collocation vertices row0..row0+nrows-1 when tpts is null.  Otherwise they are the test points
+// tpts[row0..row0+nrows-1] = (x, y, z, test face), each excluding every face that shares a vertex with its
+// test face (the Galerkin non-singular set), and row r of the launch goes to rows 3r..3r+2 of UG/UH.
 hipError_t launch_colloc_gh_pair(const Geom &g, const double4 *pts, int npts_even, const Freq *freqs, int l0, int nl, const Mat &m,
-                                 int row0, int nrows, const int *patch_pass, const int2 *pass_info, const int4 *pass_faces,
-                                 const int *patch_verts, int npatches, cd *UG, cd *UH, long long ld, long long lstride,
-                                 hipStream_t st);
+                                 int row0, int nrows, const double4 *tpts, const int *patch_pass, const int2 *pass_info,
+                                 const int4 *pass_faces, const int *patch_verts, int npatches, cd *UG, cd *UH, long long ld,
+                                 long long lstride, hipStream_t st);
 int pair_patch_size();
 int pair_pass_faces();
 hipError_t launch_colloc_b(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
                            int row0, int nrows, cd *Bt, cd *Ba, hipStream_t st);
+// B_trans / B_angular rows of the test points tpts[row0..row0+nrows-1] (Galerkin non-singular set, as above), into
+// rows 3r..3r+2 of Bt (out_rows x 3 per frequency) and Ba (out_rows x 9).
+hipError_t launch_points_b(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
+                           const double4 *tpts, int row0, int nrows, cd *Bt, cd *Ba, long long out_rows, hipStream_t st);
 hipError_t launch_colloc_singular(int kind, const Geom &g, const double2 *line, int nline, const Freq *freqs, int l0, int nl,
                                   const Mat &m, int row0, int nrows, cd *U, long long ld, long long lstride, cd *Bt,
                                   cd *Ba, hipStream_t st);

@@ -1,6 +1,10 @@
 """GPU parity of the Galerkin assembly (non-singular pairs + Sauter-Schwab touching pairs) against the
 CPU oracle, per frequency and in the time domain.  Small quadrature orders keep the oracle fast; one
-case runs the shipped order 5 x subdivision 2 rule at a single frequency."""
+case runs the shipped order 5 x subdivision 2 rule at a single frequency.
+
+Two non-singular decompositions are checked against the same bounds: "points" (default: collocation
+rows at every test quadrature point, then projection onto the test basis) and "star" (one lane per
+test face and source vertex star; CQBEM_GALERKIN_KERNEL=star)."""
 import numpy as np
 import pytest
 
@@ -34,14 +38,18 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-@pytest.mark.parametrize("name,order,sub", [("ico0", 5, 2), ("ico1", 2, 1), ("cube2", 3, 1)])
-@pytest.mark.parametrize("l", [0, 7])
-def test_galerkin_laplace_parity(ctx, oracle, name, order, sub, l):
+LAPLACE_CASES = [(m, o, s, l, "points") for (m, o, s) in [("ico0", 5, 2), ("ico1", 2, 1), ("cube2", 3, 1)] for l in (0, 7)]
+LAPLACE_CASES += [("ico1", 2, 1, 7, "star"), ("cube2", 3, 1, 0, "star")]
+
+
+@pytest.mark.parametrize("name,order,sub,l,variant", LAPLACE_CASES)
+def test_galerkin_laplace_parity(ctx, oracle, name, order, sub, l, variant, monkeypatch):
+    monkeypatch.setenv("CQBEM_GALERKIN_KERNEL", variant)
     V, F = mesh(name)
     _, c1, c2 = M.material_speeds(1e6, 0.35, RHO)
     L, dt = 16, 5e-4
     R = 1e-10 ** (1 / (2 * L))
-    print(f"galerkin laplace {name} order {order} sub {sub} l={l}", flush=True)
+    print(f"galerkin laplace {name} order {order} sub {sub} l={l} {variant}", flush=True)
     g = ctx.laplace(abi.make_mesh(V, F), abi.make_params(c1, c2, RHO, 0.35, dt, R, L, order=order, subdivision=sub,
                                                         galerkin=True), l)
     s = oracle.frequencies(L, R, dt, 2)[l]
@@ -53,13 +61,14 @@ def test_galerkin_laplace_parity(ctx, oracle, name, order, sub, l):
     assert rel(g["Ba"], o["Ba"]) < 1e-13
 
 
-@pytest.mark.parametrize("name,L,bdf", [("ico1", 16, 2), ("cube2", 9, 1)])
-def test_galerkin_time_domain_parity(ctx, oracle, name, L, bdf):
+@pytest.mark.parametrize("name,L,bdf,variant", [("ico1", 16, 2, "points"), ("cube2", 9, 1, "points"), ("ico1", 16, 2, "star")])
+def test_galerkin_time_domain_parity(ctx, oracle, name, L, bdf, variant, monkeypatch):
+    monkeypatch.setenv("CQBEM_GALERKIN_KERNEL", variant)
     V, F = mesh(name)
     _, c1, c2 = M.material_speeds(1e6, 0.35, RHO)
     dt = 5e-4
     R = 1e-10 ** (1 / (2 * L))
-    print(f"galerkin time domain {name} L={L}", flush=True)
+    print(f"galerkin time domain {name} L={L} {variant}", flush=True)
     res = ctx.assemble(abi.make_mesh(V, F), abi.make_params(c1, c2, RHO, 0.35, dt, R, L, bdf=bdf, order=2,
                                                            subdivision=1, galerkin=True))
     op = lambda mode: oracle.params(c1, c2, RHO, 0.35, dt, R, L, bdf=bdf, order=2, subdiv=1, galerkin=True, mode=mode,
@@ -90,3 +99,36 @@ def test_galerkin_single_layer_symmetry(ctx):
     for n in range(L):
         Gn = res["G"][n]
         assert np.abs(Gn - Gn.T).max() < 1e-6 * gmax, n
+
+
+def test_galerkin_chunks_are_bitwise_identical():
+    """A small HBM budget cuts the test faces into chunks of 3 faces for the per-test-point rows; the faces
+    of one colour class share no vertex, so every output row still receives its face contributions in
+    colour-class order and the result is bit-identical to the unchunked run."""
+    import torch
+    V, F = mesh("ico1")
+    _, c1, c2 = M.material_speeds(1e6, 0.35, RHO)
+    L = 16
+    params = abi.make_params(c1, c2, RHO, 0.35, 5e-4, 1e-10 ** (1 / (2 * L)), L, order=2, subdivision=1, galerkin=True)
+    m = abi.make_mesh(V, F)
+    nv, nf, npts, nl = len(V), len(F), 3, L // 2 + 1
+    per_face = npts * 3 * nl * (2 * 3 * nv + 12) * 16  # G, H and B rows of one test face's points
+    full_fp = abi.job_footprint(m, params, outputs_on_device=True)  # all faces in one chunk
+    n3 = 3 * nv
+    dev = torch.device("cuda:0")
+    shapes = {"G": (L, n3, n3), "H": (L, n3, n3), "B": (L, n3, 6)}
+    full = abi.Context(0)
+    small = abi.Context(0, hbm_budget_bytes=full_fp - (nf - 1 - 3) * per_face)
+    print("galerkin chunks ico1", flush=True)
+    try:
+        res = []
+        for ctx in (full, small):
+            outs = {k: torch.full(s, float("nan"), dtype=torch.float64, device=dev) for k, s in shapes.items()}
+            ctx.assemble_device(m, params, {k: t.data_ptr() for k, t in outs.items()})
+            res.append({k: t.cpu().numpy() for k, t in outs.items()})
+        for k in shapes:
+            assert np.isfinite(res[1][k]).all(), k
+            assert np.array_equal(res[0][k], res[1][k]), k
+    finally:
+        full.close()
+        small.close()
