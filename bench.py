@@ -30,18 +30,38 @@ PEAK_FP64_TFLOPS = 78.6
 FLOPS_PER_EVAL_GH = 168 + 348
 
 
-def workload(precision: str):
+WORKLOADS = ("ico4-collocation", "cube8-collocation", "ico3-galerkin", "ico4-galerkin")
+
+
+def workload(name: str, precision: str):
+    """The single-object workloads: ico4-collocation (the default line), the cube k = 8 collocation case, the
+    icosphere-L3 Galerkin case (r = 0.1 m, E = 1e6 Pa, nu = 0.35, rho = 1100, dt = 5e-4 s, L = 64) and
+    icosphere-L4 in Galerkin form."""
     from surfaceonlydeformables_amd import abi
     from surfaceonlydeformables_amd import mesh as M
-    V, F = M.icosphere(4, 0.2)
-    V = M.center(M.jitter(V, F, 0.01, 0), F)
-    E, nu, rho, dt, L = 5e5, 0.3, 1000.0, 1e-3, 32
+    prec = abi.FP64 if precision == "fp64" else abi.MIXED
+    if name == "cube8-collocation":
+        V, F = M.cube(8, 0.5)
+        V = M.center(V, F)
+        E, nu, rho, dt, L, galerkin = 1e5, 0.3, 1000.0, 1e-3, 32, False
+        shape = "cube k=8 (386 V) side 1 m, E=1e5 Pa nu=0.3 rho=1000, dt=1e-3 s BDF2 L=32, collocation"
+    elif name == "ico3-galerkin":
+        V, F = M.icosphere(3, 0.1)
+        V = M.center(M.jitter(V, F, 0.01, 0), F)
+        E, nu, rho, dt, L, galerkin = 1e6, 0.35, 1100.0, 5e-4, 64, True
+        shape = "icosphere-L3 (642 V) r=0.1 m 1% jitter, E=1e6 Pa nu=0.35 rho=1100, dt=5e-4 s BDF2 L=64, Galerkin"
+    else:
+        V, F = M.icosphere(4, 0.2)
+        V = M.center(M.jitter(V, F, 0.01, 0), F)
+        E, nu, rho, dt, L = 5e5, 0.3, 1000.0, 1e-3, 32
+        galerkin = name == "ico4-galerkin"
+        shape = ("icosphere-L4 (2562 V) r=0.2 m 1% jitter, E=5e5 Pa nu=0.3 rho=1000, dt=1e-3 s BDF2 L=32, "
+                 + ("Galerkin" if galerkin else "collocation"))
     _, c1, c2 = M.material_speeds(E, nu, rho)
     R = 1e-10 ** (1.0 / (2 * L))
-    prec = abi.FP64 if precision == "fp64" else abi.MIXED
-    params = abi.make_params(c1, c2, rho, nu, dt, R, L, bdf=abi.BDF2, order=5, subdivision=2, precision=prec)
-    desc = ("icosphere-L4 r=0.2 m 1% jitter, E=5e5 Pa nu=0.3 rho=1000, dt=1e-3 s BDF2 L=32 R=1e-10^(1/64), "
-            "collocation order 5 x subdivision 2, G+H+B_trans+B_angular")
+    params = abi.make_params(c1, c2, rho, nu, dt, R, L, bdf=abi.BDF2, order=5, subdivision=2, precision=prec,
+                             galerkin=galerkin)
+    desc = f"{shape}, R=1e-10^(1/(2L)), order 5 x subdivision 2, G+H+B_trans+B_angular"
     return V, F, params, desc
 
 
@@ -198,6 +218,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--workload", choices=WORKLOADS, default=WORKLOADS[0],
+                    help="single-object workload (default: the icosphere-L4 collocation line)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one untimed assembly followed by a 1 GiB device copy, for HBM counter runs "
@@ -219,7 +241,7 @@ def main():
     import torch
     from surfaceonlydeformables_amd import abi
 
-    V, F, params, desc = workload(args.precision)
+    V, F, params, desc = workload(args.workload, args.precision)
     nv, L = len(V), params.L
     n3 = 3 * nv
     mesh = abi.make_mesh(V, F)
@@ -262,17 +284,49 @@ def main():
     value = entries * args.steps / wall
     last = ctx.last_timings()
 
+    roof = None
+    if not params.galerkin:
+        roof = roofline(gh_ms, gh_evals, gh_launches, with_traffic=args.workload == WORKLOADS[0])
+
+    # time stepping on the resident matrices: the history product of one step with a full history
+    history = history_step(ctx, ptrs, n3, L, torch, dev)
+
+    phase_keys = ("regular_gh", "regular_b", "singular", "transform", "host")
+    if params.galerkin:  # non-singular G/H/B with the test projection in one phase
+        phases = {"nonsingular": last["regular_gh"], "singular_pairs": last["singular"], "transform": last["transform"],
+                  "host": last["host"]}
+    else:
+        phases = {k: last[k] for k in phase_keys}
+    line = {"metric": METRIC, "value": value, "unit": "entries/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f64+f32",
+            "data": "synthetic",
+            "config": {"workload": desc, "nv": nv, "nf": len(F), "L": L, "precision": args.precision,
+                       "entries_per_step": entries},
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+            "phases_ms_last_step": phases, "roofline": roof, "time_step_history": history}
+    if not args.no_cpu_baseline and not params.galerkin:
+        line["cpu_baseline"] = cpu_baseline(V, F, params)
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+def roofline(gh_ms, gh_evals, gh_launches, with_traffic):
+    """VALU roofline of the dominant kernel from the library's HIP-event timers; the HBM traffic per launch is
+    the committed counter measurement of the icosphere-L4 shape."""
     achieved = FLOPS_PER_EVAL_GH * gh_evals / (gh_ms * 1e-3) / 1e12
     roof = {"bound": "valu", "kernel": "k_colloc_gh_pair<true,true>", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
             "launch_ms": gh_ms / max(gh_launches, 1),
             "flops_per_launch": FLOPS_PER_EVAL_GH * gh_evals / max(gh_launches, 1)}
     prof = os.path.join(ROOT, "profiles", "k_colloc_gh_pair_icosphere_l4_traffic.json")
-    if os.path.exists(prof):
+    if with_traffic and os.path.exists(prof):
         with open(prof) as fh:
             roof["traffic"] = json.load(fh).get("hbm_bytes_per_launch")
+    return roof
 
-    # time stepping on the resident matrices: the history product of one step with a full history
+
+def history_step(ctx, ptrs, n3, L, torch, dev):
     nh = L - 1
     hp = torch.randn((nh, n3), dtype=torch.float64, device=dev)
     hu = torch.randn((nh, n3), dtype=torch.float64, device=dev)
@@ -286,24 +340,9 @@ def main():
         ctx.history(*hist_args)
     hist_ms = (time.perf_counter() - t0) * 1e3 / reps
     hist_bytes = 2.0 * nh * n3 * n3 * 8 + nh * n3 * 6 * 8
-    history = {"kernel": "k_history", "ms": hist_ms, "matrices_read_GB": hist_bytes / 1e9,
-               "achieved_TBps": hist_bytes / (hist_ms * 1e-3) / 1e12, "peak_TBps": 8.0,
-               "note": "b = sum_i G_i p + H_i u + B_i acc over the 31 resident history weights (one CQ time step)"}
-
-    line = {"metric": METRIC, "value": value, "unit": "entries/s", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f64+f32",
-            "data": "synthetic",
-            "config": {"workload": desc, "nv": nv, "nf": len(F), "L": L, "precision": args.precision,
-                       "entries_per_step": entries},
-            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-            "phases_ms_last_step": {k: last[k] for k in ("regular_gh", "regular_b", "singular", "transform", "host")},
-            "roofline": roof, "time_step_history": history}
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(V, F, params)
-    ctx.close()
-    print(json.dumps(line), flush=True)
-
+    return {"kernel": "k_history", "ms": hist_ms, "matrices_read_GB": hist_bytes / 1e9,
+            "achieved_TBps": hist_bytes / (hist_ms * 1e-3) / 1e12, "peak_TBps": 8.0,
+            "note": f"b = sum_i G_i p + H_i u + B_i acc over the {nh} resident history weights (one CQ time step)"}
 
 if __name__ == "__main__":
     main()

@@ -95,14 +95,26 @@ int cqbem_job_footprint(const cqbem_mesh *mesh, const cqbem_params *params, int 
 int cqbem_laplace(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params *params, int32_t l, double *U,
                   double *P, double *Bt, double *Ba);
 
+/* Traction-discontinuity vertex split, the mesh step of PhysicsBaseMatrixCollection::load_mesh with
+ * enable_traction_discontinuity (src/physics_base_matrix_collection.cpp:203-365, vertex reordering off):
+ * vertices on sharp edges (face normals more than threshold_deg apart; the reference's default is 20)
+ * get one copy per smooth sheet, placed right after the original.  In: V (nv x 3), F (nf x 3) of a
+ * closed, consistently oriented manifold.  Out: V_out (nv_out x 3), F_out (nf x 3), vertex_map_inverse
+ * (nv_out; output vertex -> input vertex), ready for cqbem_mesh.  With V_out, F_out and
+ * vertex_map_inverse all NULL only *nv_out is set; otherwise *nv_out must hold the capacity on entry. */
+int cqbem_split_traction_discontinuity(const double *V, int64_t nv, const int32_t *F, int64_t nf, double threshold_deg,
+                                       double *V_out, int32_t *F_out, int32_t *vertex_map_inverse, int64_t *nv_out);
+
 /* CQ frequencies s_l = gamma(R zeta^l)/dt, l = 0..L/2, complex interleaved (G.cu:19-24,65). Returns L/2+1. */
 int cqbem_frequencies(int32_t L, double R, double dt, int32_t bdf, double *s_out);
 
 /* Statistics of the last cqbem_assemble on this context (device times from HIP events on the context's
- * stream, milliseconds): [0] total wall, [1] regular G/H quadrature (the dominant kernel), [2] regular
- * B_trans/B_angular quadrature, [3] singular-pair kernels, [4] inverse transform and post-processing,
- * [5] host preparation and staging, [6] launches of the dominant kernel, [7] far-field kernel evaluations
- * (collocation point or test point, source point, frequency) those launches performed.
+ * stream, milliseconds): [0] total wall, [1] regular G/H quadrature (the dominant kernel; Galerkin: all
+ * non-singular quadrature, G, H and B, with the test-basis projection), [2] regular B_trans/B_angular
+ * quadrature (collocation), [3] singular kernels (collocation: faces at the point; Galerkin: touching
+ * pairs and their gather), [4] inverse transform and post-processing, [5] host preparation and staging,
+ * [6] launches of the dominant collocation kernel, [7] far-field kernel evaluations (collocation point,
+ * source point, frequency) those launches performed.
  * Copies min(n, 8) values; returns that count. */
 int cqbem_last_timings(const cqbem_context *ctx, double *out, int32_t n);
 

@@ -988,6 +988,23 @@ void C_collocation(const double *V, int nv, const int *F, int nf, double nu, dou
     }
 }
 
+// Vertex-split mesh: the jump term is evaluated on the original (unsplit) mesh and copied to every copy
+// of a vertex (elastics_bem_matrix_collection.cpp:392-398,436-442).  The original mesh is recovered from
+// the map: original vertex vmap[v] sits at V[v] and face f reads vmap[F[f]].
+void C_collocation_split(const double *V, int nv, const int *F, int nf, const int *vmap, double nu, double *C) {
+    int no = 0;
+    for (int v = 0; v < nv; v++) no = std::max(no, vmap[v] + 1);
+    std::vector<double> Vo(3 * (size_t)no, 0.0);
+    for (int v = 0; v < nv; v++)
+        for (int k = 0; k < 3; k++) Vo[3 * (size_t)vmap[v] + k] = V[3 * v + k];
+    std::vector<int> Fo(3 * (size_t)nf);
+    for (size_t k = 0; k < Fo.size(); k++) Fo[k] = vmap[F[k]];
+    std::vector<double> Co(9 * (size_t)no);
+    C_collocation(Vo.data(), no, Fo.data(), nf, nu, Co.data());
+    for (int v = 0; v < nv; v++)
+        for (int e = 0; e < 9; e++) C[9 * (size_t)v + e] = Co[9 * (size_t)vmap[v] + e];
+}
+
 // Galerkin jump term: 1/2 of the P1 mass matrix per displacement component
 // (compute_C_matrix_galerkin, elastics_bem_matrix_collection.cpp:447-467).
 void C_galerkin_add(const double *V, int nv, const int *F, int nf, double *H0 /* dense 3nv x 3nv */) {
@@ -1160,7 +1177,10 @@ int assemble_impl(const Mesh &m, const orc_params &p, int mask, double *G, doubl
             for (size_t e = 0; e < n3 * n3; e++) WH[e] += (A)H0[e];
         } else {
             std::vector<double> C(n3 * 3);
-            C_collocation(m.V, m.nv, m.F, m.nf, p.nu, C.data());
+            if (m.vmap)
+                C_collocation_split(m.V, m.nv, m.F, m.nf, m.vmap, p.nu, C.data());
+            else
+                C_collocation(m.V, m.nv, m.F, m.nf, p.nu, C.data());
             for (int v = 0; v < m.nv; v++)
                 for (int a = 0; a < 3; a++)
                     for (int b = 0; b < 3; b++) WH[(size_t)(3 * v + a) * n3 + 3 * v + b] += (A)C[(3 * v + a) * 3 + b];
@@ -1301,6 +1321,11 @@ int orc_C_collocation(const double *V, int nv, const int *F, int nf, double nu, 
     return 0;
 }
 
+int orc_C_collocation_split(const double *V, int nv, const int *F, int nf, const int *vmap, double nu, double *C) {
+    C_collocation_split(V, nv, F, nf, vmap, nu, C);
+    return 0;
+}
+
 int orc_C_galerkin(const double *V, int nv, const int *F, int nf, double *H0) {
     C_galerkin_add(V, nv, F, nf, H0);
     return 0;
@@ -1381,4 +1406,5 @@ int orc_assemble(const double *V, int nv, const int *F, int nf, const int *vmap,
 //   galerkin_* rules              static-limit Galerkin identities and symmetry on tiny meshes
 //   cq_frequencies, transform     known CQ weights of exp(-s T), 1/s, 1/s^2 (BDF1, BDF2)
 //   C_collocation                 sum of jump terms; 1/2 I on flat interior vertices
+//   C_collocation_split           rigid-body identity on the vertex-split cube (test_oracle_identities)
 //   C_galerkin_add                total mass = surface area per component

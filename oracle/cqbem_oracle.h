@@ -26,6 +26,7 @@ double orc_polar_inv_r(const double *y1, const double *y2, const double *y3, int
 double orc_far_poly(const double *y1, const double *y2, const double *y3, const double *a, const double *b, int order,
                     int sub);
 int orc_C_collocation(const double *V, int nv, const int *F, int nf, double nu, double *C);
+int orc_C_collocation_split(const double *V, int nv, const int *F, int nf, const int *vmap, double nu, double *C);
 int orc_C_galerkin(const double *V, int nv, const int *F, int nf, double *H0);
 int orc_transform(const double *spec, long long M, int L, double R, double *W);
 int orc_laplace(const double *V, int nv, const int *F, int nf, const int *vmap, const orc_params *p, double s_re,

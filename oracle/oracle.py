@@ -108,10 +108,15 @@ def far_poly(y1, y2, y3, a, b, order, sub):
     return lib().orc_far_poly(*[_dp(t) for t in v], C.c_int(order), C.c_int(sub))
 
 
-def C_collocation(V, F, nu):
+def C_collocation(V, F, nu, vmap=None):
     V = np.ascontiguousarray(V, np.float64); F = np.ascontiguousarray(F, np.int32)
     out = np.zeros((3 * len(V), 3))
-    lib().orc_C_collocation(_dp(V), C.c_int(len(V)), _ip(F), C.c_int(len(F)), C.c_double(nu), _dp(out))
+    if vmap is None:
+        lib().orc_C_collocation(_dp(V), C.c_int(len(V)), _ip(F), C.c_int(len(F)), C.c_double(nu), _dp(out))
+    else:
+        vm = np.ascontiguousarray(vmap, np.int32)
+        lib().orc_C_collocation_split(_dp(V), C.c_int(len(V)), _ip(F), C.c_int(len(F)), _ip(vm), C.c_double(nu),
+                                      _dp(out))
     return out
 
 

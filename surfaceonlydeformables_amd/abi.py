@@ -22,7 +22,7 @@ G, H, B_TRANS, B_ANGULAR, ALL = 1, 2, 4, 8, 15
 EXPORTED = [
     "cqbem_create", "cqbem_destroy", "cqbem_last_error", "cqbem_assemble", "cqbem_job_footprint",
     "cqbem_laplace", "cqbem_frequencies", "cqbem_last_timings", "cqbem_submit", "cqbem_poll",
-    "cqbem_wait", "cqbem_reserved_bytes", "cqbem_history",
+    "cqbem_wait", "cqbem_reserved_bytes", "cqbem_history", "cqbem_split_traction_discontinuity",
 ]
 
 
@@ -81,6 +81,8 @@ def lib():
         L.cqbem_wait.argtypes = [C.c_void_p, C.c_uint64]
         L.cqbem_reserved_bytes.argtypes = [C.c_void_p, P(C.c_uint64)]
         L.cqbem_history.argtypes = [C.c_void_p, C.c_int64, C.c_int32] + [C.c_void_p] * 6 + [C.c_int32, C.c_void_p]
+        L.cqbem_split_traction_discontinuity.argtypes = [P(C.c_double), C.c_int64, P(C.c_int32), C.c_int64, C.c_double,
+                                                         P(C.c_double), P(C.c_int32), P(C.c_int32), P(C.c_int64)]
         _lib = L
     return _lib
 
@@ -102,6 +104,25 @@ def make_mesh(V, F, vmap=None):
 def make_params(c1, c2, rho, nu, dt, R, L, bdf=BDF2, order=5, subdivision=2, galerkin=False, precision=FP64,
                 which=ALL):
     return cqbem_params(c1, c2, rho, nu, dt, R, L, bdf, order, subdivision, int(galerkin), precision, which)
+
+
+def split_traction_discontinuity(V, F, threshold_deg=20.0):
+    """Vertex split along sharp edges (the reference's enable_traction_discontinuity mesh step):
+    returns (V_split, F_split, vertex_map_inverse), ready for make_mesh(V_split, F_split, vmap)."""
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    F = np.ascontiguousarray(F, dtype=np.int32)
+    pd, pi = C.POINTER(C.c_double), C.POINTER(C.c_int32)
+    n = C.c_int64(0)
+    args = (V.ctypes.data_as(pd), len(V), F.ctypes.data_as(pi), len(F), threshold_deg)
+    rc = lib().cqbem_split_traction_discontinuity(*args, None, None, None, C.byref(n))
+    if rc != CQBEM_OK:
+        raise CQBEMError(rc, "vertex split failed (the mesh must be a closed, consistently oriented manifold)")
+    Vs, Fs, vm = np.zeros((n.value, 3)), np.zeros((len(F), 3), np.int32), np.zeros(n.value, np.int32)
+    rc = lib().cqbem_split_traction_discontinuity(*args, Vs.ctypes.data_as(pd), Fs.ctypes.data_as(pi),
+                                                  vm.ctypes.data_as(pi), C.byref(n))
+    if rc != CQBEM_OK:
+        raise CQBEMError(rc, "vertex split failed")
+    return Vs, Fs, vm
 
 
 def frequencies(L, R, dt, bdf):

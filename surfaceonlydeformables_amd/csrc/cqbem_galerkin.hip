@@ -809,7 +809,7 @@ void run_galerkin(const GalerkinJob &job, const std::vector<int> &vs_ptr, const 
     check(hipMalloc(&dscratch.p, (size_t)npairs * nl * kPairStride * sizeof(cd)), "hipMalloc scratch");
     cd *scratch = (cd *)dscratch.p;
     {
-        EvTimer t(st, &tm[2]);
+        EvTimer t(st, &tm[3]);
         const int beg[3] = {0, kind_off[1], kind_off[2]}, end[3] = {kind_off[1], kind_off[2], kind_off[3]};
 #define CQBEM_SING(MODE, KIND)                                                                                        \
     if (end[KIND - 1] > beg[KIND - 1])                                                                                \
@@ -841,7 +841,7 @@ void run_galerkin(const GalerkinJob &job, const std::vector<int> &vs_ptr, const 
     const int2 *db = dblocks.up(blocks, st);
     const int *dpt = dptr.up(ptr, st), *dit = ditems.up(items, st), *dvp = dvptr.up(vptr, st), *dvi = dvitems.up(vitems, st);
     {
-        EvTimer t(st, &tm[2]);
+        EvTimer t(st, &tm[3]);
         if (U || P) {
             const long long n = (long long)blocks.size() * nl;
             hipLaunchKernelGGL(k_gal_gather_gh, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, db, dpt, dit, (int)blocks.size(), nl,

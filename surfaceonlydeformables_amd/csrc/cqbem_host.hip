@@ -74,9 +74,17 @@ void check_inputs(const cqbem_mesh *mesh, const cqbem_params *p) {
     if (mesh->nv > (1 << 24) || mesh->nf > (1 << 26)) throw Err{CQBEM_ERR_ARG, "mesh too large"};
     for (int64_t k = 0; k < 3 * mesh->nf; k++)
         if (mesh->F[k] < 0 || mesh->F[k] >= mesh->nv) throw Err{CQBEM_ERR_ARG, "face index out of range"};
-    if (mesh->vertex_map_inverse)
-        for (int64_t k = 0; k < mesh->nv; k++)
-            if (mesh->vertex_map_inverse[k] < 0) throw Err{CQBEM_ERR_ARG, "negative vertex_map_inverse entry"};
+    if (const int *vm = mesh->vertex_map_inverse) {
+        // copies of one original vertex must sit at one position (the split only duplicates vertices)
+        std::unordered_map<int, int64_t> first;
+        for (int64_t k = 0; k < mesh->nv; k++) {
+            if (vm[k] < 0 || vm[k] >= mesh->nv) throw Err{CQBEM_ERR_ARG, "vertex_map_inverse entry out of range"};
+            auto it = first.emplace(vm[k], k).first;
+            for (int d = 0; d < 3; d++)
+                if (mesh->V[3 * k + d] != mesh->V[3 * it->second + d])
+                    throw Err{CQBEM_ERR_ARG, "vertices mapped to one original vertex differ in position"};
+        }
+    }
     if (!(p->c1 > 0) || !(p->c2 > 0) || !(p->rho > 0) || !(p->dt > 0) || !(p->R > 0))
         throw Err{CQBEM_ERR_ARG, "c1, c2, rho, dt and R must be positive"};
     if (p->L < 2 || p->L > 4096) throw Err{CQBEM_ERR_ARG, "L must be in [2, 4096]"};
@@ -85,8 +93,6 @@ void check_inputs(const cqbem_mesh *mesh, const cqbem_params *p) {
     if (p->subdivision < 1 || p->subdivision > 16) throw Err{CQBEM_ERR_ARG, "subdivision must be in [1, 16]"};
     if (p->precision != CQBEM_FP64 && p->precision != CQBEM_MIXED) throw Err{CQBEM_ERR_ARG, "unknown precision"};
     if ((p->which & ~CQBEM_ALL) != 0) throw Err{CQBEM_ERR_ARG, "unknown matrix bits in `which`"};
-    if (!p->galerkin && (p->which & CQBEM_H) && mesh->vertex_map_inverse)
-        throw Err{CQBEM_ERR_UNSUPPORTED, "collocation C matrix of a vertex-split mesh needs the original mesh"};
     if (p->galerkin && mesh->vertex_map_inverse)
         throw Err{CQBEM_ERR_UNSUPPORTED, "Galerkin assembly of a vertex-split mesh is not supported"};
     // the device sincos reduces |Im(s) r / c| <= max|s_l| diam / min(c1, c2) exactly below kTrigArgMax
@@ -555,7 +561,9 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
             }
         }
         if (wantH) {
-            Chost = c_matrix_collocation(mesh->V, nv, mesh->F, P.nf, P.N.data(), p->nu);
+            Chost = mesh->vertex_map_inverse
+                        ? c_matrix_collocation_split(mesh->V, nv, mesh->F, P.nf, P.N.data(), mesh->vertex_map_inverse, p->nu)
+                        : c_matrix_collocation(mesh->V, nv, mesh->F, P.nf, P.N.data(), p->nu);
             upload(dC, Chost, st);
             HIPCHK(launch_add_c_colloc(H, dC.as<double>(), nv, st));
         }

@@ -1,6 +1,7 @@
 // Quadrature tables, CQ frequency set, transform table and jump (C) matrices,
 // built on the host in FP64 and uploaded once per job.
 #pragma once
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <complex>
@@ -204,6 +205,25 @@ inline std::vector<double> c_matrix_collocation(const double *V, int nv, const i
                 for (int b = 0; b < 3; b++) C[9 * F[3 * f + i] + 3 * a + b] -= kf * tt[a] * nn[b];
         }
     }
+    return C;
+}
+
+// Jump term of a vertex-split mesh: evaluated on the original (unsplit) mesh and copied to every copy of a
+// vertex (elastics_bem_matrix_collection.cpp:392-398,436-442).  The original mesh is recovered from the
+// map: original vertex vmap[v] sits at V[v] (copies share the position) and face f reads vmap[F[f]].
+inline std::vector<double> c_matrix_collocation_split(const double *V, int nv, const int *F, int nf, const double *N,
+                                                      const int *vmap, double nu) {
+    int no = 0;
+    for (int v = 0; v < nv; v++) no = std::max(no, vmap[v] + 1);
+    std::vector<double> Vo(3 * (size_t)no, 0.0);
+    for (int v = 0; v < nv; v++)
+        for (int k = 0; k < 3; k++) Vo[3 * (size_t)vmap[v] + k] = V[3 * v + k];
+    std::vector<int> Fo(3 * (size_t)nf);
+    for (size_t k = 0; k < Fo.size(); k++) Fo[k] = vmap[F[k]];
+    const std::vector<double> Co = c_matrix_collocation(Vo.data(), no, Fo.data(), nf, N, nu);
+    std::vector<double> C(9 * (size_t)nv);
+    for (int v = 0; v < nv; v++)
+        for (int e = 0; e < 9; e++) C[9 * (size_t)v + e] = Co[9 * (size_t)vmap[v] + e];
     return C;
 }
 

@@ -211,9 +211,18 @@ bool ElastodynamicsCQBEMMatrixCollection::init(const Json &config) {
         if (!load_obj(m["file"].string(), v, f, error_)) return false;
         if (m.contains("scale"))
             for (double &x : v) x *= m["scale"].number();
-        if (m.contains("enable_traction_discontinuity") && m["enable_traction_discontinuity"].boolean())
-            return fail("traction-discontinuity vertex splitting is not part of this build");
-        return init(config, v, f);
+        if (m.contains("enable_traction_discontinuity") && m["enable_traction_discontinuity"].boolean()) {
+            // physics_base_matrix_collection.cpp:68-71,203-365 (vertex reordering off)
+            const double deg = m.contains("sharp_edge_threshold_degree") ? m["sharp_edge_threshold_degree"].number() : 20.0;
+            std::vector<double> vs;
+            std::vector<int32_t> fs, vm;
+            std::string err;
+            if (split_traction_discontinuity(v.data(), (int)(v.size() / 3), f.data(), (int)(f.size() / 3), deg, vs, fs, vm,
+                                             err) != CQBEM_OK)
+                return fail("traction-discontinuity split: " + err);
+            return init_mesh(config, std::move(vs), std::move(fs), std::move(vm));
+        }
+        return init_mesh(config, std::move(v), std::move(f), {});
     } catch (const std::exception &e) {
         return fail(e.what());
     }
@@ -221,8 +230,13 @@ bool ElastodynamicsCQBEMMatrixCollection::init(const Json &config) {
 
 bool ElastodynamicsCQBEMMatrixCollection::init(const Json &config, const std::vector<double> &v,
                                                const std::vector<int32_t> &f) {
+    return init_mesh(config, v, f, {});
+}
+
+bool ElastodynamicsCQBEMMatrixCollection::init_mesh(const Json &config, std::vector<double> v, std::vector<int32_t> f,
+                                                    std::vector<int32_t> vmap) {
     try {
-        V = v, F = f;
+        V = std::move(v), F = std::move(f), vertex_map_inverse = std::move(vmap);
         num_vertices = (int64_t)(V.size() / 3);
         return run(config);
     } catch (const std::exception &e) {
@@ -287,7 +301,8 @@ bool ElastodynamicsCQBEMMatrixCollection::run(const Json &config) {
     if (!ctx) return fail(err);
     const int64_t n3 = 3 * num_vertices;
     std::vector<double> g((size_t)L * n3 * n3), h((size_t)L * n3 * n3), b((size_t)L * n3 * 6);
-    cqbem_mesh mesh{V.data(), num_vertices, F.data(), (int64_t)(F.size() / 3), nullptr};
+    cqbem_mesh mesh{V.data(), num_vertices, F.data(), (int64_t)(F.size() / 3),
+                    vertex_map_inverse.empty() ? nullptr : vertex_map_inverse.data()};
     cqbem_params prm{c1, c2, rho, nu, dt_, R, (int32_t)L, multistep_method == BDF1 ? CQBEM_BDF1 : CQBEM_BDF2,
                      order, sub, use_galerkin ? 1 : 0, CQBEM_FP64, CQBEM_ALL};
     cqbem_outputs out{g.data(), h.data(), nullptr, nullptr, b.data(), 0};

@@ -48,9 +48,10 @@ class ElastodynamicsCQBEMMatrixCollection {
     ElastodynamicsCQBEMMatrixCollection(double dt, bool enable_cuda, unsigned int cuda_thread_per_block);
     ~ElastodynamicsCQBEMMatrixCollection();
 
-    // Reads mesh (OBJ file, scale), material (rho, mu, nu) and simulation (dt, gaussian_quadrature_order,
-    // quadrature_subdivision, multistep_method, use_galerkin) like the reference; optional
-    // simulation.cq_length / simulation.cq_radius override the mesh-derived L and R.
+    // Reads mesh (OBJ file, scale, enable_traction_discontinuity with sharp_edge_threshold_degree),
+    // material (rho, mu, nu) and simulation (dt, gaussian_quadrature_order, quadrature_subdivision,
+    // multistep_method, use_galerkin) like the reference; optional simulation.cq_length /
+    // simulation.cq_radius override the mesh-derived L and R.
     bool init(const Json &config);
     // Same, with an in-memory mesh instead of config["mesh"].
     bool init(const Json &config, const std::vector<double> &V, const std::vector<int32_t> &F);
@@ -64,6 +65,7 @@ class ElastodynamicsCQBEMMatrixCollection {
     int64_t num_vertices = 0;
     std::vector<double> V;  // centred vertices, nv x 3
     std::vector<int32_t> F;
+    std::vector<int32_t> vertex_map_inverse;  // split vertex -> input vertex (empty without the split)
     double cm[3] = {0, 0, 0};
     double mass = 0.0;
     double I[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // inertia tensor about the centroid, row-major
@@ -85,6 +87,7 @@ class ElastodynamicsCQBEMMatrixCollection {
     std::vector<double> rotation_removal_weights;  // nv
 
   private:
+    bool init_mesh(const Json &config, std::vector<double> v, std::vector<int32_t> f, std::vector<int32_t> vmap);
     bool fail(const std::string &msg);
     bool run(const Json &config);
     bool post_process();
@@ -94,6 +97,11 @@ class ElastodynamicsCQBEMMatrixCollection {
     unsigned int cuda_thread_per_block_;
     std::string error_;
 };
+
+// Traction-discontinuity vertex split (mesh_split.cpp); returns a cqbem status.
+int split_traction_discontinuity(const double *V, int nv, const int32_t *F, int nf, double threshold_deg,
+                                 std::vector<double> &V_out, std::vector<int32_t> &F_out, std::vector<int32_t> &vmap_out,
+                                 std::string &err);
 
 // Loads an OBJ (v / f lines, polygons fanned into triangles).
 bool load_obj(const std::string &path, std::vector<double> &V, std::vector<int32_t> &F, std::string &err);

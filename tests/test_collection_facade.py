@@ -161,3 +161,46 @@ def test_facade_time_stepping_matrices(tmp_path):
     assert rel(post("B_trans_inv_G_prime", (3, n3)), lsq(Bt, Gp)) < 1e-10
     assert rel(post("B_euler_inv_G_prime", (3, n3)), lsq(Be, Gp)) < 1e-10
     assert rel(post("B_euler_inv_B_trans", (3, 3)), lsq(Be, Bt)) < 1e-10
+
+
+@pytest.mark.gpu
+def test_facade_reference_cube_scene(tmp_path):
+    """The object group of the reference's data/cube_drop.json (cube_reordered.obj with
+    enable_traction_discontinuity, its material and simulation keys) through the facade: it splits the mesh
+    along the cube's sharp edges (870 vertices, the reference's own split output) and assembles with the
+    vertex map.  The matrices equal a direct cqbem_assemble of the split, centred mesh with its map (to the
+    rounding of the two centroid computations, amplified by R^-n with R = 0.1 here)."""
+    import torch  # noqa: F401  (one HIP runtime: abi loads torch first)
+    from surfaceonlydeformables_amd import abi
+    obj = os.path.join(ROOT, "tests", "golden", "cube_reordered.obj")
+    cfg = {"enable_cuda": True,
+           "mesh": {"file": obj, "enable_vertex_reordering": False, "enable_traction_discontinuity": True},
+           "material": {"rho": 1000.0, "mu": 100000.0, "nu": 0.44999998807907104},
+           "simulation": {"dt": 0.004999999888241291, "gaussian_quadrature_order": 5, "quadrature_subdivision": 2,
+                          "multistep_method": "BDF1", "use_galerkin": False, "update_local_frame": True,
+                          "compression_ratio": 0.0, "coeffs_folder_path": "precomputed_data/cube"}}
+    path = tmp_path / "cube_drop_object.json"
+    path.write_text(json.dumps(cfg))
+    print("collection facade reference cube scene", flush=True)
+    r = run(path, tmp_path / "out")
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    V, F = M.read_obj(obj)
+    Vs, Fs, vm = abi.split_traction_discontinuity(V, F, 20.0)
+    assert info["nv"] == len(Vs) == 870
+    Vc = M.center(Vs, Fs)
+    L, n3 = info["L"], 3 * len(Vs)
+    p = abi.make_params(info["c1"], info["c2"], 1000.0, 0.44999998807907104, 0.004999999888241291, info["R"], L,
+                        bdf=abi.BDF1)
+    ctx = abi.Context(0)
+    try:
+        direct = ctx.assemble(abi.make_mesh(Vc, Fs, vm), p)
+    finally:
+        ctx.close()
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    G = np.fromfile(tmp_path / "out_G.bin").reshape(L, n3, n3)
+    H = np.fromfile(tmp_path / "out_H.bin").reshape(L, n3, n3)
+    B = np.fromfile(tmp_path / "out_B.bin").reshape(L, n3, 6)
+    assert rel(G, direct["G"]) < 1e-9
+    assert rel(-H, direct["H"]) < 1e-9
+    assert rel(B, direct["B"]) < 1e-9

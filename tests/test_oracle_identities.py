@@ -61,3 +61,25 @@ def test_flat_vertex_jump_is_half(oracle):
     assert len(flat) == 6 * 4
     for i in flat:
         assert np.abs(C[i] - 0.5 * np.eye(3)).max() < 1e-14
+
+
+def test_rigid_body_identity_split_cube(oracle):
+    """Vertex-split cube (every cube edge sharp, the reference's enable_traction_discontinuity): the identity
+    holds when the singular faces of a vertex include those of its copies (vertex_map_inverse) and C is
+    evaluated on the original mesh and copied (elastics_bem_matrix_collection.cpp:392-398,436-442)."""
+    from surfaceonlydeformables_amd import abi
+    V, F = M.cube(2, 0.2)
+    V = M.center(V, F)
+    Vs, Fs, vm = abi.split_traction_discontinuity(V, F, 20.0)
+    nv = len(Vs)
+    C = oracle.C_collocation(Vs, Fs, NU, vmap=vm)
+    p = oracle.params(C1, C2, RHO, NU, 1e-3, 0.7, 8, order=5, subdiv=2, mode=oracle.MODE_LD, nthreads=8)
+    P = oracle.laplace(Vs, Fs, p, 1e-2 + 0j, mask=oracle.MASK_H, vmap=vm)["P"]
+    err = P.reshape(nv, 3, nv, 3).sum(axis=2) + C.reshape(nv, 3, 3)
+    assert np.abs(err).max() < 1e-5
+    # the copies carry the jump term of their original vertex
+    C0 = oracle.C_collocation(V, F, NU).reshape(-1, 3, 3)
+    assert np.array_equal(C.reshape(nv, 3, 3), C0[vm])
+    # the split mesh's own C (its sharp edges are open boundaries there) breaks the identity
+    C_open = oracle.C_collocation(Vs, Fs, NU)
+    assert np.abs(P.reshape(nv, 3, nv, 3).sum(axis=2) + C_open.reshape(nv, 3, 3)).max() > 1e-2
