@@ -18,13 +18,17 @@
 //   PolLD        every scalar is long double (x87, 64-bit mantissa); used only
 //                to measure the rounding noise of the FP64 modes  ("ld")
 //
-// Two deliberate, documented differences from the reference's operation
-// sequence (same mathematics, less rounding noise; both are needed because the
-// R^-n factor of the inverse transform amplifies per-frequency rounding):
+// Three deliberate, documented differences from the reference's operation
+// sequence (same mathematics, less rounding noise; the first two are needed because
+// the R^-n factor of the inverse transform amplifies per-frequency rounding):
 //   * the transform twiddles zeta^(-n l) are evaluated as cis(-2 pi ((n l) mod L)/L)
 //     instead of std::pow(exp(2 pi i/L), -n l) (G.cu:172-174);
 //   * zeta^l in the CQ frequencies is evaluated as cis(2 pi l/L) instead of
-//     std::pow(exp(2 pi i/L), l) (G.cu:65).
+//     std::pow(exp(2 pi i/L), l) (G.cu:65);
+//   * the polar rule's angle beta between the foot-of-perpendicular direction and the
+//     first edge is atan2(|w x u1|, w . u1) instead of acos(clamp(w . u1))
+//     (integrator_collocation.hpp:36): near a right angle at the second vertex acos
+//     loses half the digits (1e-8 in beta, 1e-9 in the singular G rows of a cube).
 // Parity pinning status of every function is listed at the end of this file.
 
 #include <algorithm>
@@ -401,7 +405,7 @@ void polar_rule(const v3<T> &y1, const v3<T> &y2, const v3<T> &y3, const QParams
     const v3<T> v12 = v1 + dot(v1, unit(v1 - v2)) * unit(v2 - v1);
     const T a = norm(v12);
     const T alpha = std::acos(dot(u1, u2));
-    const T beta = std::acos(std::min(T(1), std::max(T(-1), dot(unit(v12), u1))));
+    const T beta = std::atan2(norm(cross(unit(v12), u1)), dot(unit(v12), u1));  // deliberate difference 3 (header)
     Line1D<T> th = composite_line<T>(T(0), alpha, q.order, q.sub);
     for (size_t it = 0; it < th.x.size(); it++) {
         const T t = th.x[it];

@@ -652,14 +652,30 @@ size_t galerkin_scratch_bytes(int nv, int nf, int npts, int nl, bool G, bool H, 
            (size_t)nl * rows * ((G ? 3ULL * nv : 0) + (H ? 3ULL * nv : 0) + (B ? 12 : 0)) * sizeof(cd);
 }
 
-void run_galerkin(const GalerkinJob &job, const std::vector<int> &vs_ptr, const std::vector<int> &vs_face, const int *F,
-                  cd *U, cd *P, cd *Bt, cd *Ba, double *tm, hipStream_t st) {
+std::vector<std::vector<int>> touching_candidates(const int *F, int nf, int nv, const int *vmap) {
+    // faces around each vertex, or around each original vertex when the mesh is split
+    std::vector<std::vector<int>> around(nv);
+    auto key = [&](int v) { return vmap ? vmap[v] : v; };
+    for (int f = 0; f < nf; f++)
+        for (int k = 0; k < 3; k++) around[key(F[3 * f + k])].push_back(f);
+    std::vector<std::vector<int>> cand(nf);
+    for (int f1 = 0; f1 < nf; f1++) {
+        auto &c = cand[f1];
+        for (int k = 0; k < 3; k++) {
+            const auto &a = around[key(F[3 * f1 + k])];
+            c.insert(c.end(), a.begin(), a.end());
+        }
+        std::sort(c.begin(), c.end());
+        c.erase(std::unique(c.begin(), c.end()), c.end());
+    }
+    return cand;
+}
+
+void run_galerkin(const GalerkinJob &job, const int *F, const int *vmap, cd *U, cd *P, cd *Bt, cd *Ba, double *tm,
+                  hipStream_t st) {
     const Geom &g = job.g;
     const int nv = g.nv, nf = g.nf, nl = job.nl;
     const long long n3 = 3LL * nv;
-    const int *vm = nullptr;  // host copy of the vertex map is not needed: touching tests run on device,
-                              // and the host pair list below uses exact vertex identity plus the map if given
-    (void)vm;
     zero(U, U ? (long long)nl * n3 * n3 : 0, job.num_cus, st);
     zero(P, P ? (long long)nl * n3 * n3 : 0, job.num_cus, st);
     zero(Bt, Bt ? (long long)nl * n3 * 3 : 0, job.num_cus, st);
@@ -760,15 +776,10 @@ void run_galerkin(const GalerkinJob &job, const std::vector<int> &vs_ptr, const 
     // Touching pairs, classified exactly as integrate_galerkin_singular (integrator_galerkin.hpp:262-334).
     std::vector<Pair> pairs;
     {
-        std::vector<std::vector<int>> vfaces(nv);
-        for (int v = 0; v < nv; v++)
-            for (int t = vs_ptr[v]; t < vs_ptr[v + 1]; t++) vfaces[v].push_back(vs_face[t] >> 2);
+        const std::vector<std::vector<int>> cands = touching_candidates(F, nf, nv, vmap);
+        auto same = [&](int a, int b) { return a == b || (vmap && vmap[a] == vmap[b]); };
         for (int f1 = 0; f1 < nf; f1++) {
-            std::vector<int> cand;
-            for (int k = 0; k < 3; k++) cand.insert(cand.end(), vfaces[F[3 * f1 + k]].begin(), vfaces[F[3 * f1 + k]].end());
-            std::sort(cand.begin(), cand.end());
-            cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
-            for (int f2 : cand) {
+            for (int f2 : cands[f1]) {
                 Pair p{};
                 p.f2 = f2;
                 const int *A = F + 3 * f1, *B = F + 3 * f2;
@@ -778,7 +789,8 @@ void run_galerkin(const GalerkinJob &job, const std::vector<int> &vs_ptr, const 
                 } else {
                     for (int i = 0; i < 3; i++)
                         for (int j = 0; j < 3; j++)
-                            if (A[i] == B[j] && A[(i + 1) % 3] == B[(j + 2) % 3]) {
+                            if ((A[i] == B[j] && A[(i + 1) % 3] == B[(j + 2) % 3]) ||
+                                (vmap && vmap[A[i]] == vmap[B[j]] && vmap[A[(i + 1) % 3]] == vmap[B[(j + 2) % 3]])) {
                                 p.kind = 2;
                                 p.j1[0] = A[i], p.j1[1] = A[(i + 1) % 3], p.j1[2] = A[(i + 2) % 3];
                                 p.j2[0] = B[j], p.j2[1] = B[(j + 2) % 3], p.j2[2] = B[(j + 1) % 3];
@@ -787,7 +799,7 @@ void run_galerkin(const GalerkinJob &job, const std::vector<int> &vs_ptr, const 
                     if (!p.kind)
                         for (int i = 0; i < 3; i++)
                             for (int j = 0; j < 3; j++)
-                                if (A[i] == B[j]) {
+                                if (same(A[i], B[j])) {
                                     p.kind = 3;
                                     p.j1[0] = A[i], p.j1[1] = A[(i + 1) % 3], p.j1[2] = A[(i + 2) % 3];
                                     p.j2[0] = B[j], p.j2[1] = B[(j + 1) % 3], p.j2[2] = B[(j + 2) % 3];

@@ -37,7 +37,12 @@ size_t galerkin_scratch_bytes(int nv, int nf, int npts, int nl, bool G, bool H, 
 // Accumulates the Laplace-domain spectra of all frequencies of `job` into
 // U, P (nl x 3nv x 3nv), Bt (nl x 3nv x 3), Ba (nl x 3nv x 9); any may be null.
 // The buffers are zero-filled first.  tm[1] += non-singular ms (G, H and B), tm[3] += singular ms.
-void run_galerkin(const GalerkinJob &job, const std::vector<int> &vs_ptr, const std::vector<int> &vs_face, const int *F,
-                  cd *U, cd *P, cd *Bt, cd *Ba, double *tm, hipStream_t st);
+// F and vmap (nullable vertex_map_inverse) are the host copies of the job's mesh.
+void run_galerkin(const GalerkinJob &job, const int *F, const int *vmap, cd *U, cd *P, cd *Bt, cd *Ba, double *tm,
+                  hipStream_t st);
+
+// Per face f1, the faces that share a vertex with it (or a copy of one, with vmap): the candidates of the
+// touching-pair classification, f1 itself included, ascending.
+std::vector<std::vector<int>> touching_candidates(const int *F, int nf, int nv, const int *vmap);
 
 }  // namespace cqbem

@@ -93,8 +93,6 @@ void check_inputs(const cqbem_mesh *mesh, const cqbem_params *p) {
     if (p->subdivision < 1 || p->subdivision > 16) throw Err{CQBEM_ERR_ARG, "subdivision must be in [1, 16]"};
     if (p->precision != CQBEM_FP64 && p->precision != CQBEM_MIXED) throw Err{CQBEM_ERR_ARG, "unknown precision"};
     if ((p->which & ~CQBEM_ALL) != 0) throw Err{CQBEM_ERR_ARG, "unknown matrix bits in `which`"};
-    if (p->galerkin && mesh->vertex_map_inverse)
-        throw Err{CQBEM_ERR_UNSUPPORTED, "Galerkin assembly of a vertex-split mesh is not supported"};
     // the device sincos reduces |Im(s) r / c| <= max|s_l| diam / min(c1, c2) exactly below kTrigArgMax
     double lo[3] = {mesh->V[0], mesh->V[1], mesh->V[2]}, hi[3] = {lo[0], lo[1], lo[2]};
     for (int64_t k = 0; k < mesh->nv; k++)
@@ -442,18 +440,9 @@ uint64_t footprint(const cqbem_mesh *mesh, const cqbem_params *p, int outputs_on
         if (!outputs_on_device) bytes += (uint64_t)nmat * matrix_bytes(nv, L);
     }
     if (p->galerkin) {
-        // touching-pair scratch: one 270-complex block per ordered pair of faces sharing a vertex and frequency
-        std::vector<std::vector<int>> star(nv);
-        for (int64_t f = 0; f < mesh->nf; f++)
-            for (int c = 0; c < 3; c++) star[mesh->F[3 * f + c]].push_back((int)f);
+        // touching-pair scratch: one 270-complex block per ordered pair of touching faces and frequency
         uint64_t npairs = 0;
-        std::vector<int> cand;
-        for (int64_t f = 0; f < mesh->nf; f++) {
-            cand.clear();
-            for (int c = 0; c < 3; c++) cand.insert(cand.end(), star[mesh->F[3 * f + c]].begin(), star[mesh->F[3 * f + c]].end());
-            std::sort(cand.begin(), cand.end());
-            npairs += (uint64_t)(std::unique(cand.begin(), cand.end()) - cand.begin());
-        }
+        for (const auto &c : touching_candidates(mesh->F, (int)mesh->nf, nv, mesh->vertex_map_inverse)) npairs += c.size();
         bytes += npairs * ((uint64_t)Lf * 270 * sizeof(cd) + 256);
         const int npts = (int)composite_tri_rule(p->order, p->subdivision).size();
         const bool B = (p->which & (CQBEM_B_TRANS | CQBEM_B_ANGULAR)) != 0;
@@ -578,8 +567,8 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
             const long long lstride = n3 * n3;
             if (wantG) specG.alloc((size_t)Lf * lstride * sizeof(cd));
             if (wantH) specH.alloc((size_t)Lf * lstride * sizeof(cd));
-            run_galerkin(gj, P.vs_ptr, P.vs_face, mesh->F, specG.as<cd>(), specH.as<cd>(), wantB ? specB.as<cd>() : nullptr,
-                         wantB ? specBa.as<cd>() : nullptr, tm, st);
+            run_galerkin(gj, mesh->F, mesh->vertex_map_inverse, specG.as<cd>(), specH.as<cd>(),
+                         wantB ? specB.as<cd>() : nullptr, wantB ? specBa.as<cd>() : nullptr, tm, st);
             Timer t(st, &tm[4]);
             if (wantG) HIPCHK(launch_transform(specG.as<cd>(), n3 * n3, n3 * n3, Lf, tab, L, G, n3 * n3, ctx->num_cus, st));
             if (wantH) HIPCHK(launch_transform(specH.as<cd>(), n3 * n3, n3 * n3, Lf, tab, L, H, n3 * n3, ctx->num_cus, st));
@@ -782,7 +771,7 @@ int cqbem_laplace(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params
                            (int)P.quad.size(), d.pair_patch_pass.as<int>(), d.pair_pass_info.as<int2>(),
                            d.pair_pass_faces.as<int4>(), d.pair_patch_verts.as<int>(), P.pair_npatches, (size_t)1 << 30};
             double tm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            run_galerkin(gj, P.vs_ptr, P.vs_face, mesh->F, wG ? dU.as<cd>() : nullptr, wH ? dP.as<cd>() : nullptr,
+            run_galerkin(gj, mesh->F, mesh->vertex_map_inverse, wG ? dU.as<cd>() : nullptr, wH ? dP.as<cd>() : nullptr,
                          wB ? dBt.as<cd>() : nullptr, wB ? dBa.as<cd>() : nullptr, tm, st);
         }
         if (wG) HIPCHK(hipMemcpyAsync(U, dU.p, dU.bytes, hipMemcpyDeviceToHost, st));

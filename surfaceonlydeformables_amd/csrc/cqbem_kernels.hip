@@ -502,8 +502,12 @@ __global__ void __launch_bounds__(64) k_colloc_singular(Geom g, const double2 *_
             const double a = sqrt(v12[0] * v12[0] + v12[1] * v12[1] + v12[2] * v12[2]);
             unit3(v12);
             const double alpha = acos(u1[0] * u2[0] + u1[1] * u2[1] + u1[2] * u2[2]);
-            const double cb = fmin(1.0, fmax(-1.0, v12[0] * u1[0] + v12[1] * u1[1] + v12[2] * u1[2]));
-            const double beta = acos(cb);
+            // beta = angle(v12, u1), which the reference takes as acos of the clamped dot product
+            // (integrator_collocation.hpp:36); atan2(|v12 x u1|, v12 . u1) is the same angle without acos's
+            // loss of half the digits near 0 (a right angle at y2, common on cube meshes)
+            const double bx = v12[1] * u1[2] - v12[2] * u1[1], by = v12[2] * u1[0] - v12[0] * u1[2],
+                         bz = v12[0] * u1[1] - v12[1] * u1[0];
+            const double beta = atan2(sqrt(bx * bx + by * by + bz * bz), v12[0] * u1[0] + v12[1] * u1[1] + v12[2] * u1[2]);
             // barycentric projection setup (barycentric_coordinates.cuh:15-38)
             const double d00 = v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2];
             const double d01 = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];

@@ -65,7 +65,11 @@ def test_split_cube_time_domain_parity(ctx, oracle):
 def test_reference_cube_scene_laplace(ctx, oracle, l):
     """The mesh and material of the reference's cube scenes (data/cube_drop.json: cube_reordered.obj with
     enable_traction_discontinuity, rho 1000, mu 1e5, nu 0.45 as stored in float, dt 0.005 as stored in
-    float, BDF1, order 5 x subdivision 2; L from the reference's history length)."""
+    float, BDF1, order 5 x subdivision 2; L from the reference's history length).  The fine 0.1 m cube at
+    these frequencies has |s r / c| of a few 1e-2 on neighbouring faces, where the kernels cancel two to
+    three digits and the Guiggiani finite part more: the FP64 oracle itself sits ~5e-13 (U) and ~1e-10 (P)
+    from the long-double result.  The GPU must be at least as close to the long-double rows as the FP64
+    oracle is (x2, floor 1e-13)."""
     V, F = M.read_obj(os.path.join(GOLDEN, "cube_reordered.obj"))
     Vs, Fs, vm = abi.split_traction_discontinuity(V, F, 20.0)
     Vs = M.center(Vs, Fs)
@@ -78,9 +82,73 @@ def test_reference_cube_scene_laplace(ctx, oracle, l):
     s = oracle.frequencies(L, R, dt, 1)[l]
     r0, r1 = 0, 64  # the oracle's rows 0..63 (a cube corner and edges: vertices with copies among them)
     assert len(np.unique(vm[r0:r1])) < r1 - r0
-    o = oracle.laplace(Vs, Fs, oracle.params(c1, c2, rho, nu, dt, R, L, bdf=1, nthreads=0), s, rows=(r0, r1), vmap=vm)
+    o = {mode: oracle.laplace(Vs, Fs, oracle.params(c1, c2, rho, nu, dt, R, L, bdf=1, mode=mode, nthreads=0), s,
+                              rows=(r0, r1), vmap=vm) for mode in (oracle.MODE_FP64, oracle.MODE_LD)}
     sl = slice(3 * r0, 3 * r1)
-    assert rel(g["U"][sl], o["U"][sl]) < 1e-13
-    assert rel(g["P"][sl], o["P"][sl]) < 1e-10
-    assert rel(g["Bt"][sl], o["Bt"][sl]) < 1e-13
-    assert rel(g["Ba"][sl], o["Ba"][sl]) < 1e-13
+    for k in ("U", "P", "Bt", "Ba"):
+        ld = o[oracle.MODE_LD][k][sl]
+        e_gpu, e_f64 = rel(g[k][sl], ld), rel(o[oracle.MODE_FP64][k][sl], ld)
+        assert e_gpu <= max(2 * e_f64, 1e-13), (k, e_gpu, e_f64)
+
+
+def test_split_cube_galerkin_time_domain_parity(ctx, oracle):
+    """Galerkin on the split cube: touching pairs found through the map (integrator_galerkin.hpp:289-291,313-314)."""
+    V, F = M.cube(2, 0.1)
+    V = M.center(V, F)
+    Vs, Fs, vm = abi.split_traction_discontinuity(V, F, 20.0)
+    rho, nu = 1100.0, 0.35
+    _, c1, c2 = M.material_speeds(1e6, nu, rho)
+    L, dt = 9, 5e-4
+    R = 1e-10 ** (1 / (2 * L))
+    print(f"split cube Galerkin time domain nv={len(Vs)} L={L}", flush=True)
+    res = ctx.assemble(abi.make_mesh(Vs, Fs, vm), abi.make_params(c1, c2, rho, nu, dt, R, L, bdf=abi.BDF1, order=2,
+                                                                  subdivision=1, galerkin=True))
+    op = lambda mode: oracle.params(c1, c2, rho, nu, dt, R, L, bdf=1, order=2, subdiv=1, galerkin=True, mode=mode,
+                                    nthreads=0)
+    f64 = oracle.assemble(Vs, Fs, op(oracle.MODE_FP64), vmap=vm)
+    ld = oracle.assemble(Vs, Fs, op(oracle.MODE_LD), vmap=vm)
+    for mine, theirs in (("G", "G"), ("H", "H"), ("B_trans", "Bt"), ("B_angular", "Ba"), ("B", "B6")):
+        a = res[mine].reshape(L, -1); b = f64[theirs].reshape(L, -1); c = ld[theirs].reshape(L, -1)
+        assert rel(a, b) < 1e-8, mine
+        for n in range(L):
+            e_gpu, e_f64 = rel(a[n], c[n]), rel(b[n], c[n])
+            assert e_gpu <= max(2 * e_f64, 1e-12), (mine, n, e_gpu, e_f64)
+
+
+def collapse(vm, n_orig):
+    """P (3 n_orig x 3 n_split): sums the columns (or rows) of a vertex's copies."""
+    P = np.zeros((3 * n_orig, 3 * len(vm)))
+    for v, o in enumerate(vm):
+        P[3 * o:3 * o + 3, 3 * v:3 * v + 3] = np.eye(3)
+    return P
+
+
+@pytest.mark.parametrize("galerkin", [False, True])
+def test_split_collapses_to_unsplit(ctx, galerkin):
+    """Size-independent property of the split at the cube k = 8 size: the copies of a vertex partition its
+    basis function, so summing the columns of the copies gives the unsplit matrices.  In collocation every copy
+    sits at the original's position and sees the same singular faces, so its rows equal the original's row
+    (H_0 up to where its C block sits); in Galerkin the test side collapses too (P A_split P^T = A_unsplit)."""
+    V, F = M.cube(8, 0.5) if not galerkin else M.cube(4, 0.5)
+    V = M.center(V, F)
+    Vs, Fs, vm = abi.split_traction_discontinuity(V, F, 20.0)
+    _, c1, c2 = M.material_speeds(1e5, 0.3, 1000.0)
+    L, dt = (32, 1e-3) if not galerkin else (16, 1e-3)
+    R = 1e-10 ** (1 / (2 * L))
+    kw = dict(galerkin=True, order=3, subdivision=1) if galerkin else {}
+    print(f"split collapse galerkin={galerkin} nv={len(V)}->{len(Vs)} L={L}", flush=True)
+    a = ctx.assemble(abi.make_mesh(Vs, Fs, vm), abi.make_params(c1, c2, 1000.0, 0.3, dt, R, L, **kw))
+    b = ctx.assemble(abi.make_mesh(V, F), abi.make_params(c1, c2, 1000.0, 0.3, dt, R, L, **kw))
+    P = collapse(vm, len(V))
+    first = np.array([np.flatnonzero(vm == o)[0] for o in range(len(V))])
+    rows = (3 * first[:, None] + np.arange(3)).reshape(-1)
+    for k in ("G", "H"):
+        for n in range(L):
+            got = P @ a[k][n] @ P.T if galerkin else a[k][n][rows] @ P.T
+            assert rel(got, b[k][n]) < 1e-8, (k, n, rel(got, b[k][n]))
+    if not galerkin:  # the G rows of all copies of a vertex are the same row, bit for bit
+        for v, o in enumerate(vm):
+            assert np.array_equal(a["G"][:, 3 * v:3 * v + 3, :], a["G"][:, 3 * first[o]:3 * first[o] + 3, :]), v
+    for n in range(L):
+        got = P @ a["B"][n] if galerkin else a["B"][n][rows]
+        assert rel(got, b["B"][n]) < 1e-8, ("B", n)
