@@ -10,6 +10,32 @@
 #include "elastodynamics_cqbem_matrix_collection.hpp"
 
 int main(int argc, char **argv) {
+    if (argc == 3 && (std::string(argv[1]) == "--cache-key" || std::string(argv[1]) == "--dump")) {
+        // --cache-key scene.json: per elastodynamicsCQBEM object group of a full scene file, the key of the
+        // reference's matrix cache and the JSON it hashes; --dump doc.json: the compact serialisation
+        std::ifstream in(argv[2]);
+        std::stringstream ss;
+        ss << in.rdbuf();
+        try {
+            const cqbem::Json doc = cqbem::Json::parse(ss.str());
+            if (std::string(argv[1]) == "--dump") {
+                std::cout << doc.dump() << "\n";
+                return 0;
+            }
+            for (const auto &g : doc["physics_object_groups"].arr) {
+                const cqbem::Json &cfg = g["physics_object_group"];
+                if (cfg.kind != cqbem::Json::Object || !cfg.contains("type") || cfg["type"].string() != "elastodynamicsCQBEM")
+                    continue;
+                cqbem::Json stripped;
+                const std::string key = cqbem::cache_key(cfg, &stripped);
+                std::cout << key << "\t" << stripped.dump() << "\n";
+            }
+        } catch (const std::exception &e) {
+            std::cerr << e.what() << std::endl;
+            return 2;
+        }
+        return 0;
+    }
     if (argc != 3) {
         std::cerr << "usage: collection_driver scene.json out_prefix   (out_prefix '-': no files, timings only)" << std::endl;
         return 2;
@@ -33,8 +59,9 @@ int main(int argc, char **argv) {
     }
     const std::string pre = argv[2];
     if (pre == "-") {
-        std::printf("{\"L\": %zu, \"nv\": %lld, \"mass\": %.17g, \"ms\": %.3f, \"factor_ms\": %.3f}\n", col.max_time_history + 1,
-                    (long long)col.num_vertices, col.mass, col.last_assembly_ms, col.last_factor_ms);
+        std::printf("{\"L\": %zu, \"nv\": %lld, \"mass\": %.17g, \"ms\": %.3f, \"factor_ms\": %.3f, \"cached\": %s}\n",
+                    col.max_time_history + 1, (long long)col.num_vertices, col.mass, col.last_assembly_ms, col.last_factor_ms,
+                    col.loaded_from_cache ? "true" : "false");
         return 0;
     }
     auto dump = [&](const std::string &name, const std::vector<cqbem::Dense> &mats) {
@@ -55,8 +82,8 @@ int main(int argc, char **argv) {
     dump1("rotation_removal_weights", col.rotation_removal_weights);
     dump1("I", std::vector<double>(col.I, col.I + 9));
     std::printf("{\"L\": %zu, \"nv\": %lld, \"R\": %.17g, \"c1\": %.17g, \"c2\": %.17g, \"mass\": %.17g, "
-                "\"ms\": %.3f, \"factor_ms\": %.3f}\n",
+                "\"ms\": %.3f, \"factor_ms\": %.3f, \"cached\": %s}\n",
                 col.max_time_history + 1, (long long)col.num_vertices, col.R, col.c1, col.c2, col.mass, col.last_assembly_ms,
-                col.last_factor_ms);
+                col.last_factor_ms, col.loaded_from_cache ? "true" : "false");
     return 0;
 }

@@ -6,9 +6,14 @@
 #include "cqbem_rules.hpp"
 
 #include <algorithm>
+#include <cerrno>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <filesystem>
+#include <functional>
 #include <fstream>
 #include <mutex>
 #include <sstream>
@@ -35,20 +40,58 @@ struct Parser {
         }
         return false;
     }
+    unsigned hex4() {
+        if (p + 4 > s.size()) err("bad \\u escape");
+        unsigned v = 0;
+        for (int k = 0; k < 4; k++) {
+            const char c = s[p++];
+            v = v * 16 + (c >= '0' && c <= '9' ? c - '0' : c >= 'a' && c <= 'f' ? c - 'a' + 10 : c >= 'A' && c <= 'F' ? c - 'A' + 10 : (err("bad \\u escape"), 0));
+        }
+        return v;
+    }
+    static void utf8(std::string &out, unsigned cp) {
+        if (cp < 0x80) {
+            out += (char)cp;
+        } else if (cp < 0x800) {
+            out += (char)(0xC0 | cp >> 6), out += (char)(0x80 | (cp & 0x3F));
+        } else if (cp < 0x10000) {
+            out += (char)(0xE0 | cp >> 12), out += (char)(0x80 | (cp >> 6 & 0x3F)), out += (char)(0x80 | (cp & 0x3F));
+        } else {
+            out += (char)(0xF0 | cp >> 18), out += (char)(0x80 | (cp >> 12 & 0x3F));
+            out += (char)(0x80 | (cp >> 6 & 0x3F)), out += (char)(0x80 | (cp & 0x3F));
+        }
+    }
     std::string str() {
         if (s[p] != '"') err("expected string");
         p++;
         std::string out;
         while (p < s.size() && s[p] != '"') {
-            if (s[p] == '\\') {
-                p++;
-                if (p >= s.size()) err("bad escape");
-                const char c = s[p];
-                out += c == 'n' ? '\n' : c == 't' ? '\t' : c;
-            } else {
-                out += s[p];
+            if (s[p] != '\\') {
+                out += s[p++];
+                continue;
             }
-            p++;
+            if (++p >= s.size()) err("bad escape");
+            const char c = s[p++];
+            switch (c) {
+            case 'b': out += '\b'; break;
+            case 'f': out += '\f'; break;
+            case 'n': out += '\n'; break;
+            case 'r': out += '\r'; break;
+            case 't': out += '\t'; break;
+            case '"': case '\\': case '/': out += c; break;
+            case 'u': {
+                unsigned cp = hex4();
+                if (cp >= 0xD800 && cp < 0xDC00) {  // surrogate pair
+                    if (!lit("\\u")) err("unpaired surrogate");
+                    const unsigned lo = hex4();
+                    if (lo < 0xDC00 || lo >= 0xE000) err("unpaired surrogate");
+                    cp = 0x10000 + ((cp - 0xD800) << 10) + (lo - 0xDC00);
+                }
+                utf8(out, cp);
+                break;
+            }
+            default: err("bad escape");
+            }
         }
         if (p >= s.size()) err("unterminated string");
         p++;
@@ -104,8 +147,15 @@ struct Parser {
         if (lit("false")) return j.kind = Json::Bool, j.b = false, j;
         if (lit("null")) return j;
         char *end = nullptr;
-        j.num = std::strtod(s.c_str() + p, &end);
-        if (end == s.c_str() + p) err("bad value");
+        const char *start = s.c_str() + p;
+        j.num = std::strtod(start, &end);
+        if (end == start) err("bad value");
+        const std::string tok(start, static_cast<const char *>(end));
+        if (tok.find_first_of(".eE") == std::string::npos) {  // an integer literal, if it fits
+            errno = 0;
+            const long long v = std::strtoll(tok.c_str(), nullptr, 10);
+            if (errno == 0) j.integer = true, j.inum = v;
+        }
         p = end - s.c_str();
         j.kind = Json::Number;
         return j;
@@ -141,6 +191,274 @@ const std::string &Json::string() const {
 bool Json::boolean() const {
     if (kind != Bool) throw std::runtime_error("json: expected a boolean");
     return b;
+}
+
+namespace {
+
+// Shortest-digit generation of the reference's JSON library for doubles: Grisu2 (F. Loitsch, "Printing
+// floating-point numbers quickly and accurately with integers", PLDI 2010) with the scaled boundaries narrowed
+// by one unit, so that the digits read back to the same double; most of the time they are the shortest
+// such digits, and where Grisu2 emits a longer string the cache key must contain that same string.
+struct DiyFp {
+    uint64_t f;
+    int e;
+};
+
+DiyFp diy_mul(DiyFp x, DiyFp y) {  // upper 64 bits of the 128-bit product, rounded half up
+    const unsigned __int128 p = (unsigned __int128)x.f * y.f;
+    const uint64_t hi = (uint64_t)(p >> 64), lo = (uint64_t)p;
+    return {hi + (lo >> 63), x.e + y.e + 64};
+}
+
+DiyFp diy_normalize(DiyFp x) {
+    while ((x.f >> 63) == 0) x.f <<= 1, x.e--;
+    return x;
+}
+
+// 10^k for k = -300, -292, ..., 324: the significand rounded to 64 bits and its binary exponent (computed
+// from the exact powers of ten).
+struct CachedPower {
+    uint64_t f;
+    int e, k;
+};
+constexpr CachedPower kCachedPowers[] = {
+    {0xAB70FE17C79AC6CA, -1060, -300}, {0xFF77B1FCBEBCDC4F, -1034, -292},
+    {0xBE5691EF416BD60C, -1007, -284}, {0x8DD01FAD907FFC3C, -980, -276},
+    {0xD3515C2831559A83, -954, -268}, {0x9D71AC8FADA6C9B5, -927, -260},
+    {0xEA9C227723EE8BCB, -901, -252}, {0xAECC49914078536D, -874, -244},
+    {0x823C12795DB6CE57, -847, -236}, {0xC21094364DFB5637, -821, -228},
+    {0x9096EA6F3848984F, -794, -220}, {0xD77485CB25823AC7, -768, -212},
+    {0xA086CFCD97BF97F4, -741, -204}, {0xEF340A98172AACE5, -715, -196},
+    {0xB23867FB2A35B28E, -688, -188}, {0x84C8D4DFD2C63F3B, -661, -180},
+    {0xC5DD44271AD3CDBA, -635, -172}, {0x936B9FCEBB25C996, -608, -164},
+    {0xDBAC6C247D62A584, -582, -156}, {0xA3AB66580D5FDAF6, -555, -148},
+    {0xF3E2F893DEC3F126, -529, -140}, {0xB5B5ADA8AAFF80B8, -502, -132},
+    {0x87625F056C7C4A8B, -475, -124}, {0xC9BCFF6034C13053, -449, -116},
+    {0x964E858C91BA2655, -422, -108}, {0xDFF9772470297EBD, -396, -100},
+    {0xA6DFBD9FB8E5B88F, -369, -92}, {0xF8A95FCF88747D94, -343, -84},
+    {0xB94470938FA89BCF, -316, -76}, {0x8A08F0F8BF0F156B, -289, -68},
+    {0xCDB02555653131B6, -263, -60}, {0x993FE2C6D07B7FAC, -236, -52},
+    {0xE45C10C42A2B3B06, -210, -44}, {0xAA242499697392D3, -183, -36},
+    {0xFD87B5F28300CA0E, -157, -28}, {0xBCE5086492111AEB, -130, -20},
+    {0x8CBCCC096F5088CC, -103, -12}, {0xD1B71758E219652C, -77, -4},
+    {0x9C40000000000000, -50, 4}, {0xE8D4A51000000000, -24, 12},
+    {0xAD78EBC5AC620000, 3, 20}, {0x813F3978F8940984, 30, 28},
+    {0xC097CE7BC90715B3, 56, 36}, {0x8F7E32CE7BEA5C70, 83, 44},
+    {0xD5D238A4ABE98068, 109, 52}, {0x9F4F2726179A2245, 136, 60},
+    {0xED63A231D4C4FB27, 162, 68}, {0xB0DE65388CC8ADA8, 189, 76},
+    {0x83C7088E1AAB65DB, 216, 84}, {0xC45D1DF942711D9A, 242, 92},
+    {0x924D692CA61BE758, 269, 100}, {0xDA01EE641A708DEA, 295, 108},
+    {0xA26DA3999AEF774A, 322, 116}, {0xF209787BB47D6B85, 348, 124},
+    {0xB454E4A179DD1877, 375, 132}, {0x865B86925B9BC5C2, 402, 140},
+    {0xC83553C5C8965D3D, 428, 148}, {0x952AB45CFA97A0B3, 455, 156},
+    {0xDE469FBD99A05FE3, 481, 164}, {0xA59BC234DB398C25, 508, 172},
+    {0xF6C69A72A3989F5C, 534, 180}, {0xB7DCBF5354E9BECE, 561, 188},
+    {0x88FCF317F22241E2, 588, 196}, {0xCC20CE9BD35C78A5, 614, 204},
+    {0x98165AF37B2153DF, 641, 212}, {0xE2A0B5DC971F303A, 667, 220},
+    {0xA8D9D1535CE3B396, 694, 228}, {0xFB9B7CD9A4A7443C, 720, 236},
+    {0xBB764C4CA7A44410, 747, 244}, {0x8BAB8EEFB6409C1A, 774, 252},
+    {0xD01FEF10A657842C, 800, 260}, {0x9B10A4E5E9913129, 827, 268},
+    {0xE7109BFBA19C0C9D, 853, 276}, {0xAC2820D9623BF429, 880, 284},
+    {0x80444B5E7AA7CF85, 907, 292}, {0xBF21E44003ACDD2D, 933, 300},
+    {0x8E679C2F5E44FF8F, 960, 308}, {0xD433179D9C8CB841, 986, 316},
+    {0x9E19DB92B4E31BA9, 1013, 324},
+};
+
+// Digits of v (finite, > 0) into buf; v = digits x 10^dec_exp.
+void grisu2(double v, char *buf, int &len, int &dec_exp) {
+    uint64_t bits;
+    std::memcpy(&bits, &v, 8);
+    const uint64_t F = bits & ((1ULL << 52) - 1);
+    const int E = (int)(bits >> 52);
+    const DiyFp x = E == 0 ? DiyFp{F, 1 - 1075} : DiyFp{F | (1ULL << 52), E - 1075};
+    // the half-way points to the neighbouring doubles; the lower one is closer at a power of two
+    const bool closer = F == 0 && E > 1;
+    const DiyFp m_plus = diy_normalize({2 * x.f + 1, x.e - 1});
+    DiyFp m_minus = closer ? DiyFp{4 * x.f - 1, x.e - 2} : DiyFp{2 * x.f - 1, x.e - 1};
+    m_minus = {m_minus.f << (m_minus.e - m_plus.e), m_plus.e};
+    const DiyFp w = diy_normalize(x);
+    // cached power bringing the product's binary exponent into [-60, -32]: the smallest tabulated decimal
+    // exponent >= ceil((-60 - e - 1) log10 2), log10 2 taken as 78913 / 2^18
+    const int fexp = -60 - m_plus.e - 1;
+    const int kmin = (fexp * 78913) / (1 << 18) + (fexp > 0 ? 1 : 0);
+    const CachedPower c = kCachedPowers[(300 + kmin + 7) / 8];
+    const DiyFp cp{c.f, c.e};
+    const DiyFp W = diy_mul(w, cp), Wm = diy_mul(m_minus, cp), Wp = diy_mul(m_plus, cp);
+    const DiyFp lo{Wm.f + 1, Wm.e}, hi{Wp.f - 1, Wp.e};  // narrowed by one unit: the scaling is inexact
+    dec_exp = -c.k;
+    // digits of hi, stopped once hi - digits-so-far is within delta = hi - lo
+    uint64_t delta = hi.f - lo.f, dist = hi.f - W.f;
+    const int sh = -hi.e;
+    const uint64_t one = 1ULL << sh;
+    uint32_t p1 = (uint32_t)(hi.f >> sh);
+    uint64_t p2 = hi.f & (one - 1);
+    auto round_last = [&](uint64_t rest, uint64_t ten_k) {  // move the last digit towards W while it stays in range
+        while (rest < dist && delta - rest >= ten_k && (rest + ten_k < dist || dist - rest > rest + ten_k - dist)) {
+            buf[len - 1]--;
+            rest += ten_k;
+        }
+    };
+    len = 0;
+    uint32_t pow10 = 1;
+    int n = 1;
+    while (n < 10 && pow10 * 10ULL <= p1) pow10 *= 10, n++;
+    while (n > 0) {
+        buf[len++] = (char)('0' + p1 / pow10);
+        p1 %= pow10;
+        n--;
+        const uint64_t rest = ((uint64_t)p1 << sh) + p2;
+        if (rest <= delta) {
+            dec_exp += n;
+            round_last(rest, (uint64_t)pow10 << sh);
+            return;
+        }
+        pow10 /= 10;
+    }
+    int m = 0;
+    for (;;) {
+        p2 *= 10;
+        buf[len++] = (char)('0' + (p2 >> sh));
+        p2 &= one - 1;
+        m++;
+        delta *= 10;
+        dist *= 10;
+        if (p2 <= delta) break;
+    }
+    dec_exp -= m;
+    round_last(p2, one);
+}
+
+// A finite double as the reference's JSON library prints it: the Grisu2 digits, then fixed notation for
+// decimal exponents in (-4, 15] (with ".0" on integral values) and d.ddde+XX otherwise.
+std::string format_double(double v) {
+    if (!std::isfinite(v)) return "null";
+    std::string out = std::signbit(v) ? "-" : "";
+    v = std::fabs(v);
+    if (v == 0.0) return out + "0.0";
+    char buf[32];
+    int len = 0, dec = 0;
+    grisu2(v, buf, len, dec);
+    const std::string digits(buf, buf + len);
+    const int k = len, n = len + dec;  // value = 0.digits x 10^n
+    constexpr int kMinExp = -4, kMaxExp = 15;
+    if (k <= n && n <= kMaxExp) return out + digits + std::string(n - k, '0') + ".0";
+    if (0 < n && n <= kMaxExp) return out + digits.substr(0, n) + "." + digits.substr(n);
+    if (kMinExp < n && n <= 0) return out + "0." + std::string(-n, '0') + digits;
+    std::string m = k == 1 ? digits : digits.substr(0, 1) + "." + digits.substr(1);
+    const int x = n - 1;
+    char ex[8];
+    std::snprintf(ex, sizeof(ex), "%c%02d", x < 0 ? '-' : '+', x < 0 ? -x : x);
+    return out + m + "e" + ex;
+}
+
+void dump_string(std::string &o, const std::string &str) {
+    o += '"';
+    for (unsigned char c : str) {
+        switch (c) {
+        case '\b': o += "\\b"; break;
+        case '\t': o += "\\t"; break;
+        case '\n': o += "\\n"; break;
+        case '\f': o += "\\f"; break;
+        case '\r': o += "\\r"; break;
+        case '"': o += "\\\""; break;
+        case '\\': o += "\\\\"; break;
+        default:
+            if (c < 0x20) {
+                char u[8];
+                std::snprintf(u, sizeof(u), "\\u%04x", c);
+                o += u;
+            } else {
+                o += (char)c;
+            }
+        }
+    }
+    o += '"';
+}
+
+void dump_value(std::string &o, const Json &j, int indent, int level) {
+    const bool pretty = indent >= 0;
+    auto nl = [&](int lv) {
+        if (pretty) o += '\n' + std::string((size_t)(indent * lv), ' ');
+    };
+    switch (j.kind) {
+    case Json::Null: o += "null"; return;
+    case Json::Bool: o += j.b ? "true" : "false"; return;
+    case Json::Number: o += j.integer ? std::to_string(j.inum) : format_double(j.num); return;
+    case Json::String: dump_string(o, j.str); return;
+    case Json::Array:
+        if (j.arr.empty()) {
+            o += "[]";
+            return;
+        }
+        o += '[';
+        for (size_t k = 0; k < j.arr.size(); k++) {
+            if (k) o += ',';
+            nl(level + 1);
+            dump_value(o, j.arr[k], indent, level + 1);
+        }
+        nl(level);
+        o += ']';
+        return;
+    case Json::Object:
+        if (j.obj.empty()) {
+            o += "{}";
+            return;
+        }
+        o += '{';
+        bool first = true;
+        for (const auto &kv : j.obj) {
+            if (!first) o += ',';
+            first = false;
+            nl(level + 1);
+            dump_string(o, kv.first);
+            o += pretty ? ": " : ":";
+            dump_value(o, kv.second, indent, level + 1);
+        }
+        nl(level);
+        o += '}';
+        return;
+    }
+}
+
+}  // namespace
+
+std::string Json::dump(int indent) const {
+    std::string o;
+    dump_value(o, *this, indent, 0);
+    return o;
+}
+
+std::string cache_key(const Json &group, Json *stripped) {
+    Json copy = group;
+    if (copy.contains("simulation")) {
+        auto &sim = copy.obj["simulation"].obj;
+        sim.erase("compression_ratio");
+        sim.erase("update_local_frame");
+    }
+    std::ostringstream ss;
+    ss << std::uppercase << std::hex << std::hash<std::string>()(copy.dump());
+    if (stripped) *stripped = std::move(copy);
+    return ss.str();
+}
+
+bool save_matrix(const Dense &m, const std::string &path) {
+    std::ofstream out(path, std::ios::binary);
+    if (!out) return false;
+    const int64_t rc[2] = {m.rows, m.cols};
+    out.write(reinterpret_cast<const char *>(rc), sizeof(rc));
+    out.write(reinterpret_cast<const char *>(m.data.data()), (std::streamsize)(m.data.size() * sizeof(double)));
+    return (bool)out;
+}
+
+bool load_matrix(Dense &m, const std::string &path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) return false;
+    int64_t rc[2] = {0, 0};
+    in.read(reinterpret_cast<char *>(rc), sizeof(rc));
+    if (!in || rc[0] < 0 || rc[1] < 0) return false;
+    m = Dense(rc[0], rc[1]);
+    in.read(reinterpret_cast<char *>(m.data.data()), (std::streamsize)(m.data.size() * sizeof(double)));
+    return (bool)in;
 }
 
 // ------------------------------------------------------------------- OBJ --
@@ -296,25 +614,57 @@ bool ElastodynamicsCQBEMMatrixCollection::run(const Json &config) {
     R = sim.contains("cq_radius") ? sim["cq_radius"].number() : std::pow(1e-10, 1. / (2. * max_time_history));
     const size_t L = max_time_history + 1;
 
-    std::string err;
-    cqbem_context *ctx = context(err);
-    if (!ctx) return fail(err);
+    // the reference's matrix cache: <coeffs_folder_path>/elastodynamicsCQBEM_<key>/{G,H}_<n>.mat, H stored
+    // un-negated, config.json beside them (matrix_collection.cpp:41-57,77-94,135-143)
+    std::string folder;
+    if (sim.contains("coeffs_folder_path")) {
+        Json stripped;
+        folder = sim["coeffs_folder_path"].string() + "/elastodynamicsCQBEM_" + cache_key(config, &stripped);
+        std::filesystem::create_directories(folder);
+        std::ofstream(folder + "/config.json") << stripped.dump(4);
+    }
     const int64_t n3 = 3 * num_vertices;
-    std::vector<double> g((size_t)L * n3 * n3), h((size_t)L * n3 * n3), b((size_t)L * n3 * 6);
-    cqbem_mesh mesh{V.data(), num_vertices, F.data(), (int64_t)(F.size() / 3),
-                    vertex_map_inverse.empty() ? nullptr : vertex_map_inverse.data()};
-    cqbem_params prm{c1, c2, rho, nu, dt_, R, (int32_t)L, multistep_method == BDF1 ? CQBEM_BDF1 : CQBEM_BDF2,
-                     order, sub, use_galerkin ? 1 : 0, CQBEM_FP64, CQBEM_ALL};
-    cqbem_outputs out{g.data(), h.data(), nullptr, nullptr, b.data(), 0};
-    const auto t0 = std::chrono::steady_clock::now();
-    if (cqbem_assemble(ctx, &mesh, &prm, &out) != CQBEM_OK) return fail(cqbem_last_error(ctx));
-    last_assembly_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     G_matrices.assign(L, Dense(n3, n3));
     H_matrices.assign(L, Dense(n3, n3));
     B_matrices.assign(L, Dense(n3, 6));
+    loaded_from_cache = !folder.empty();
+    for (size_t n = 0; n < L && loaded_from_cache; n++)
+        loaded_from_cache = std::filesystem::exists(folder + "/H_" + std::to_string(n) + ".mat") &&
+                            std::filesystem::exists(folder + "/G_" + std::to_string(n) + ".mat");
+    if (loaded_from_cache)
+        for (size_t n = 0; n < L; n++) {
+            if (!load_matrix(H_matrices[n], folder + "/H_" + std::to_string(n) + ".mat") ||
+                !load_matrix(G_matrices[n], folder + "/G_" + std::to_string(n) + ".mat") || G_matrices[n].rows != n3 ||
+                G_matrices[n].cols != n3 || H_matrices[n].rows != n3 || H_matrices[n].cols != n3)
+                return fail("cached matrices in " + folder + " do not match this mesh");
+            for (double &x : H_matrices[n].data) x = -x;  // matrix_collection.cpp:140
+        }
+
+    std::string err;
+    cqbem_context *ctx = context(err);
+    if (!ctx) return fail(err);
+    std::vector<double> g, h, b((size_t)L * n3 * 6);
+    if (!loaded_from_cache) g.resize((size_t)L * n3 * n3), h.resize((size_t)L * n3 * n3);
+    cqbem_mesh mesh{V.data(), num_vertices, F.data(), (int64_t)(F.size() / 3),
+                    vertex_map_inverse.empty() ? nullptr : vertex_map_inverse.data()};
+    // with cached G and H only B is assembled (the reference computes B on every init, :146-177)
+    cqbem_params prm{c1, c2, rho, nu, dt_, R, (int32_t)L, multistep_method == BDF1 ? CQBEM_BDF1 : CQBEM_BDF2,
+                     order, sub, use_galerkin ? 1 : 0, CQBEM_FP64,
+                     loaded_from_cache ? (CQBEM_B_TRANS | CQBEM_B_ANGULAR) : CQBEM_ALL};
+    cqbem_outputs out{loaded_from_cache ? nullptr : g.data(), loaded_from_cache ? nullptr : h.data(), nullptr, nullptr,
+                      b.data(), 0};
+    const auto t0 = std::chrono::steady_clock::now();
+    if (cqbem_assemble(ctx, &mesh, &prm, &out) != CQBEM_OK) return fail(cqbem_last_error(ctx));
+    last_assembly_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     for (size_t n = 0; n < L; n++) {
-        std::copy(g.begin() + n * n3 * n3, g.begin() + (n + 1) * n3 * n3, G_matrices[n].data.begin());
-        for (int64_t e = 0; e < n3 * n3; e++) H_matrices[n].data[e] = -h[n * n3 * n3 + e];  // matrix_collection.cpp:140
+        if (!loaded_from_cache) {
+            std::copy(g.begin() + n * n3 * n3, g.begin() + (n + 1) * n3 * n3, G_matrices[n].data.begin());
+            std::copy(h.begin() + n * n3 * n3, h.begin() + (n + 1) * n3 * n3, H_matrices[n].data.begin());
+            if (!folder.empty() && (!save_matrix(H_matrices[n], folder + "/H_" + std::to_string(n) + ".mat") ||
+                                    !save_matrix(G_matrices[n], folder + "/G_" + std::to_string(n) + ".mat")))
+                return fail("cannot write the matrix cache in " + folder);
+            for (double &x : H_matrices[n].data) x = -x;  // matrix_collection.cpp:140
+        }
         std::copy(b.begin() + n * n3 * 6, b.begin() + (n + 1) * n3 * 6, B_matrices[n].data.begin());
     }
     (void)cuda_thread_per_block_;  // launch shapes are chosen by the library

@@ -19,9 +19,11 @@ struct Json {
     enum Kind { Null, Bool, Number, String, Array, Object } kind = Null;
     bool b = false;
     double num = 0.0;
+    bool integer = false;  // written without fraction or exponent (then inum holds it exactly)
+    int64_t inum = 0;
     std::string str;
     std::vector<Json> arr;
-    std::map<std::string, Json> obj;
+    std::map<std::string, Json> obj;  // keys in byte order, as the reference's JSON library keeps them
 
     static Json parse(const std::string &text);  // throws std::runtime_error on malformed input
     bool contains(const std::string &k) const { return kind == Object && obj.count(k) != 0; }
@@ -29,7 +31,14 @@ struct Json {
     double number() const;
     const std::string &string() const;
     bool boolean() const;
+    // Serialisation in the reference's JSON library format: compact (indent < 0) or pretty-printed.
+    std::string dump(int indent = -1) const;
 };
+
+// Key of the reference's matrix cache for one object group (matrix_collection.cpp:41-57): the group's
+// JSON without simulation.compression_ratio and simulation.update_local_frame, compact dump, std::hash,
+// upper-case hex.  `stripped` receives that JSON.
+std::string cache_key(const Json &group, Json *stripped = nullptr);
 
 // Row-major dense double matrix (the layout of the reference's MatrixXs, type_declaration.hpp:16-32).
 struct Dense {
@@ -41,6 +50,11 @@ struct Dense {
     double operator()(int64_t r, int64_t c) const { return data[(size_t)(r * cols + c)]; }
 };
 
+// The reference's .mat format (include/matrix_io.hpp:13-61): int64 rows, int64 cols, then the entries in
+// row-major order (MatrixXs is RowMajor, type_declaration.hpp:18,32).
+bool save_matrix(const Dense &m, const std::string &path);
+bool load_matrix(Dense &m, const std::string &path);
+
 enum MultistepMethod { BDF1, BDF2 };
 
 class ElastodynamicsCQBEMMatrixCollection {
@@ -50,8 +64,8 @@ class ElastodynamicsCQBEMMatrixCollection {
 
     // Reads mesh (OBJ file, scale, enable_traction_discontinuity with sharp_edge_threshold_degree),
     // material (rho, mu, nu) and simulation (dt, gaussian_quadrature_order, quadrature_subdivision,
-    // multistep_method, use_galerkin) like the reference; optional simulation.cq_length /
-    // simulation.cq_radius override the mesh-derived L and R.
+    // multistep_method, use_galerkin, coeffs_folder_path: the reference's on-disk G/H cache) like the
+    // reference; optional simulation.cq_length / simulation.cq_radius override the mesh-derived L and R.
     bool init(const Json &config);
     // Same, with an in-memory mesh instead of config["mesh"].
     bool init(const Json &config, const std::vector<double> &V, const std::vector<int32_t> &F);
@@ -74,6 +88,7 @@ class ElastodynamicsCQBEMMatrixCollection {
     bool update_local_frame = true;
     MultistepMethod multistep_method = BDF2;
     double last_assembly_ms = 0.0, last_factor_ms = 0.0;
+    bool loaded_from_cache = false;  // G and H came from simulation.coeffs_folder_path
 
     // --- after the assembly (matrix_collection.cpp:174-215, physics_base_matrix_collection.cpp:40-50) ---
     Dense translational_acc_mat, rotational_acc_mat;  // 3 x 3 nv

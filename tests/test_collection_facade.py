@@ -204,3 +204,50 @@ def test_facade_reference_cube_scene(tmp_path):
     assert rel(G, direct["G"]) < 1e-9
     assert rel(-H, direct["H"]) < 1e-9
     assert rel(B, direct["B"]) < 1e-9
+
+
+@pytest.mark.gpu
+def test_facade_matrix_cache(tmp_path):
+    """simulation.coeffs_folder_path: the first init assembles and writes G_n.mat / H_n.mat (H un-negated,
+    include/matrix_io.hpp layout) and config.json under elastodynamicsCQBEM_<key>; the next init finds them,
+    assembles only B, and reads G and H back; a file edited in the reference's format is what gets read."""
+    V, F = M.icosphere(1, 0.1)
+    V = M.jitter(V, F, 0.01, 4)
+    path, cfg = scene(tmp_path, V, F, coeffs_folder_path=str(tmp_path / "cache"), update_local_frame=True)
+    print("collection facade matrix cache ico1", flush=True)
+    wrapper = tmp_path / "scene_wrapper.json"
+    cfg["type"] = "elastodynamicsCQBEM"
+    path.write_text(json.dumps(cfg))
+    wrapper.write_text(json.dumps({"physics_object_groups": [{"physics_object_group": cfg}]}))
+    key_line = subprocess.run([DRIVER, "--cache-key", str(wrapper)], capture_output=True, text=True).stdout.strip()
+    key, stripped = key_line.split("\t")
+    folder = tmp_path / "cache" / f"elastodynamicsCQBEM_{key}"
+
+    r1 = run(path, tmp_path / "a")
+    assert r1.returncode == 0, r1.stderr
+    info = json.loads(r1.stdout.strip().splitlines()[-1])
+    assert info["cached"] is False
+    L, n3 = info["L"], 3 * len(V)
+    G = np.fromfile(tmp_path / "a_G.bin").reshape(L, n3, n3)
+    H = np.fromfile(tmp_path / "a_H.bin").reshape(L, n3, n3)
+    B = np.fromfile(tmp_path / "a_B.bin").reshape(L, n3, 6)
+    assert json.loads((folder / "config.json").read_text()) == json.loads(stripped)
+    for n in range(L):
+        for name, want in (("G", G[n]), ("H", -H[n])):
+            raw = (folder / f"{name}_{n}.mat").read_bytes()
+            assert np.frombuffer(raw[:16], np.int64).tolist() == [n3, n3]
+            assert np.array_equal(np.frombuffer(raw[16:], np.float64).reshape(n3, n3), want), (name, n)
+
+    r2 = run(path, tmp_path / "b")
+    assert r2.returncode == 0, r2.stderr
+    assert json.loads(r2.stdout.strip().splitlines()[-1])["cached"] is True
+    for name, want in (("G", G), ("H", H), ("B", B)):
+        got = np.fromfile(tmp_path / f"b_{name}.bin").reshape(want.shape)
+        assert np.array_equal(got, want), name
+
+    # a cache entry written in the reference's format (int64 rows, int64 cols, row-major doubles) is read as is
+    edited = 2.0 * G[1]
+    (folder / "G_1.mat").write_bytes(np.array([n3, n3], np.int64).tobytes() + edited.tobytes())
+    r3 = run(path, tmp_path / "c")
+    assert r3.returncode == 0, r3.stderr
+    assert np.array_equal(np.fromfile(tmp_path / "c_G.bin").reshape(L, n3, n3)[1], edited)
