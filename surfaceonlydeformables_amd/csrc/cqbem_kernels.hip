@@ -463,6 +463,8 @@ __global__ void __launch_bounds__(256) k_colloc_b(Geom g, const double4 *__restr
 //   KIND 1: B_trans + B_angular, same polar rule
 //   KIND 2: H, Guiggiani finite-part rule (integrator_collocation.hpp:65-124)
 // ------------------------------------------------------------------------
+constexpr int kSingMaxFaces = 16;  // singular faces per batch of one (row, frequency) wavefront
+
 __device__ inline void unit3(double v[3]) {
     const double n = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     if (n > 0) v[0] /= n, v[1] /= n, v[2] /= n;
@@ -473,13 +475,21 @@ __global__ void __launch_bounds__(64) k_colloc_singular(Geom g, const double2 *_
                                                          const Freq *__restrict__ freqs, int l0, int nl, Mat m, int row0,
                                                          int nrows, cd *__restrict__ U, long long ld, long long lstride,
                                                          cd *__restrict__ Bt, cd *__restrict__ Ba, long long nv3) {
-    const int idx = blockIdx.x * 64 + threadIdx.x;
-    if (idx >= nrows * nl) return;
-    const int il = idx / nl, l = idx % nl;
+    constexpr int NE = KIND == 1 ? 36 : 27;  // per face: B 9 + 27 entries; G/H 3 corners x 3 x 3
+    __shared__ cd fsum[kSingMaxFaces][NE];
+    __shared__ int fj[kSingMaxFaces][3];
+    const int il = blockIdx.x / nl, l = blockIdx.x % nl;
     const int i = row0 + il;
     const Freq f = freqs[l0 + l];
-    for (int t = g.sg_ptr[i]; t < g.sg_ptr[i + 1]; ++t) {
-        const int pk = g.sg_face[t];
+    // GS lanes per singular face, lane t of a group takes the angular points t, t + GS, ...
+    const int GS = nline < 64 / kSingMaxFaces ? 64 / kSingMaxFaces : (nline > 64 ? 64 : nline);
+    const int FB = 64 / GS;  // faces per batch
+    const int lane = threadIdx.x, slot = lane / GS, t = lane % GS;
+    const int fbeg = g.sg_ptr[i], fend = g.sg_ptr[i + 1];
+    for (int fb = fbeg; fb < fend; fb += FB) {
+        const int nb = min(FB, fend - fb);
+        const bool active = slot < nb;
+        const int pk = g.sg_face[active ? fb + slot : fbeg];
         const int fc = pk >> 2, c = pk & 3;
         const int j[3] = {g.F[3 * fc + c], g.F[3 * fc + (c + 1) % 3], g.F[3 * fc + (c + 2) % 3]};
         double y1[3], y2[3], y3[3], n[3];
@@ -487,6 +497,7 @@ __global__ void __launch_bounds__(64) k_colloc_singular(Geom g, const double2 *_
             y1[k] = g.V[3 * j[0] + k], y2[k] = g.V[3 * j[1] + k], y3[k] = g.V[3 * j[2] + k], n[k] = g.N[3 * fc + k];
         double v1[3], v2[3];
         for (int k = 0; k < 3; k++) v1[k] = y2[k] - y1[k], v2[k] = y3[k] - y1[k];
+        cd fout[NE];  // this face's contribution (the group leader's after the reduction)
         if (KIND == 0 || KIND == 1) {
             // polar frame around y1
             double u1[3] = {v1[0], v1[1], v1[2]}, u2[3] = {v2[0], v2[1], v2[2]};
@@ -525,7 +536,7 @@ __global__ void __launch_bounds__(64) k_colloc_singular(Geom g, const double2 *_
                 for (int k = 0; k < 6; k++) M[k] = cd{0, 0};
                 m2 = S = S3 = cd{0, 0};
             }
-            for (int it = 0; it < nline; ++it) {
+            for (int it = active ? t : nline; it < nline; it += GS) {
                 const double th = alpha * line[it].x, wth = alpha * line[it].y;
                 double sth, cth;
                 sincos(th, &sth, &cth);
@@ -573,32 +584,29 @@ __global__ void __launch_bounds__(64) k_colloc_singular(Geom g, const double2 *_
                     }
                 }
             }
-            const long long base = (long long)l * lstride + (long long)(3 * il) * ld;
             if (KIND == 0) {
                 for (int k = 0; k < 3; k++)
                     for (int a2 = 0; a2 < 3; a2++)
                         for (int b = 0; b < 3; b++) {
                             cd v = ga[k][sidx(a2, b)];
                             if (a2 == b) v += gb[k];
-                            U[base + a2 * ld + 3 * (long long)j[k] + b] += v;
+                            fout[k * 9 + a2 * 3 + b] = v;
                         }
             } else {
-                cd *bt_out = Bt + (long long)l * nv3 * 3 + (long long)(3 * i) * 3;
-                cd *ba_out = Ba + (long long)l * nv3 * 9 + (long long)(3 * i) * 9;
                 for (int a2 = 0; a2 < 3; a2++)
                     for (int b = 0; b < 3; b++) {
                         cd v = {0, 0};
                         fma_cr(v, mt[a2], n[b]);
                         fma_cr(v, mt[b], n[a2]);
                         if (a2 == b) v += m2;
-                        bt_out[a2 * 3 + b] += v;
+                        fout[a2 * 3 + b] = v;
                         for (int k = 0; k < 3; k++) {
                             cd u = {0, 0};
                             fma_cr(u, M[sidx(a2, k)], n[b]);
                             fma_cr(u, M[sidx(b, k)], n[a2]);
                             fma_cr(u, S, (b == k ? n[a2] : 0.0) + (a2 == k ? n[b] : 0.0));
                             if (a2 == b) fma_cr(u, S3, n[k]);
-                            ba_out[a2 * 9 + b * 3 + k] += u;
+                            fout[9 + a2 * 9 + b * 3 + k] = u;
                         }
                     }
             }
@@ -609,7 +617,7 @@ __global__ void __launch_bounds__(64) k_colloc_singular(Geom g, const double2 *_
             cd out[27];
             for (int k = 0; k < 27; k++) out[k] = cd{0, 0};
             const double half_pi = 1.5707963267948966192;
-            for (int it = 0; it < nline; ++it) {
+            for (int it = active ? t : nline; it < nline; it += GS) {
                 const double th = half_pi * line[it].x, wth = half_pi * line[it].y;
                 double st, ct;
                 sincos(th, &st, &ct);
@@ -654,11 +662,35 @@ __global__ void __launch_bounds__(64) k_colloc_singular(Geom g, const double2 *_
                         }
                     }
             }
-            const long long base = (long long)l * lstride + (long long)(3 * il) * ld;
             for (int p = 0; p < 3; p++)
                 for (int qq = 0; qq < 3; qq++)
-                    for (int k = 0; k < 3; k++) U[base + p * ld + 3 * (long long)j[k] + qq] += out[(p * 3 + qq) * 3 + k];
+                    for (int k = 0; k < 3; k++) fout[k * 9 + p * 3 + qq] = out[(p * 3 + qq) * 3 + k];
         }
+        // sum over the group's lanes in lane order (the leader's result is used)
+        for (int e = 0; e < NE; e++) {
+            double sx = fout[e].x, sy = fout[e].y;
+            for (int q = 1; q < GS; q++) sx += __shfl(fout[e].x, lane + q), sy += __shfl(fout[e].y, lane + q);
+            fout[e] = cd{sx, sy};
+        }
+        if (active && t == 0) {
+            for (int e = 0; e < NE; e++) fsum[slot][e] = fout[e];
+            for (int k = 0; k < 3; k++) fj[slot][k] = j[k];
+        }
+        __syncthreads();
+        // add the batch's faces into the row in face order; each lane owns distinct output entries
+        if (KIND == 1) {
+            if (lane < 36) {
+                cd *o = lane < 9 ? Bt + (long long)l * nv3 * 3 + (long long)(3 * i) * 3 + lane
+                                 : Ba + (long long)l * nv3 * 9 + (long long)(3 * i) * 9 + (lane - 9);
+                for (int s2 = 0; s2 < nb; s2++) *o += fsum[s2][lane];
+            }
+        } else if (lane < 9) {
+            const int a2 = lane / 3, b = lane % 3;
+            cd *o = U + (long long)l * lstride + (long long)(3 * il + a2) * ld + b;
+            for (int s2 = 0; s2 < nb; s2++)
+                for (int k = 0; k < 3; k++) o[3 * (long long)fj[s2][k]] += fsum[s2][k * 9 + lane];
+        }
+        __syncthreads();
     }
 }
 
@@ -890,8 +922,9 @@ hipError_t launch_points_b(const Geom &g, const double4 *quad, int npts, const F
 hipError_t launch_colloc_singular(int kind, const Geom &g, const double2 *line, int nline, const Freq *freqs, int l0, int nl,
                                   const Mat &m, int row0, int nrows, cd *U, long long ld, long long lstride, cd *Bt,
                                   cd *Ba, hipStream_t st) {
-    const int n = nrows * nl;
-    dim3 grid((n + 63) / 64);
+    const long long n = (long long)nrows * nl;
+    if (n <= 0 || n > 0x7fffffffLL) return hipErrorInvalidValue;
+    dim3 grid((unsigned)n);  // one wavefront per (row, frequency)
     if (kind == 0)
         hipLaunchKernelGGL((k_colloc_singular<0>), grid, dim3(64), 0, st, g, line, nline, freqs, l0, nl, m, row0, nrows, U,
                            ld, lstride, Bt, Ba, 3LL * g.nv);
