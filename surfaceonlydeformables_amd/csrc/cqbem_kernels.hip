@@ -465,19 +465,26 @@ __global__ void __launch_bounds__(256) k_colloc_b(Geom g, const double4 *__restr
 // ------------------------------------------------------------------------
 constexpr int kSingMaxFaces = 16;  // singular faces per batch of one (row, frequency) wavefront
 
+// Sum of v over lanes lane .. lane + gs - 1, in lane order (the group leader's value is the group's sum).
+__device__ inline cd group_sum(cd v, int lane, int gs) {
+    double sx = v.x, sy = v.y;
+    for (int q = 1; q < gs; q++) sx += __shfl(v.x, lane + q), sy += __shfl(v.y, lane + q);
+    return cd{sx, sy};
+}
+
 __device__ inline void unit3(double v[3]) {
     const double n = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     if (n > 0) v[0] /= n, v[1] /= n, v[2] /= n;
 }
 
 template <int KIND>
-__global__ void __launch_bounds__(64) k_colloc_singular(Geom g, const double2 *__restrict__ line, int nline,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_colloc_singular(Geom g, const double2 *__restrict__ line, int nline,
                                                          const Freq *__restrict__ freqs, int l0, int nl, Mat m, int row0,
                                                          int nrows, cd *__restrict__ U, long long ld, long long lstride,
                                                          cd *__restrict__ Bt, cd *__restrict__ Ba, long long nv3) {
     constexpr int NE = KIND == 1 ? 36 : 27;  // per face: B 9 + 27 entries; G/H 3 corners x 3 x 3
     __shared__ cd fsum[kSingMaxFaces][NE];
-    __shared__ int fj[kSingMaxFaces][3];
+    __shared__ int fj[kSingMaxFaces][3], umap[kSingMaxFaces][3], ucol[3 * kSingMaxFaces + 1];
     const int il = blockIdx.x / nl, l = blockIdx.x % nl;
     const int i = row0 + il;
     const Freq f = freqs[l0 + l];
@@ -497,7 +504,7 @@ __global__ void __launch_bounds__(64) k_colloc_singular(Geom g, const double2 *_
             y1[k] = g.V[3 * j[0] + k], y2[k] = g.V[3 * j[1] + k], y3[k] = g.V[3 * j[2] + k], n[k] = g.N[3 * fc + k];
         double v1[3], v2[3];
         for (int k = 0; k < 3; k++) v1[k] = y2[k] - y1[k], v2[k] = y3[k] - y1[k];
-        cd fout[NE];  // this face's contribution (the group leader's after the reduction)
+        const bool leader = active && t == 0;  // holds the face's sums after group_sum
         if (KIND == 0 || KIND == 1) {
             // polar frame around y1
             double u1[3] = {v1[0], v1[1], v1[2]}, u2[3] = {v2[0], v2[1], v2[2]};
@@ -585,28 +592,37 @@ __global__ void __launch_bounds__(64) k_colloc_singular(Geom g, const double2 *_
                 }
             }
             if (KIND == 0) {
-                for (int k = 0; k < 3; k++)
-                    for (int a2 = 0; a2 < 3; a2++)
-                        for (int b = 0; b < 3; b++) {
-                            cd v = ga[k][sidx(a2, b)];
-                            if (a2 == b) v += gb[k];
-                            fout[k * 9 + a2 * 3 + b] = v;
-                        }
+                for (int k = 0; k < 3; k++) {
+                    gb[k] = group_sum(gb[k], lane, GS);
+                    for (int e2 = 0; e2 < 6; e2++) ga[k][e2] = group_sum(ga[k][e2], lane, GS);
+                }
+                if (leader)
+                    for (int k = 0; k < 3; k++)
+                        for (int a2 = 0; a2 < 3; a2++)
+                            for (int b = 0; b < 3; b++) {
+                                cd v = ga[k][sidx(a2, b)];
+                                if (a2 == b) v += gb[k];
+                                fsum[slot][k * 9 + a2 * 3 + b] = v;
+                            }
             } else {
+                for (int k = 0; k < 3; k++) mt[k] = group_sum(mt[k], lane, GS);
+                for (int k = 0; k < 6; k++) M[k] = group_sum(M[k], lane, GS);
+                m2 = group_sum(m2, lane, GS), S = group_sum(S, lane, GS), S3 = group_sum(S3, lane, GS);
+                if (leader)
                 for (int a2 = 0; a2 < 3; a2++)
                     for (int b = 0; b < 3; b++) {
                         cd v = {0, 0};
                         fma_cr(v, mt[a2], n[b]);
                         fma_cr(v, mt[b], n[a2]);
                         if (a2 == b) v += m2;
-                        fout[a2 * 3 + b] = v;
+                        fsum[slot][a2 * 3 + b] = v;
                         for (int k = 0; k < 3; k++) {
                             cd u = {0, 0};
                             fma_cr(u, M[sidx(a2, k)], n[b]);
                             fma_cr(u, M[sidx(b, k)], n[a2]);
                             fma_cr(u, S, (b == k ? n[a2] : 0.0) + (a2 == k ? n[b] : 0.0));
                             if (a2 == b) fma_cr(u, S3, n[k]);
-                            fout[9 + a2 * 9 + b * 3 + k] = u;
+                            fsum[slot][9 + a2 * 9 + b * 3 + k] = u;
                         }
                     }
             }
@@ -662,33 +678,48 @@ __global__ void __launch_bounds__(64) k_colloc_singular(Geom g, const double2 *_
                         }
                     }
             }
-            for (int p = 0; p < 3; p++)
-                for (int qq = 0; qq < 3; qq++)
-                    for (int k = 0; k < 3; k++) fout[k * 9 + p * 3 + qq] = out[(p * 3 + qq) * 3 + k];
+            for (int e = 0; e < 27; e++) out[e] = group_sum(out[e], lane, GS);
+            if (leader)
+                for (int p = 0; p < 3; p++)
+                    for (int qq = 0; qq < 3; qq++)
+                        for (int k = 0; k < 3; k++) fsum[slot][k * 9 + p * 3 + qq] = out[(p * 3 + qq) * 3 + k];
         }
-        // sum over the group's lanes in lane order (the leader's result is used)
-        for (int e = 0; e < NE; e++) {
-            double sx = fout[e].x, sy = fout[e].y;
-            for (int q = 1; q < GS; q++) sx += __shfl(fout[e].x, lane + q), sy += __shfl(fout[e].y, lane + q);
-            fout[e] = cd{sx, sy};
-        }
-        if (active && t == 0) {
-            for (int e = 0; e < NE; e++) fsum[slot][e] = fout[e];
+        if (leader)
             for (int k = 0; k < 3; k++) fj[slot][k] = j[k];
-        }
         __syncthreads();
-        // add the batch's faces into the row in face order; each lane owns distinct output entries
+        // add the batch's faces into the row: each output entry sums its faces in face order, then one
+        // read-modify-write per entry (no dependent chains through memory)
         if (KIND == 1) {
             if (lane < 36) {
+                cd sum = fsum[0][lane];
+                for (int s2 = 1; s2 < nb; s2++) sum += fsum[s2][lane];
                 cd *o = lane < 9 ? Bt + (long long)l * nv3 * 3 + (long long)(3 * i) * 3 + lane
                                  : Ba + (long long)l * nv3 * 9 + (long long)(3 * i) * 9 + (lane - 9);
-                for (int s2 = 0; s2 < nb; s2++) *o += fsum[s2][lane];
+                *o += sum;
             }
-        } else if (lane < 9) {
-            const int a2 = lane / 3, b = lane % 3;
-            cd *o = U + (long long)l * lstride + (long long)(3 * il + a2) * ld + b;
-            for (int s2 = 0; s2 < nb; s2++)
-                for (int k = 0; k < 3; k++) o[3 * (long long)fj[s2][k]] += fsum[s2][k * 9 + lane];
+        } else {
+            // distinct columns of the batch in order of first appearance
+            if (lane == 0) {
+                int nu = 0;
+                for (int s2 = 0; s2 < nb; s2++)
+                    for (int k = 0; k < 3; k++) {
+                        int u = 0;
+                        while (u < nu && ucol[u] != fj[s2][k]) u++;
+                        if (u == nu) ucol[nu++] = fj[s2][k];
+                        umap[s2][k] = u;
+                    }
+                ucol[3 * kSingMaxFaces] = nu;
+            }
+            __syncthreads();
+            const int nu = ucol[3 * kSingMaxFaces];
+            for (int item = lane; item < 9 * nu; item += 64) {
+                const int u = item / 9, e = item % 9, a2 = e / 3, b = e % 3;
+                cd sum = {0, 0};
+                for (int s2 = 0; s2 < nb; s2++)
+                    for (int k = 0; k < 3; k++)
+                        if (umap[s2][k] == u) sum += fsum[s2][k * 9 + e];
+                U[(long long)l * lstride + (long long)(3 * il + a2) * ld + 3 * (long long)ucol[u] + b] += sum;
+            }
         }
         __syncthreads();
     }
