@@ -357,6 +357,38 @@ __device__ __forceinline__ void sing_eval(const double x[3], const double y[3], 
     }
 }
 
+// Point r of the Sauter-Schwab region maps (integrator_galerkin.hpp:42-212) at the 4D rule point
+// (e1, e2, e3, xi) with rule weight w: test barycentric coordinates (a0, a1), source (b0, b1), weight wt.
+template <int KIND>
+__device__ __forceinline__ void region_point(int r, double e1, double e2, double e3, double xi, double w, double &a0,
+                                             double &a1, double &b0, double &b1, double &wt) {
+    if (KIND == 1) {
+        wt = w * xi * xi * xi * e1 * e1 * e2;
+        switch (r) {
+        case 0: a0 = xi, a1 = xi * (1.0 - e1 + e1 * e2), b0 = xi * (1.0 - e1 * e2 * e3), b1 = xi * (1.0 - e1); break;
+        case 1: a0 = xi * (1.0 - e1 * e2 * e3), a1 = xi * (1.0 - e1), b0 = xi, b1 = xi * (1.0 - e1 + e1 * e2); break;
+        case 2: a0 = xi, a1 = xi * e1 * (1.0 - e2 + e2 * e3), b0 = xi * (1.0 - e1 * e2), b1 = xi * e1 * (1.0 - e2); break;
+        case 3: a0 = xi * (1.0 - e1 * e2), a1 = xi * e1 * (1.0 - e2), b0 = xi, b1 = xi * e1 * (1.0 - e2 + e2 * e3); break;
+        case 4: a0 = xi * (1.0 - e1 * e2 * e3), a1 = xi * e1 * (1.0 - e2 * e3), b0 = xi, b1 = xi * e1 * (1.0 - e2); break;
+        default: a0 = xi, a1 = xi * e1 * (1.0 - e2), b0 = xi * (1.0 - e1 * e2 * e3), b1 = xi * e1 * (1.0 - e2 * e3); break;
+        }
+    } else if (KIND == 2) {
+        const double W = w * xi * xi * xi * e1 * e1;
+        wt = r == 0 ? W : W * e2;
+        switch (r) {
+        case 0: a0 = xi, a1 = xi * e1 * e3, b0 = xi * (1.0 - e1 * e2), b1 = xi * e1 * (1.0 - e2); break;
+        case 1: a0 = xi, a1 = xi * e1, b0 = xi * (1.0 - e1 * e2 * e3), b1 = xi * e1 * e2 * (1.0 - e3); break;
+        case 2: a0 = xi * (1.0 - e1 * e2), a1 = xi * e1 * (1.0 - e2), b0 = xi, b1 = xi * e1 * e2 * e3; break;
+        case 3: a0 = xi * (1.0 - e1 * e2 * e3), a1 = xi * e1 * e2 * (1.0 - e3), b0 = xi, b1 = xi * e1; break;
+        default: a0 = xi * (1.0 - e1 * e2 * e3), a1 = xi * e1 * (1.0 - e2 * e3), b0 = xi, b1 = xi * e1 * e2; break;
+        }
+    } else {
+        wt = w * xi * xi * xi * e2;
+        if (r == 0) a0 = xi, a1 = xi * e1, b0 = xi * e2, b1 = xi * e2 * e3;
+        else a0 = xi * e2, a1 = xi * e2 * e3, b0 = xi, b1 = xi * e1;
+    }
+}
+
 template <int MODE, int KIND>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) k_gal_sing(Geom g, const Pair *__restrict__ pairs, int pair0,
                                                   const double2 *__restrict__ line, int nline,
@@ -396,41 +428,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
         const int ia = p / (nline * nline * nline), ib = (p / (nline * nline)) % nline, ic = (p / nline) % nline, id = p % nline;
         const double e1 = line[ia].x, e2 = line[ib].x, e3 = line[ic].x, xi = line[id].x;
         const double w = line[ia].y * line[ib].y * line[ic].y * line[id].y;
-        // region points (x^, y^) and weights; evaluated in one loop so the accumulators stay in registers
         constexpr int NR = KIND == 1 ? 6 : (KIND == 2 ? 5 : 2);
-        double ra0[NR], ra1[NR], rb0[NR], rb1[NR], rw[NR];
-        int nr_ = 0;
-        auto ev = [&](double a0, double a1, double b0, double b1, double wt) {
-            ra0[nr_] = a0, ra1[nr_] = a1, rb0[nr_] = b0, rb1[nr_] = b1, rw[nr_] = wt, nr_++;
-        };
-        if (KIND == 1) {
-            const double W = w * xi * xi * xi * e1 * e1 * e2;
-            ev(xi, xi * (1.0 - e1 + e1 * e2), xi * (1.0 - e1 * e2 * e3), xi * (1.0 - e1), W);
-            ev(xi * (1.0 - e1 * e2 * e3), xi * (1.0 - e1), xi, xi * (1.0 - e1 + e1 * e2), W);
-            ev(xi, xi * e1 * (1.0 - e2 + e2 * e3), xi * (1.0 - e1 * e2), xi * e1 * (1.0 - e2), W);
-            ev(xi * (1.0 - e1 * e2), xi * e1 * (1.0 - e2), xi, xi * e1 * (1.0 - e2 + e2 * e3), W);
-            ev(xi * (1.0 - e1 * e2 * e3), xi * e1 * (1.0 - e2 * e3), xi, xi * e1 * (1.0 - e2), W);
-            ev(xi, xi * e1 * (1.0 - e2), xi * (1.0 - e1 * e2 * e3), xi * e1 * (1.0 - e2 * e3), W);
-        } else if (KIND == 2) {
-            const double W = w * xi * xi * xi * e1 * e1;
-            ev(xi, xi * e1 * e3, xi * (1.0 - e1 * e2), xi * e1 * (1.0 - e2), W);
-            ev(xi, xi * e1, xi * (1.0 - e1 * e2 * e3), xi * e1 * e2 * (1.0 - e3), W * e2);
-            ev(xi * (1.0 - e1 * e2), xi * e1 * (1.0 - e2), xi, xi * e1 * e2 * e3, W * e2);
-            ev(xi * (1.0 - e1 * e2 * e3), xi * e1 * e2 * (1.0 - e3), xi, xi * e1, W * e2);
-            ev(xi * (1.0 - e1 * e2 * e3), xi * e1 * (1.0 - e2 * e3), xi, xi * e1 * e2, W * e2);
-        } else {
-            const double W = w * xi * xi * xi * e2;
-            ev(xi, xi * e1, xi * e2, xi * e2 * e3, W);
-            ev(xi * e2, xi * e2 * e3, xi, xi * e1, W);
-        }
 #pragma unroll 1
         for (int r = 0; r < NR; r++) {
+            double a0, a1, b0, b1, wt;
+            region_point<KIND>(r, e1, e2, e3, xi, w, a0, a1, b0, b1, wt);
             double x[3], y[3];
             #pragma unroll
-            for (int k = 0; k < 3; k++)
-                x[k] = px[k] + dx0[k] * ra0[r] + dx1[k] * ra1[r], y[k] = px[k] + dy0[k] * rb0[r] + dy1[k] * rb1[r];
-            const double Lx[3] = {1.0 - ra0[r], ra0[r] - ra1[r], ra1[r]}, Ly[3] = {1.0 - rb0[r], rb0[r] - rb1[r], rb1[r]};
-            sing_eval<MODE>(x, y, Lx, Ly, rw[r], n, f, m, acc);
+            for (int k = 0; k < 3; k++) x[k] = px[k] + dx0[k] * a0 + dx1[k] * a1, y[k] = px[k] + dy0[k] * b0 + dy1[k] * b1;
+            const double Lx[3] = {1.0 - a0, a0 - a1, a1}, Ly[3] = {1.0 - b0, b0 - b1, b1};
+            sing_eval<MODE>(x, y, Lx, Ly, wt, n, f, m, acc);
         }
     }
     // wave reduction; lane 0 expands and writes
@@ -489,6 +496,176 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
                     }
                 }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Touching pairs, G and H, quad form.  The 9 (test corner i, source corner j) basis pairs of a point
+// pair need 63 (G) or 81 (H) complex accumulators when one lane takes all of them, which spills.  Here
+// the four lanes of a quad each evaluate their own point, pass the point's kernel data around the quad
+// (three DPP rotations), and each lane accumulates only its basis pairs c = q, q + 4, q + 8 over all four
+// points.  The quads' partial sums are added across the wavefront at the end.
+// ---------------------------------------------------------------------------
+template <int D> __device__ inline double quad_rot(double v) {  // lane q receives lane (q + D) mod 4's value
+    constexpr int sel = ((0 + D) & 3) | (((1 + D) & 3) << 2) | (((2 + D) & 3) << 4) | (((3 + D) & 3) << 6);
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, sel, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), sel, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// Kernel data of one point pair.  G: alpha/r, beta/r, the 6 products rh_a rh_b; H: the 9 entries of p_hat.
+// Then Lx[3], Ly[3] and the weight.
+template <int MODE> struct QuadK {
+    static constexpr int NK = MODE == 0 ? 10 : 18;
+    double v[NK + 7];
+};
+
+template <int MODE>
+__device__ __forceinline__ void quad_point(const double x[3], const double y[3], const double Lx[3], const double Ly[3], double wt,
+                                           const double n[3], const Freq &f, const Mat &m, QuadK<MODE> &K) {
+    const double r0 = y[0] - x[0], r1 = y[1] - x[1], r2 = y[2] - x[2];
+    const double d2 = r0 * r0 + r1 * r1 + r2 * r2;
+    const double ir = rsqrt_nr(d2), r = d2 * ir, ir2 = ir * ir;
+    const double h[3] = {r0 * ir, r1 * ir, r2 * ir};
+    const Expo E = expo(r, f);
+    if (MODE == 0) {
+        cd al, be;
+        g_coef(E, ir, ir2, m, al, be);
+        al = al * ir, be = be * ir;
+        K.v[0] = al.x, K.v[1] = al.y, K.v[2] = be.x, K.v[3] = be.y;
+        K.v[4] = h[0] * h[0], K.v[5] = h[1] * h[1], K.v[6] = h[2] * h[2];
+        K.v[7] = h[0] * h[1], K.v[8] = h[0] * h[2], K.v[9] = h[1] * h[2];
+    } else {
+        cd c1, c2, c4;
+        h_coef(E, ir, ir2, f, m, c1, c2, c4);
+        const double rn = h[0] * n[0] + h[1] * n[1] + h[2] * n[2];
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                cd v = c1 * (h[p] * n[b]);
+                fma_cr(v, c2, n[p] * h[b] + (p == b ? rn : 0.0));
+                fma_cr(v, c4, rn * h[p] * h[b]);
+                K.v[2 * (p * 3 + b)] = v.x, K.v[2 * (p * 3 + b) + 1] = v.y;
+            }
+    }
+    constexpr int o = QuadK<MODE>::NK;
+#pragma unroll
+    for (int k = 0; k < 3; k++) K.v[o + k] = Lx[k], K.v[o + 3 + k] = Ly[k];
+    K.v[o + 6] = wt;
+}
+
+__device__ __forceinline__ double pick3(const double *a, int i) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
+
+template <int MODE>
+__device__ __forceinline__ void quad_acc(const QuadK<MODE> &K, const int ci[3], const int cj[3], const double cv[3],
+                                         cd (*acc)[MODE == 0 ? 7 : 9]) {
+    constexpr int o = QuadK<MODE>::NK;
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+        const double w = K.v[o + 6] * pick3(K.v + o, ci[s]) * pick3(K.v + o + 3, cj[s]) * cv[s];
+        if (MODE == 0) {
+            const cd aw = cd{K.v[0], K.v[1]} * w;
+#pragma unroll
+            for (int e = 0; e < 6; e++) fma_cr(acc[s][e], aw, K.v[4 + e]);
+            fma_cr(acc[s][6], cd{K.v[2], K.v[3]}, w);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 9; e++) fma_cr(acc[s][e], cd{K.v[2 * e], K.v[2 * e + 1]}, w);
+        }
+    }
+}
+
+template <int MODE, int KIND>
+__global__ void __launch_bounds__(64) k_gal_sing_quad(Geom g, const Pair *__restrict__ pairs, int pair0,
+                                                      const double2 *__restrict__ line, int nline,
+                                                      const Freq *__restrict__ freqs, Mat m, cd *__restrict__ out) {
+    constexpr int NE = MODE == 0 ? 7 : 9;
+    const int pi = pair0 + blockIdx.x, l = blockIdx.y;
+    const Pair P = pairs[pi];
+    const Freq f = freqs[l];
+    const double n[3] = {g.N[3 * P.f2], g.N[3 * P.f2 + 1], g.N[3 * P.f2 + 2]};
+    double px[3], dx0[3], dx1[3], dy0[3], dy1[3], jac;
+    {
+        const double *A = g.V + 3 * P.j1[0], *B = g.V + 3 * P.j1[1], *Cc = g.V + 3 * P.j1[2];
+        for (int k = 0; k < 3; k++) px[k] = A[k];
+        if (KIND == 1) {
+            for (int k = 0; k < 3; k++) dx0[k] = dy0[k] = B[k] - A[k], dx1[k] = dy1[k] = Cc[k] - B[k];
+        } else if (KIND == 2) {
+            const double *Y2 = g.V + 3 * P.j2[2];
+            for (int k = 0; k < 3; k++) dx0[k] = dy0[k] = B[k] - A[k], dx1[k] = Cc[k] - B[k], dy1[k] = Y2[k] - B[k];
+        } else {
+            const double *Y1 = g.V + 3 * P.j2[1], *Y2 = g.V + 3 * P.j2[2];
+            for (int k = 0; k < 3; k++) dx0[k] = B[k] - A[k], dx1[k] = Cc[k] - B[k], dy0[k] = Y1[k] - A[k], dy1[k] = Y2[k] - Y1[k];
+        }
+        const double cx[3] = {dx0[1] * dx1[2] - dx0[2] * dx1[1], dx0[2] * dx1[0] - dx0[0] * dx1[2], dx0[0] * dx1[1] - dx0[1] * dx1[0]};
+        const double cy[3] = {dy0[1] * dy1[2] - dy0[2] * dy1[1], dy0[2] * dy1[0] - dy0[0] * dy1[2], dy0[0] * dy1[1] - dy0[1] * dy1[0]};
+        jac = sqrt(cx[0] * cx[0] + cx[1] * cx[1] + cx[2] * cx[2]) * sqrt(cy[0] * cy[0] + cy[1] * cy[1] + cy[2] * cy[2]);
+    }
+    // this lane's basis pairs c = q + 4 s (c = 9..11 are empty slots)
+    const int q = threadIdx.x & 3;
+    int ci[3], cj[3];
+    double cv[3];
+    for (int s = 0; s < 3; s++) {
+        const int c = q + 4 * s;
+        cv[s] = c < 9 ? 1.0 : 0.0;
+        ci[s] = (c < 9 ? c : 0) / 3, cj[s] = (c < 9 ? c : 0) % 3;
+    }
+    cd acc[3][NE];
+    for (int s = 0; s < 3; s++)
+        for (int e = 0; e < NE; e++) acc[s][e] = cd{0, 0};
+    const int n4 = nline * nline * nline * nline;
+    const int iters = (n4 + 63) / 64;  // every lane runs the same count: the quad exchanges need all four
+    for (int itr = 0; itr < iters; ++itr) {
+        int p = threadIdx.x + 64 * itr;
+        const double pw = p < n4 ? 1.0 : 0.0;  // past the end: a repeated point with weight 0
+        if (p >= n4) p = n4 - 1;
+        const int ia = p / (nline * nline * nline), ib = (p / (nline * nline)) % nline, ic = (p / nline) % nline, id = p % nline;
+        const double e1 = line[ia].x, e2 = line[ib].x, e3 = line[ic].x, xi = line[id].x;
+        const double w = line[ia].y * line[ib].y * line[ic].y * line[id].y * pw;
+        constexpr int NR = KIND == 1 ? 6 : (KIND == 2 ? 5 : 2);
+#pragma unroll 1
+        for (int r = 0; r < NR; r++) {
+            double a0, a1, b0, b1, wt;
+            region_point<KIND>(r, e1, e2, e3, xi, w, a0, a1, b0, b1, wt);
+            double x[3], y[3];
+            for (int k = 0; k < 3; k++) x[k] = px[k] + dx0[k] * a0 + dx1[k] * a1, y[k] = px[k] + dy0[k] * b0 + dy1[k] * b1;
+            const double Lx[3] = {1.0 - a0, a0 - a1, a1}, Ly[3] = {1.0 - b0, b0 - b1, b1};
+            QuadK<MODE> K;
+            quad_point<MODE>(x, y, Lx, Ly, wt, n, f, m, K);
+            quad_acc<MODE>(K, ci, cj, cv, acc);
+#pragma unroll 1
+            for (int rot = 0; rot < 3; rot++) {  // the next lane's point, then the one after, then the last
+#pragma unroll
+                for (int k = 0; k < QuadK<MODE>::NK + 7; k++) K.v[k] = quad_rot<1>(K.v[k]);
+                quad_acc<MODE>(K, ci, cj, cv, acc);
+            }
+        }
+    }
+    // add the 16 quads' partial sums (lanes with the same q)
+    for (int s = 0; s < 3; s++)
+        for (int e = 0; e < NE; e++) {
+            double vx = acc[s][e].x, vy = acc[s][e].y;
+            for (int o = 4; o < 64; o <<= 1) vx += __shfl_xor(vx, o, 64), vy += __shfl_xor(vy, o, 64);
+            acc[s][e] = cd{vx * jac, vy * jac};
+        }
+    if (threadIdx.x >= 4) return;
+    cd *ob = out + ((long long)pi * gridDim.y + l) * kPairStride;
+    for (int s = 0; s < 3; s++) {
+        const int c = q + 4 * s;
+        if (c >= 9) continue;
+        const int i = c / 3, j = c % 3;
+        for (int p = 0; p < 3; p++)
+            for (int b = 0; b < 3; b++) {
+                if (MODE == 0) {
+                    cd v = acc[s][sidx(p, b)];
+                    if (p == b) v += acc[s][6];
+                    ob[kPairG + (3 * i + p) * 9 + 3 * j + b] = v;
+                } else {
+                    ob[kPairH + (3 * i + p) * 9 + 3 * j + b] = acc[s][p * 3 + b] * kInv4Pi;
+                }
+            }
     }
 }
 
@@ -827,8 +1004,19 @@ void run_galerkin(const GalerkinJob &job, const int *F, const int *vmap, cd *U, 
     if (end[KIND - 1] > beg[KIND - 1])                                                                                \
         hipLaunchKernelGGL((k_gal_sing<MODE, KIND>), dim3((unsigned)(end[KIND - 1] - beg[KIND - 1]), nl), dim3(64), 0, st, g, \
                            dp, beg[KIND - 1], job.line, job.nline, job.freqs, job.mat, scratch);
-        if (U) { CQBEM_SING(0, 1) CQBEM_SING(0, 2) CQBEM_SING(0, 3) }
-        if (P) { CQBEM_SING(1, 1) CQBEM_SING(1, 2) CQBEM_SING(1, 3) }
+        const char *sv = std::getenv("CQBEM_GALERKIN_SINGULAR");
+        if (!(sv && !std::strcmp(sv, "lane"))) {
+#define CQBEM_SINGQ(MODE, KIND)                                                                                       \
+    if (end[KIND - 1] > beg[KIND - 1])                                                                                \
+        hipLaunchKernelGGL((k_gal_sing_quad<MODE, KIND>), dim3((unsigned)(end[KIND - 1] - beg[KIND - 1]), nl), dim3(64), 0, \
+                           st, g, dp, beg[KIND - 1], job.line, job.nline, job.freqs, job.mat, scratch);
+            if (U) { CQBEM_SINGQ(0, 1) CQBEM_SINGQ(0, 2) CQBEM_SINGQ(0, 3) }
+            if (P) { CQBEM_SINGQ(1, 1) CQBEM_SINGQ(1, 2) CQBEM_SINGQ(1, 3) }
+#undef CQBEM_SINGQ
+        } else {
+            if (U) { CQBEM_SING(0, 1) CQBEM_SING(0, 2) CQBEM_SING(0, 3) }
+            if (P) { CQBEM_SING(1, 1) CQBEM_SING(1, 2) CQBEM_SING(1, 3) }
+        }
         if (Bt && Ba) { CQBEM_SING(2, 1) CQBEM_SING(2, 2) CQBEM_SING(2, 3) }
 #undef CQBEM_SING
         check(hipGetLastError(), "k_gal_sing");

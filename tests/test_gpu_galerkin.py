@@ -4,7 +4,8 @@ case runs the shipped order 5 x subdivision 2 rule at a single frequency.
 
 Two non-singular decompositions are checked against the same bounds: "points" (default: collocation
 rows at every test quadrature point, then projection onto the test basis) and "star" (one lane per
-test face and source vertex star; CQBEM_GALERKIN_KERNEL=star)."""
+test face and source vertex star; CQBEM_GALERKIN_KERNEL=star).  Touching pairs: the quad kernels for G
+and H (default) and the one-lane-per-point kernels (CQBEM_GALERKIN_SINGULAR=lane)."""
 import numpy as np
 import pytest
 
@@ -61,9 +62,13 @@ def test_galerkin_laplace_parity(ctx, oracle, name, order, sub, l, variant, monk
     assert rel(g["Ba"], o["Ba"]) < 1e-13
 
 
-@pytest.mark.parametrize("name,L,bdf,variant", [("ico1", 16, 2, "points"), ("cube2", 9, 1, "points"), ("ico1", 16, 2, "star")])
+@pytest.mark.parametrize("name,L,bdf,variant", [("ico1", 16, 2, "points"), ("cube2", 9, 1, "points"), ("ico1", 16, 2, "star"),
+                                                ("cube2", 9, 1, "points/lane")])
 def test_galerkin_time_domain_parity(ctx, oracle, name, L, bdf, variant, monkeypatch):
-    monkeypatch.setenv("CQBEM_GALERKIN_KERNEL", variant)
+    """Touching pairs run the quad kernels (G, H) by default; "/lane" selects the one-lane-per-point kernels
+    (CQBEM_GALERKIN_SINGULAR=lane)."""
+    monkeypatch.setenv("CQBEM_GALERKIN_KERNEL", variant.split("/")[0])
+    monkeypatch.setenv("CQBEM_GALERKIN_SINGULAR", "lane" if variant.endswith("/lane") else "quad")
     V, F = mesh(name)
     _, c1, c2 = M.material_speeds(1e6, 0.35, RHO)
     dt = 5e-4
