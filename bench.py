@@ -315,7 +315,7 @@ def roofline(gh_ms, gh_evals, gh_launches, with_traffic):
     """VALU roofline of the dominant kernel from the library's HIP-event timers; the HBM traffic per launch is
     the committed counter measurement of the icosphere-L4 shape."""
     achieved = FLOPS_PER_EVAL_GH * gh_evals / (gh_ms * 1e-3) / 1e12
-    roof = {"bound": "valu", "kernel": "k_colloc_gh_pair<true,true>", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+    roof = {"bound": "valu", "kernel": "k_colloc_gh_pair<true,true,false>", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
             "launch_ms": gh_ms / max(gh_launches, 1),
             "flops_per_launch": FLOPS_PER_EVAL_GH * gh_evals / max(gh_launches, 1)}
