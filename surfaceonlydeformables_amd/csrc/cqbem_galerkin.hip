@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "cqbem_galerkin.hpp"
+#include "cqbem_pool.hpp"
 
 namespace cqbem {
 
@@ -780,11 +781,11 @@ void check(hipError_t e, const char *what) {
 
 struct DevVec {
     void *p = nullptr;
-    ~DevVec() {
-        if (p) (void)hipFree(p);
-    }
+    bool pooled = false;
+    ~DevVec() { dev_free(p, pooled); }
+    void alloc(size_t n, const char *what) { check(dev_alloc(&p, std::max<size_t>(n, 1), &pooled), what); }
     template <class T> T *up(const std::vector<T> &v, hipStream_t st) {
-        check(hipMalloc(&p, std::max<size_t>(v.size(), 1) * sizeof(T)), "hipMalloc");
+        alloc(v.size() * sizeof(T), "device allocation");
         if (!v.empty()) check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
         return (T *)p;
     }
@@ -888,8 +889,8 @@ void run_galerkin(const GalerkinJob &job, const int *F, const int *vmap, cd *U, 
         const int npts = job.npts;
         const long long ntp = (long long)nf * npts;
         DevVec dtp, dpsi, dsG, dsH, dsBt, dsBa;
-        check(hipMalloc(&dtp.p, (size_t)ntp * sizeof(double4)), "hipMalloc test points");
-        check(hipMalloc(&dpsi.p, (size_t)ntp * 3 * sizeof(double)), "hipMalloc test weights");
+        dtp.alloc((size_t)ntp * sizeof(double4), "device allocation: test points");
+        dpsi.alloc((size_t)ntp * 3 * sizeof(double), "device allocation: test weights");
         double4 *tpts = (double4 *)dtp.p;
         double *psi = (double *)dpsi.p;
         hipLaunchKernelGGL(k_gal_points, dim3((unsigned)((ntp + 255) / 256)), dim3(256), 0, st, g, job.quad, npts, faces, nf,
@@ -897,11 +898,11 @@ void run_galerkin(const GalerkinJob &job, const int *F, const int *vmap, cd *U, 
         check(hipGetLastError(), "k_gal_points");
         const int cap = galerkin_chunk_faces(nv, nf, npts, nl, U != nullptr, P != nullptr, Bt && Ba, job.scratch_bytes);
         const long long cap_rows = 3LL * cap * npts, ls = cap_rows * n3;
-        if (U) check(hipMalloc(&dsG.p, (size_t)nl * ls * sizeof(cd)), "hipMalloc point rows G");
-        if (P) check(hipMalloc(&dsH.p, (size_t)nl * ls * sizeof(cd)), "hipMalloc point rows H");
+        if (U) dsG.alloc((size_t)nl * ls * sizeof(cd), "device allocation: point rows G");
+        if (P) dsH.alloc((size_t)nl * ls * sizeof(cd), "device allocation: point rows H");
         if (Bt && Ba) {
-            check(hipMalloc(&dsBt.p, (size_t)nl * cap_rows * 3 * sizeof(cd)), "hipMalloc point rows Bt");
-            check(hipMalloc(&dsBa.p, (size_t)nl * cap_rows * 9 * sizeof(cd)), "hipMalloc point rows Ba");
+            dsBt.alloc((size_t)nl * cap_rows * 3 * sizeof(cd), "device allocation: point rows Bt");
+            dsBa.alloc((size_t)nl * cap_rows * 9 * sizeof(cd), "device allocation: point rows Ba");
         }
         cd *sG = (cd *)dsG.p, *sH = (cd *)dsH.p, *sBt = (cd *)dsBt.p, *sBa = (cd *)dsBa.p;
         auto project = [&](const cd *src, long long ew, long long src_ls, const int *cf, int nfc, long long row0, cd *dst,
@@ -995,7 +996,7 @@ void run_galerkin(const GalerkinJob &job, const int *F, const int *vmap, cd *U, 
     const long long npairs = (long long)pairs.size();
     DevVec dpairs, dscratch, dblocks, dptr, ditems, dvptr, dvitems;
     const Pair *dp = dpairs.up(pairs, st);
-    check(hipMalloc(&dscratch.p, (size_t)npairs * nl * kPairStride * sizeof(cd)), "hipMalloc scratch");
+    dscratch.alloc((size_t)npairs * nl * kPairStride * sizeof(cd), "device allocation: touching-pair scratch");
     cd *scratch = (cd *)dscratch.p;
     {
         EvTimer t(st, &tm[3]);

@@ -25,6 +25,7 @@
 
 #include "../../include/cqbem.h"
 #include "cqbem_galerkin.hpp"
+#include "cqbem_pool.hpp"
 #include "cqbem_kernels.hpp"
 #include "cqbem_rules.hpp"
 
@@ -49,16 +50,15 @@ struct Err {
 struct DBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool pooled = false;  // from the running job's arena (returned with it)
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
-    ~DBuf() {
-        if (p) (void)hipFree(p);
-    }
+    ~DBuf() { dev_free(p, pooled); }
     void alloc(size_t n) {
-        if (p) (void)hipFree(p), p = nullptr;
+        dev_free(p, pooled), p = nullptr;
         bytes = n;
-        if (n) HIPCHK(hipMalloc(&p, n));
+        if (n) HIPCHK(dev_alloc(&p, n, &pooled));
     }
     template <class T> T *as() const { return reinterpret_cast<T *>(p); }
 };
@@ -142,6 +142,7 @@ struct cqbem_context {
     std::unordered_map<uint64_t, std::shared_ptr<JobRecord>> jobs;
     std::vector<hipStream_t> slots;
     std::vector<uint64_t> slot_job;
+    DevicePool pool;  // job buffers, reused across jobs
 };
 
 namespace {
@@ -457,6 +458,14 @@ uint64_t footprint(const cqbem_mesh *mesh, const cqbem_params *p, int outputs_on
 // Runs one job on stream `st`.  `slab_budget` bounds the Laplace-domain scratch.
 void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const cqbem_params *p, const cqbem_outputs *out,
              size_t slab_budget, double *tm, bool sync_outputs) {
+    // every device buffer of the job comes from the context's pool and goes back to it when the stream is done
+    // (CQBEM_POOL=0: plain hipMalloc / hipFree per buffer)
+    static const bool use_pool = [] {
+        const char *e = std::getenv("CQBEM_POOL");
+        return !(e && !std::strcmp(e, "0"));
+    }();
+    JobArena arena(&ctx->pool, st);
+    ArenaScope arena_scope(use_pool ? &arena : nullptr);
     const auto t_host0 = std::chrono::steady_clock::now();
     Prepared P = prepare(mesh, p);
     const int nv = P.nv, L = P.L, Lf = P.Lf;
@@ -708,7 +717,7 @@ int cqbem_assemble(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_param
         HIPCHK(hipSetDevice(ctx->device));
         size_t fr = 0, tot = 0;
         HIPCHK(hipMemGetInfo(&fr, &tot));
-        const uint64_t avail = std::min<uint64_t>(ctx->budget, (uint64_t)fr);
+        const uint64_t avail = std::min<uint64_t>(ctx->budget, (uint64_t)fr + ctx->pool.cached_bytes());
         // everything but the Laplace-domain scratch, then give the scratch what is left (capped)
         const uint64_t fixed = footprint(mesh, params, out->on_device, 0);
         if (fixed > avail) throw Err{CQBEM_ERR_NOMEM, "job does not fit the HBM budget"};

@@ -1,0 +1,124 @@
+// Device memory cache of a context.  hipMalloc of the multi-GB spectra costs time on every job, and hipFree
+// waits for the whole device, so concurrent jobs on other streams would hold each other up at every release.
+// A job takes its buffers from the context's pool through a JobArena and gives them back when its stream has
+// finished; blocks are reused by later jobs of similar sizes and released to HIP only when an allocation
+// fails (then the pool is emptied and the allocation retried) or the context is destroyed.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+namespace cqbem {
+
+class DevicePool {
+  public:
+    DevicePool() = default;
+    DevicePool(const DevicePool &) = delete;
+    DevicePool &operator=(const DevicePool &) = delete;
+    ~DevicePool() { trim(); }
+
+    // A block of at least n bytes (its size in *got); nullptr if HIP cannot provide one.
+    void *get(size_t n, size_t *got) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            auto it = free_.lower_bound(n);
+            if (it != free_.end() && it->first <= n + n / 8 + (2u << 20)) {  // little waste: reuse
+                void *p = it->second;
+                *got = it->first;
+                cached_ -= it->first;
+                free_.erase(it);
+                return p;
+            }
+        }
+        void *p = nullptr;
+        if (hipMalloc(&p, n) != hipSuccess) {
+            (void)hipGetLastError();
+            trim();
+            if (hipMalloc(&p, n) != hipSuccess) {
+                (void)hipGetLastError();
+                return nullptr;
+            }
+        }
+        *got = n;
+        return p;
+    }
+    // Returns an idle block (no work in flight uses it).
+    void put(void *p, size_t n) {
+        std::lock_guard<std::mutex> lk(mu_);
+        free_.emplace(n, p);
+        cached_ += n;
+    }
+    void trim() {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto &kv : free_) (void)hipFree(kv.second);
+        free_.clear();
+        cached_ = 0;
+    }
+    size_t cached_bytes() {
+        std::lock_guard<std::mutex> lk(mu_);
+        return cached_;
+    }
+
+  private:
+    std::mutex mu_;
+    std::multimap<size_t, void *> free_;
+    size_t cached_ = 0;
+};
+
+// The buffers of one job on one stream.  Destroyed after the job: synchronises the stream, then returns
+// every block to the pool.
+class JobArena {
+  public:
+    JobArena(DevicePool *pool, hipStream_t st) : pool_(pool), st_(st) {}
+    JobArena(const JobArena &) = delete;
+    JobArena &operator=(const JobArena &) = delete;
+    ~JobArena() {
+        (void)hipStreamSynchronize(st_);
+        for (auto &b : held_) pool_->put(b.first, b.second);
+    }
+    void *get(size_t n) {
+        size_t got = 0;
+        void *p = pool_->get(n, &got);
+        if (p) held_.emplace_back(p, got);
+        return p;
+    }
+
+  private:
+    DevicePool *pool_;
+    hipStream_t st_;
+    std::vector<std::pair<void *, size_t>> held_;
+};
+
+// The arena of the job running on this thread (set for the duration of a job), or null.
+inline JobArena *&current_arena() {
+    static thread_local JobArena *a = nullptr;
+    return a;
+}
+
+// Device allocation for job-scoped buffers: from the current arena when a job is running on this thread
+// (released with the arena), otherwise hipMalloc (released with hipFree).  *pooled tells the owner which.
+inline hipError_t dev_alloc(void **p, size_t n, bool *pooled) {
+    if (JobArena *a = current_arena()) {
+        *p = a->get(n);
+        *pooled = true;
+        return *p ? hipSuccess : hipErrorOutOfMemory;
+    }
+    *pooled = false;
+    return hipMalloc(p, n);
+}
+inline void dev_free(void *p, bool pooled) {
+    if (p && !pooled) (void)hipFree(p);
+}
+
+// Sets the current arena for a scope.
+struct ArenaScope {
+    JobArena *prev;
+    explicit ArenaScope(JobArena *a) : prev(current_arena()) { current_arena() = a; }
+    ~ArenaScope() { current_arena() = prev; }
+};
+
+}  // namespace cqbem
