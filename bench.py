@@ -91,7 +91,14 @@ def cpu_baseline(V, F, params, seconds_target=15.0):
     n3 = 3 * nv
     full = t_quad * (nv / rows) * (Lf / len(ls)) + t_tr * (2 * n3 * n3 / M_sample)
     entries = 2 * n3 * n3 * L
-    return {"value": entries / full, "unit": "entries/s", "cores": threads, "kind": "port",
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((l.split(":", 1)[1].strip() for l in fh if l.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"value": entries / full, "unit": "entries/s", "cores": threads, "kind": "port", "cpu_model": model,
+            "host_cpus": os.cpu_count(),
             "sample": (f"oracle (reference float/double typedefs), G+H+B_trans+B_angular Laplace rows 0..{rows - 1} "
                        f"of {nv} at frequencies {ls} of {Lf} ({t_quad:.1f} s) + inverse transform of "
                        f"{M_sample} entries ({t_tr:.1f} s), extrapolated to the full job: {full:.0f} s")}
