@@ -2,6 +2,7 @@
 // simulation keys as in the reference's data/*.json), runs init(), and writes G, H (negated, as the
 // object reads them) and B as raw little-endian doubles to <out>_{G,H,B}.bin, the matrices init()
 // derives for the time stepping to <out>_post_<name>.bin, and a JSON line with L, nv and timings.
+// Other modes (host only): --mesh scene.json (the prepared mesh), --cache-key scene.json, --dump doc.json.
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -30,6 +31,34 @@ int main(int argc, char **argv) {
                 const std::string key = cqbem::cache_key(cfg, &stripped);
                 std::cout << key << "\t" << stripped.dump() << "\n";
             }
+        } catch (const std::exception &e) {
+            std::cerr << e.what() << std::endl;
+            return 2;
+        }
+        return 0;
+    }
+    if (argc == 3 && std::string(argv[1]) == "--mesh") {
+        // --mesh scene.json: the prepared mesh (V, F, vmap) of the scene's "mesh" block, as one JSON line
+        std::ifstream in(argv[2]);
+        std::stringstream ss;
+        ss << in.rdbuf();
+        try {
+            const cqbem::Json doc = cqbem::Json::parse(ss.str());
+            std::vector<double> V;
+            std::vector<int32_t> F, vm;
+            std::string err;
+            if (!cqbem::prepare_mesh(doc["mesh"], V, F, vm, err)) {
+                std::cerr << err << std::endl;
+                return 1;
+            }
+            auto put = [](const char *name, const auto &xs, bool last) {
+                std::printf("\"%s\": [", name);
+                for (size_t k = 0; k < xs.size(); k++) std::printf(k ? ", %.17g" : "%.17g", (double)xs[k]);
+                std::printf(last ? "]" : "], ");
+            };
+            std::printf("{");
+            put("V", V, false), put("F", F, false), put("vmap", vm, true);
+            std::printf("}\n");
         } catch (const std::exception &e) {
             std::cerr << e.what() << std::endl;
             return 2;

@@ -492,6 +492,12 @@ bool load_obj(const std::string &path, std::vector<double> &V, std::vector<int32
         err = "empty mesh in " + path;
         return false;
     }
+    const int32_t nv = (int32_t)(V.size() / 3);
+    for (int32_t x : F)
+        if (x < 0 || x >= nv) {
+            err = "face index out of range in " + path;
+            return false;
+        }
     return true;
 }
 
@@ -521,26 +527,55 @@ bool ElastodynamicsCQBEMMatrixCollection::fail(const std::string &msg) {
     return false;
 }
 
+bool prepare_mesh(const Json &m, std::vector<double> &v, std::vector<int32_t> &f, std::vector<int32_t> &vmap,
+                  std::string &err) {
+    vmap.clear();
+    if (!load_obj(m["file"].string(), v, f, err)) return false;
+    // unused vertices dropped, the survivors keeping their relative order (lines 75-103)
+    {
+        const int64_t nv0 = (int64_t)(v.size() / 3);
+        std::vector<int32_t> used(nv0, -1);
+        for (int32_t x : f) used[x] = 0;
+        int32_t nv1 = 0;
+        for (int64_t i = 0; i < nv0; i++)
+            if (used[i] == 0) used[i] = nv1++;
+        if (nv1 != nv0) {
+            std::vector<double> v1(3 * (size_t)nv1);
+            for (int64_t i = 0; i < nv0; i++)
+                if (used[i] >= 0)
+                    for (int k = 0; k < 3; k++) v1[3 * (size_t)used[i] + k] = v[3 * i + k];
+            for (int32_t &x : f) x = used[x];
+            v.swap(v1);
+        }
+    }
+    if (m.contains("scale"))
+        for (double &x : v) x *= m["scale"].number();
+    if (m.contains("subdivision_level") && m["subdivision_level"].number() > 0)
+        return err = "mesh.subdivision_level is not supported: subdivide the mesh before loading it", false;
+    if (m.contains("decimation_scale") && m["decimation_scale"].number() < 1.0)
+        return err = "mesh.decimation_scale is not supported: decimate the mesh before loading it", false;
+    if (m.contains("flip_normal") && m["flip_normal"].boolean())  // line 152
+        for (size_t t = 0; t < f.size(); t += 3) std::swap(f[t + 1], f[t + 2]);
+    if (m.contains("enable_traction_discontinuity") && m["enable_traction_discontinuity"].boolean()) {
+        // lines 68-71, 203-365
+        const double deg = m.contains("sharp_edge_threshold_degree") ? m["sharp_edge_threshold_degree"].number() : 20.0;
+        std::vector<double> vs;
+        std::vector<int32_t> fs;
+        if (split_traction_discontinuity(v.data(), (int)(v.size() / 3), f.data(), (int)(f.size() / 3), deg, vs, fs, vmap,
+                                         err) != CQBEM_OK)
+            return err = "traction-discontinuity split: " + err, false;
+        v.swap(vs), f.swap(fs);
+    }
+    return true;
+}
+
 bool ElastodynamicsCQBEMMatrixCollection::init(const Json &config) {
     try {
-        const Json &m = config["mesh"];
         std::vector<double> v;
-        std::vector<int32_t> f;
-        if (!load_obj(m["file"].string(), v, f, error_)) return false;
-        if (m.contains("scale"))
-            for (double &x : v) x *= m["scale"].number();
-        if (m.contains("enable_traction_discontinuity") && m["enable_traction_discontinuity"].boolean()) {
-            // physics_base_matrix_collection.cpp:68-71,203-365 (vertex reordering off)
-            const double deg = m.contains("sharp_edge_threshold_degree") ? m["sharp_edge_threshold_degree"].number() : 20.0;
-            std::vector<double> vs;
-            std::vector<int32_t> fs, vm;
-            std::string err;
-            if (split_traction_discontinuity(v.data(), (int)(v.size() / 3), f.data(), (int)(f.size() / 3), deg, vs, fs, vm,
-                                             err) != CQBEM_OK)
-                return fail("traction-discontinuity split: " + err);
-            return init_mesh(config, std::move(vs), std::move(fs), std::move(vm));
-        }
-        return init_mesh(config, std::move(v), std::move(f), {});
+        std::vector<int32_t> f, vm;
+        std::string err;
+        if (!prepare_mesh(config["mesh"], v, f, vm, err)) return fail(err);
+        return init_mesh(config, std::move(v), std::move(f), std::move(vm));
     } catch (const std::exception &e) {
         return fail(e.what());
     }

@@ -121,4 +121,12 @@ int split_traction_discontinuity(const double *V, int nv, const int32_t *F, int 
 // Loads an OBJ (v / f lines, polygons fanned into triangles).
 bool load_obj(const std::string &path, std::vector<double> &V, std::vector<int32_t> &F, std::string &err);
 
+// The mesh an elastodynamicsCQBEM object is assembled on, from the "mesh" block of its config
+// (PhysicsBaseMatrixCollection::load_mesh, physics_base_matrix_collection.cpp:59-365, with vertex
+// reordering off): OBJ loaded, unused vertices dropped, scaled, normals flipped on request, and
+// split along sharp edges when enable_traction_discontinuity is set (vmap then maps every vertex
+// to its original; empty otherwise).  subdivision_level > 0 and decimation_scale < 1 are rejected.
+bool prepare_mesh(const Json &mesh, std::vector<double> &V, std::vector<int32_t> &F, std::vector<int32_t> &vmap,
+                  std::string &err);
+
 }  // namespace cqbem

@@ -1,0 +1,110 @@
+"""Mesh preparation of an elastodynamicsCQBEM object (PhysicsBaseMatrixCollection::load_mesh,
+src/physics_base_matrix_collection.cpp:59-365, vertex reordering off) in the C++ facade, through the
+collection driver's --mesh mode: OBJ load, unused vertices dropped (survivors keep their order,
+lines 75-103), scale, flip_normal (line 152), traction-discontinuity split.  Host code only."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from surfaceonlydeformables_amd import mesh as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "surfaceonlydeformables_amd", "collection_driver")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def prepare(tmp_path, mesh_block):
+    path = tmp_path / "scene.json"
+    path.write_text(json.dumps({"mesh": mesh_block}))
+    r = subprocess.run([DRIVER, "--mesh", str(path)], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        return None, r.stderr
+    d = json.loads(r.stdout)
+    return (np.array(d["V"]).reshape(-1, 3), np.array(d["F"], dtype=np.int64).reshape(-1, 3),
+            np.array(d["vmap"], dtype=np.int64)), ""
+
+
+def write_obj_with_unused(path, V, F, rng):
+    """The mesh's vertices with unreferenced ones interleaved; faces renumbered to match."""
+    order = []
+    for i in range(len(V)):
+        while rng.random() < 0.3:
+            order.append(-1)
+        order.append(i)
+    newidx = np.empty(len(V), dtype=np.int64)
+    lines = []
+    for slot, i in enumerate(order):
+        if i < 0:
+            lines.append("v 9.0 9.0 9.0")
+        else:
+            lines.append("v %.17g %.17g %.17g" % tuple(V[i]))
+            newidx[i] = slot
+    lines += ["f %d %d %d" % tuple(newidx[f] + 1) for f in F]
+    path.write_text("\n".join(lines) + "\n")
+    return len(order)
+
+
+def test_reference_cube_scene_mesh(tmp_path):
+    """The cube scenes' mesh block (cube_reordered.obj, split at the default 20 degrees) gives the
+    split mesh the reference wrote for it, numbering included."""
+    got, err = prepare(tmp_path, {"file": os.path.join(GOLDEN, "cube_reordered.obj"),
+                                  "enable_vertex_reordering": False, "enable_traction_discontinuity": True})
+    assert got is not None, err
+    Vs, Fs, vm = got
+    Vd, Fd = M.read_obj(os.path.join(GOLDEN, "cube_reordered_discontinuous.obj"))
+    assert np.array_equal(Vs, Vd) and np.array_equal(Fs, Fd)
+    V0, _ = M.read_obj(os.path.join(GOLDEN, "cube_reordered.obj"))
+    assert np.array_equal(V0[vm], Vs)
+
+
+def test_unused_vertices_scale_and_flip(tmp_path):
+    V, F = M.icosphere(1, 0.3)
+    rng = np.random.default_rng(5)
+    obj = tmp_path / "holes.obj"
+    n_file = write_obj_with_unused(obj, V, F, rng)
+    assert n_file > len(V)
+    got, err = prepare(tmp_path, {"file": str(obj)})
+    assert got is not None, err
+    Vp, Fp, vm = got
+    assert np.array_equal(Vp, V) and np.array_equal(Fp, F) and len(vm) == 0
+    got, err = prepare(tmp_path, {"file": str(obj), "scale": 2.5, "flip_normal": True})
+    Vp, Fp, _ = got
+    assert np.array_equal(Vp, 2.5 * V)
+    assert np.array_equal(Fp, F[:, [0, 2, 1]])
+    # flipped orientation: the enclosed volume changes sign
+    assert M.volume_centroid(Vp, Fp)[0] < 0 < M.volume_centroid(Vp, F)[0]
+
+
+def test_flip_then_split(tmp_path):
+    """flip_normal comes before the split (line 152 before line 203): the split of the flipped cube
+    relabels the flipped faces."""
+    V, F = M.cube(2, 0.5)
+    obj = tmp_path / "cube.obj"
+    M.write_obj(str(obj), V, F)
+    got, err = prepare(tmp_path, {"file": str(obj), "flip_normal": True, "enable_traction_discontinuity": True})
+    assert got is not None, err
+    Vs, Fs, vm = got
+    assert len(Vs) == 6 * 9
+    assert np.array_equal(vm[Fs], F[:, [0, 2, 1]])
+    assert np.array_equal(V[vm], Vs)
+
+
+@pytest.mark.parametrize("key,value", [("subdivision_level", 1), ("decimation_scale", 0.5)])
+def test_unsupported_mesh_operations_are_rejected(tmp_path, key, value):
+    V, F = M.icosphere(0, 0.3)
+    obj = tmp_path / "m.obj"
+    M.write_obj(str(obj), V, F)
+    got, err = prepare(tmp_path, {"file": str(obj), key: value})
+    assert got is None and key in err
+    got, err = prepare(tmp_path, {"file": str(obj), key: 0 if key == "subdivision_level" else 1.0})
+    assert got is not None, err
+
+
+def test_bad_face_index_is_rejected(tmp_path):
+    obj = tmp_path / "bad.obj"
+    obj.write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 4\n")
+    got, err = prepare(tmp_path, {"file": str(obj)})
+    assert got is None and "out of range" in err
