@@ -219,6 +219,62 @@ def serving_main(args):
                       "makespan_s": makespan, "dtype": "f64", "data": "synthetic"}), flush=True)
 
 
+def time_stepping_main(args):
+    """CQ time steps of one body on the icosphere-L4 matrices (L = 32) through the C++ facade: the collection
+    driver assembles, post-processes, uploads the matrices once, then steps an ElastodynamicsCQBEMObject
+    (history sums, u_p p and the H_0^-1 product on the GPU, drift elimination on the host).
+    The first three steps carry a pressure pulse (constrained steps), the rest none."""
+    import subprocess
+    import tempfile
+    from surfaceonlydeformables_amd import mesh as M
+    V, F, params, desc = workload("ico4-collocation", "fp64")
+    nv, n3, L = len(V), 3 * len(V), params.L
+    steps = L + max(8, 2 * args.steps)  # step_ms is the mean once the history holds L - 1 steps
+    with tempfile.TemporaryDirectory() as tmp:
+        obj = os.path.join(tmp, "mesh.obj")
+        M.write_obj(obj, V, F)
+        cfg = {"enable_cuda": True, "mesh": {"file": obj},
+               "material": {"rho": params.rho, "mu": params.rho * params.c2 ** 2, "nu": params.nu},
+               "simulation": {"dt": params.dt, "gaussian_quadrature_order": 5, "quadrature_subdivision": 2,
+                              "multistep_method": "BDF2", "use_galerkin": False, "cq_length": L},
+               "object": {"translational_vel": [0.1, 0.0, 0.0], "rotational_vel": [0.0, 1.0, 0.0]}}
+        scene = os.path.join(tmp, "scene.json")
+        with open(scene, "w") as fh:
+            json.dump(cfg, fh)
+        Vc = V - V.mean(axis=0)
+        p = np.zeros((steps, nv, 3))
+        mask = Vc[:, 0] > 0.4 * Vc[:, 0].max()
+        p[:3, mask] = -300.0 * Vc[mask] / np.linalg.norm(Vc[mask], axis=1)[:, None]
+        tr = os.path.join(tmp, "tractions.bin")
+        p.reshape(steps, n3).tofile(tr)
+        drv = os.path.join(ROOT, "surfaceonlydeformables_amd", "collection_driver")
+        print(f"time stepping: {desc}, {steps} steps", file=sys.stderr, flush=True)
+        r = subprocess.run([drv, "--simulate", scene, str(steps), tr, "-"], capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stderr)
+            sys.exit(1)
+        info = json.loads(r.stdout.strip().splitlines()[-1])
+    # the reference's history loop on the host (Eigen GEMVs over 2 (L - 1) dense matrices): timed with numpy on a
+    # 4-matrix sample held in host memory, scaled to 2 (L - 1) matrices
+    rng = np.random.default_rng(0)
+    sample = [rng.standard_normal((n3, n3)) for _ in range(4)]
+    x = rng.standard_normal(n3)
+    t0 = time.perf_counter()
+    for A in sample:
+        A @ x
+    per = (time.perf_counter() - t0) / len(sample)
+    print(json.dumps({"metric": "CQ time step of one body on resident matrices (history + H_0 solve + drift "
+                                "elimination)", "value": info["step_ms"], "unit": "ms", "higher_is_better": False,
+                      "steps": steps, "step_phases_ms": info["step_phases_ms"], "first_step_ms": info["first_step_ms"],
+                      "upload_ms": info["upload_ms"],
+                      "assembly_ms": info["ms"], "post_process_ms": info["factor_ms"],
+                      "history_matrices_GB": 2 * (L - 1) * n3 * n3 * 8 / 1e9,
+                      "cpu_history_ms": {"value": per * 2 * (L - 1) * 1e3, "threads": os.cpu_count(),
+                                         "sample": f"numpy {n3} x {n3} GEMV, 4 matrices timed, x {2 * (L - 1)}"},
+                      "config": {"workload": desc, "nv": nv, "L": L}, "dtype": "f64", "data": "synthetic"}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,6 +290,8 @@ def main():
     ap.add_argument("--scene", action="store_true", help="multi-object scene throughput instead of the single-object line")
     ap.add_argument("--serving", type=float, default=0.0, help="Poisson serving stream at this many jobs/s")
     ap.add_argument("--duration", type=float, default=10.0, help="serving stream length in seconds")
+    ap.add_argument("--time-stepping", action="store_true",
+                    help="time steps of one icosphere-L4 body on its resident matrices instead of the assembly line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the CQBEM precompute of one object runs on one MI355X "
@@ -244,6 +302,8 @@ def main():
         return scene_main(args)
     if args.serving > 0:
         return serving_main(args)
+    if args.time_stepping:
+        return time_stepping_main(args)
 
     import torch
     from surfaceonlydeformables_amd import abi

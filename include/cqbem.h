@@ -140,6 +140,9 @@ int cqbem_reserved_bytes(const cqbem_context *ctx, uint64_t *bytes);
  * (its term is skipped).  b (n3) is complete when the call returns. */
 int cqbem_history(cqbem_context *ctx, int64_t n3, int32_t L, const double *G, const double *H, const double *B,
                   const double *p_hist, const double *u_hist, const double *acc_hist, int32_t nhist, double *b);
+/* y = A x for a row-major n x n device matrix A and device vectors x, y (the u_p * p product of a
+ * constrained step, elastodynamics_cqbem_object.cpp:136).  y is complete when the call returns. */
+int cqbem_matvec(cqbem_context *ctx, int64_t n, const double *A, const double *x, double *y);
 
 #ifdef __cplusplus
 }

@@ -22,7 +22,7 @@ G, H, B_TRANS, B_ANGULAR, ALL = 1, 2, 4, 8, 15
 EXPORTED = [
     "cqbem_create", "cqbem_destroy", "cqbem_last_error", "cqbem_assemble", "cqbem_job_footprint",
     "cqbem_laplace", "cqbem_frequencies", "cqbem_last_timings", "cqbem_submit", "cqbem_poll",
-    "cqbem_wait", "cqbem_reserved_bytes", "cqbem_history", "cqbem_split_traction_discontinuity",
+    "cqbem_wait", "cqbem_reserved_bytes", "cqbem_history", "cqbem_matvec", "cqbem_split_traction_discontinuity",
 ]
 
 
@@ -81,6 +81,7 @@ def lib():
         L.cqbem_wait.argtypes = [C.c_void_p, C.c_uint64]
         L.cqbem_reserved_bytes.argtypes = [C.c_void_p, P(C.c_uint64)]
         L.cqbem_history.argtypes = [C.c_void_p, C.c_int64, C.c_int32] + [C.c_void_p] * 6 + [C.c_int32, C.c_void_p]
+        L.cqbem_matvec.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
         L.cqbem_split_traction_discontinuity.argtypes = [P(C.c_double), C.c_int64, P(C.c_int32), C.c_int64, C.c_double,
                                                          P(C.c_double), P(C.c_int32), P(C.c_int32), P(C.c_int64)]
         _lib = L
@@ -236,6 +237,10 @@ class Context:
         """History term of one CQ step on device buffers (device pointers as ints; G, H, B may be 0)."""
         self._check(lib().cqbem_history(self._h, n3, L, G or None, H or None, B or None, p_hist or None,
                                         u_hist or None, acc_hist or None, nhist, b))
+
+    def matvec(self, n: int, A, x, y):
+        """y = A x on device buffers (row-major n x n A; device pointers as ints)."""
+        self._check(lib().cqbem_matvec(self._h, n, A, x, y))
 
 
 def job_footprint(mesh, params, outputs_on_device=True) -> int:

@@ -103,6 +103,7 @@ bool DeviceLU::factor(int64_t n, const double *A, std::string &err) {
 }
 
 bool DeviceLU::solve(double *B, int64_t nrhs, std::string &err) const {
+    std::lock_guard<std::mutex> lk(mu_);
     const RocLib &lib = roclib();
     if (!lu_) return err = "LU not computed", false;
     std::vector<double> cm;
@@ -122,6 +123,21 @@ bool DeviceLU::solve(double *B, int64_t nrhs, std::string &err) const {
     (void)hipFree(d);
     if (ok && nrhs > 1) to_colmajor(cm.data(), nrhs, n_, B);  // column-major n x nrhs back to row-major
     return ok;
+}
+
+bool DeviceLU::inverse_to_device(double *dX, std::string &err) const {
+    std::lock_guard<std::mutex> lk(mu_);
+    const RocLib &lib = roclib();
+    if (!lu_) return err = "LU not computed", false;
+    std::vector<double> eye((size_t)(n_ * n_), 0.0);
+    for (int64_t i = 0; i < n_; i++) eye[(size_t)(i * n_ + i)] = 1.0;
+    if (!hip_ok(hipMemcpy(dX, eye.data(), sizeof(double) * n_ * n_, hipMemcpyHostToDevice), "hipMemcpy identity", err))
+        return false;
+    // lu_ factors A^T; A^T Y = I gives the column-major Y = A^-T, whose memory read row-major is A^-1
+    if (lib.dgetrs(static_cast<rocblas_handle>(handle_), rocblas_operation_none, (rocblas_int)n_, (rocblas_int)n_, lu_,
+                   (rocblas_int)n_, ipiv_, dX, (rocblas_int)n_) != rocblas_status_success)
+        return err = "rocsolver_dgetrs failed", false;
+    return hip_ok(hipDeviceSynchronize(), "inverse", err);
 }
 
 bool ThinLeastSquares::compute(const std::vector<double> &B, int64_t m, int k, std::string &err) {

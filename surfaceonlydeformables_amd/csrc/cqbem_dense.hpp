@@ -7,6 +7,7 @@
 // libcqbem.so itself keeps no link-time dependency on them.
 #pragma once
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -23,8 +24,11 @@ class DeviceLU {
     // Factorises the row-major matrix A (host, n x n).  Returns false (and sets err) on a library
     // error or an exactly singular matrix.
     bool factor(int64_t n, const double *A, std::string &err);
-    // Solves A X = B in place for the row-major n x nrhs host matrix B.
+    // Solves A X = B in place for the row-major n x nrhs host matrix B.  Thread-safe.
     bool solve(double *B, int64_t nrhs, std::string &err) const;
+    // Writes A^-1 (row-major n x n) to the device buffer dX, from the factors (for repeated one-vector
+    // solves as a product: the triangular solves of one right-hand side are latency-bound).  Thread-safe.
+    bool inverse_to_device(double *dX, std::string &err) const;
     int64_t size() const { return n_; }
 
   private:
@@ -32,6 +36,7 @@ class DeviceLU {
     double *lu_ = nullptr;  // column-major factors of A^T (the row-major A read as column-major)
     int32_t *ipiv_ = nullptr;
     void *handle_ = nullptr;
+    mutable std::mutex mu_;  // the rocBLAS handle serves one call at a time
 };
 
 // Least-squares (minimum-norm for full column rank) solver for a thin m x k row-major matrix B,

@@ -695,6 +695,21 @@ int cqbem_history(cqbem_context *ctx, int64_t n3, int32_t L, const double *G, co
     return CQBEM_OK;
 }
 
+int cqbem_matvec(cqbem_context *ctx, int64_t n, const double *A, const double *x, double *y) {
+    if (!ctx) return CQBEM_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    try {
+        if (n <= 0 || !A || !x || !y) throw Err{CQBEM_ERR_ARG, "matvec: bad size or null pointer"};
+        HIPCHK(hipSetDevice(ctx->device));
+        HIPCHK(launch_matvec(A, x, n, y, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    } catch (const Err &e) {
+        ctx->last_error = e.msg;
+        return e.code;
+    }
+    return CQBEM_OK;
+}
+
 int cqbem_job_footprint(const cqbem_mesh *mesh, const cqbem_params *params, int outputs_on_device, uint64_t *bytes) {
     try {
         check_inputs(mesh, params);

@@ -859,6 +859,18 @@ __global__ void __launch_bounds__(256) k_history(const double *__restrict__ G, c
     }
 }
 
+// y = A x for a row-major n x n matrix (u_p p of a constrained time step); one wavefront per row, as k_history.
+__global__ void __launch_bounds__(256) k_matvec(const double *__restrict__ A, const double *__restrict__ x, long long n,
+                                                double *__restrict__ y) {
+    const long long r = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (r >= n) return;
+    double s = history_row<true, false>(A + r * n, nullptr, x, nullptr, n, lane, 0.0);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) y[r] = s;
+}
+
 // H_0 += C on the 3x3 diagonal blocks (collocation, H.cu:186-191).
 __global__ void k_add_c_colloc(double *H0, const double *C, int nv) {
     const int t = blockIdx.x * 256 + threadIdx.x;
@@ -999,6 +1011,11 @@ hipError_t launch_history(const double *G, const double *H, const double *B, con
         hipLaunchKernelGGL((k_history<false, true>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, m, nh, b);
     else
         hipLaunchKernelGGL((k_history<false, false>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, m, nh, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_matvec(const double *A, const double *x, long long n, double *y, hipStream_t st) {
+    hipLaunchKernelGGL(k_matvec, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, A, x, n, y);
     return hipGetLastError();
 }
 

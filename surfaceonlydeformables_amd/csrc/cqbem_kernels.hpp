@@ -54,6 +54,7 @@ hipError_t launch_add_c_colloc(double *H0, const double *C, int nv, hipStream_t 
 // b = sum_{i=1..m} (G_i p[nh-i] - H_i u[nh-i] + B_i acc[nh-i]); any of G, H, B may be null.
 hipError_t launch_history(const double *G, const double *H, const double *B, const double *p, const double *u,
                           const double *acc, long long n3, int m, int nh, double *b, hipStream_t st);
+hipError_t launch_matvec(const double *A, const double *x, long long n, double *y, hipStream_t st);
 hipError_t launch_add_triplets(double *H0, const long long *idx, const double *val, long long n, hipStream_t st);
 hipError_t launch_b_post(double *Bt, double *Ba, double *B6, const double *V, int nv, int L, hipStream_t st);
 

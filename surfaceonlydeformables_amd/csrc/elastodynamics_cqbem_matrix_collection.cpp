@@ -5,6 +5,8 @@
 
 #include "cqbem_rules.hpp"
 
+#include <hip/hip_runtime.h>
+
 #include <algorithm>
 #include <cerrno>
 #include <chrono>
@@ -520,7 +522,49 @@ ElastodynamicsCQBEMMatrixCollection::ElastodynamicsCQBEMMatrixCollection(double 
                                                                          unsigned int cuda_thread_per_block)
     : dt_(dt), enable_cuda_(enable_cuda), cuda_thread_per_block_(cuda_thread_per_block) {}
 
-ElastodynamicsCQBEMMatrixCollection::~ElastodynamicsCQBEMMatrixCollection() = default;
+ElastodynamicsCQBEMMatrixCollection::~ElastodynamicsCQBEMMatrixCollection() { free_resident(); }
+
+void ElastodynamicsCQBEMMatrixCollection::free_resident() {
+    for (double *p : {resident_.G, resident_.H, resident_.B, resident_.u_p, resident_.H_inv})
+        if (p) (void)hipFree(p);
+    resident_ = Resident{};
+    resident_ready_ = false;
+}
+
+const ElastodynamicsCQBEMMatrixCollection::Resident &ElastodynamicsCQBEMMatrixCollection::resident() const {
+    std::lock_guard<std::mutex> lk(resident_mu_);
+    if (resident_ready_) return resident_;
+    const size_t L = G_matrices.size();
+    const int64_t n3 = 3 * num_vertices;
+    if (L == 0 || (int64_t)u_p.data.size() != n3 * n3) throw std::runtime_error("collection not initialised");
+    std::string err;
+    Resident r;
+    r.ctx = context(err);
+    if (!r.ctx) throw std::runtime_error(err);
+    auto fail_up = [&](const char *what) {
+        for (double *p : {r.G, r.H, r.B, r.u_p, r.H_inv})
+            if (p) (void)hipFree(p);
+        throw std::runtime_error(std::string("resident matrices: ") + what);
+    };
+    const size_t mat = sizeof(double) * (size_t)(n3 * n3);
+    if (hipMalloc(&r.G, mat * L) != hipSuccess || hipMalloc(&r.H, mat * L) != hipSuccess ||
+        hipMalloc(&r.B, sizeof(double) * (size_t)(n3 * 6) * L) != hipSuccess || hipMalloc(&r.u_p, mat) != hipSuccess ||
+        hipMalloc(&r.H_inv, mat) != hipSuccess)
+        fail_up("hipMalloc failed");
+    std::vector<double> h((size_t)(n3 * n3));
+    for (size_t n = 0; n < L; n++) {
+        for (size_t e = 0; e < h.size(); e++) h[e] = -H_matrices[n].data[e];  // back to the assembled sign
+        if (hipMemcpy(r.G + n * n3 * n3, G_matrices[n].data.data(), mat, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(r.H + n * n3 * n3, h.data(), mat, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(r.B + n * n3 * 6, B_matrices[n].data.data(), sizeof(double) * n3 * 6, hipMemcpyHostToDevice) != hipSuccess)
+            fail_up("upload failed");
+    }
+    if (hipMemcpy(r.u_p, u_p.data.data(), mat, hipMemcpyHostToDevice) != hipSuccess) fail_up("upload failed");
+    if (!H_inv.inverse_to_device(r.H_inv, err)) fail_up(err.c_str());
+    resident_ = r;
+    resident_ready_ = true;
+    return resident_;
+}
 
 bool ElastodynamicsCQBEMMatrixCollection::fail(const std::string &msg) {
     error_ = msg;
@@ -614,9 +658,9 @@ bool ElastodynamicsCQBEMMatrixCollection::run(const Json &config) {
             for (int k = 0; k < 3; k++) c[k] += tv * (a[k] + b[k] + d[k]) / 4.0;
         }
         if (!(vol > 0)) return fail("mesh must be closed and oriented outward (non-positive volume)");
-        for (int k = 0; k < 3; k++) cm[k] = c[k] / vol;
+        for (int k = 0; k < 3; k++) original_cm[k] = c[k] / vol, cm[k] = 0.0;
         for (int64_t i = 0; i < num_vertices; i++)
-            for (int k = 0; k < 3; k++) V[3 * i + k] -= cm[k];
+            for (int k = 0; k < 3; k++) V[3 * i + k] -= original_cm[k];
         mass = vol * rho;
     }
     if (dt_ != sim["dt"].number()) return fail("dt for simulator and elastodynamics BEM need to be the same.");

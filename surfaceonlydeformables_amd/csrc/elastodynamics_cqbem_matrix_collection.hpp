@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -80,7 +81,8 @@ class ElastodynamicsCQBEMMatrixCollection {
     std::vector<double> V;  // centred vertices, nv x 3
     std::vector<int32_t> F;
     std::vector<int32_t> vertex_map_inverse;  // split vertex -> input vertex (empty without the split)
-    double cm[3] = {0, 0, 0};
+    double cm[3] = {0, 0, 0};           // centroid of the centred mesh: zero (physics_base_matrix_collection.cpp:45)
+    double original_cm[3] = {0, 0, 0};  // volume centroid of the mesh as loaded (:43)
     double mass = 0.0;
     double I[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // inertia tensor about the centroid, row-major
     double c1 = 0, c2 = 0, rho = 0, mu = 0, nu = 0, R = 0;
@@ -101,11 +103,26 @@ class ElastodynamicsCQBEMMatrixCollection {
     Dense u_p_diag;  // nv x 9: the 3 x 3 diagonal blocks of u_p
     std::vector<double> rotation_removal_weights;  // nv
 
+    // Device copies for time stepping on the GPU, on the device of the collection's context: G_n and H_n
+    // (un-negated, as cqbem_assemble writes them, for cqbem_history), B_n, u_p, and H_0^-1 (row-major,
+    // from the LU factors: one step's solve becomes a product).  Made on the first call, freed with the
+    // collection; throws std::runtime_error when that fails.
+    struct Resident {
+        cqbem_context *ctx = nullptr;
+        double *G = nullptr, *H = nullptr, *B = nullptr, *u_p = nullptr, *H_inv = nullptr;
+    };
+    const Resident &resident() const;
+
   private:
     bool init_mesh(const Json &config, std::vector<double> v, std::vector<int32_t> f, std::vector<int32_t> vmap);
     bool fail(const std::string &msg);
     bool run(const Json &config);
     bool post_process();
+
+    mutable std::mutex resident_mu_;
+    mutable Resident resident_;
+    mutable bool resident_ready_ = false;
+    void free_resident();
 
     double dt_;
     bool enable_cuda_;

@@ -1,4 +1,4 @@
-"""History term of a CQ time step on the resident matrices (cqbem_history), against numpy.
+"""History term of a CQ time step on the resident matrices (cqbem_history) and cqbem_matvec, against numpy.
 
 b = sum_{i=1..m} (G_i p[nh-i] - H_i u[nh-i] + B_i acc[nh-i]), m = min(nh, L-1): the sum
 ElastodynamicsCQBEMObject::estimate_next_state forms each step (elastodynamics_cqbem_object.cpp:42-49),
@@ -78,3 +78,18 @@ def test_history_on_assembled_matrices(ctx):
     ctx.history(n3, L, 0, outs["H"].data_ptr(), outs["B"].data_ptr(), 0, tu.data_ptr(), ta.data_ptr(), nh, b.data_ptr())
     want = reference(np.zeros_like(G), H, B, p, u, acc, L)
     assert np.linalg.norm(b.cpu().numpy() - want) <= 1e-13 * np.linalg.norm(want)
+
+
+@pytest.mark.parametrize("n", [1, 63, 258, 1031])
+def test_matvec(ctx, n):
+    """cqbem_matvec (the u_p p product of a constrained step) against numpy; relative error <= 1e-14."""
+    import torch
+    rng = np.random.default_rng(n)
+    A, x = rng.standard_normal((n, n)), rng.standard_normal(n)
+    dev = torch.device("cuda:0")
+    tA, tx = torch.from_numpy(A).to(dev), torch.from_numpy(x).to(dev)
+    y = torch.full((n,), float("nan"), dtype=torch.float64, device=dev)
+    print(f"matvec n={n}", flush=True)
+    ctx.matvec(n, tA.data_ptr(), tx.data_ptr(), y.data_ptr())
+    want = A @ x
+    assert np.linalg.norm(y.cpu().numpy() - want) <= 1e-14 * np.linalg.norm(A) * np.linalg.norm(x)
