@@ -144,6 +144,22 @@ int cqbem_history(cqbem_context *ctx, int64_t n3, int32_t L, const double *G, co
  * constrained step, elastodynamics_cqbem_object.cpp:136).  y is complete when the call returns. */
 int cqbem_matvec(cqbem_context *ctx, int64_t n, const double *A, const double *x, double *y);
 
+/* ---- wavelet-compressed matrices for the time stepping ----
+ * Haar compression of a dense matrix, the reference's CompressedMatrix<1>(mat, num_elems)
+ * (include/wavelets.hpp:154-219,433-445): the n x n row-major device matrix A (n >= 2, zero-padded to the
+ * power of two np >= n) in Beylkin's nonstandard form, (2np - 2) x (2np - 2), of which the num_elems
+ * largest-magnitude nonzero coefficients are kept, as CSR with columns ascending in each row, into
+ * caller-provided device arrays rowptr (2np - 1 entries), col and val (num_elems entries).  *nnz (host)
+ * receives the count kept (num_elems, or fewer when fewer coefficients are nonzero).  Coefficients tied
+ * with the smallest kept magnitude are kept in row-major order.  The reference keeps
+ * (int)(n * n * compression_ratio) (wavelets.hpp:443-445). */
+int cqbem_wavelet_compress(cqbem_context *ctx, int64_t n, const double *A, int64_t num_elems, int64_t *rowptr, int32_t *col,
+                           double *val, int64_t *nnz);
+/* y = M x for a CSR device matrix of `rows` rows (the product in the compressed space,
+ * CompressedMatrixVectorProduct, wavelets.hpp:352-356). */
+int cqbem_csr_matvec(cqbem_context *ctx, int64_t rows, const int64_t *rowptr, const int32_t *col, const double *val,
+                     const double *x, double *y);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,6 +23,7 @@ EXPORTED = [
     "cqbem_create", "cqbem_destroy", "cqbem_last_error", "cqbem_assemble", "cqbem_job_footprint",
     "cqbem_laplace", "cqbem_frequencies", "cqbem_last_timings", "cqbem_submit", "cqbem_poll",
     "cqbem_wait", "cqbem_reserved_bytes", "cqbem_history", "cqbem_matvec", "cqbem_split_traction_discontinuity",
+    "cqbem_wavelet_compress", "cqbem_csr_matvec",
 ]
 
 
@@ -82,6 +83,9 @@ def lib():
         L.cqbem_reserved_bytes.argtypes = [C.c_void_p, P(C.c_uint64)]
         L.cqbem_history.argtypes = [C.c_void_p, C.c_int64, C.c_int32] + [C.c_void_p] * 6 + [C.c_int32, C.c_void_p]
         L.cqbem_matvec.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.cqbem_wavelet_compress.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                             C.c_void_p, P(C.c_int64)]
+        L.cqbem_csr_matvec.argtypes = [C.c_void_p, C.c_int64] + [C.c_void_p] * 5
         L.cqbem_split_traction_discontinuity.argtypes = [P(C.c_double), C.c_int64, P(C.c_int32), C.c_int64, C.c_double,
                                                          P(C.c_double), P(C.c_int32), P(C.c_int32), P(C.c_int64)]
         _lib = L
@@ -241,6 +245,25 @@ class Context:
     def matvec(self, n: int, A, x, y):
         """y = A x on device buffers (row-major n x n A; device pointers as ints)."""
         self._check(lib().cqbem_matvec(self._h, n, A, x, y))
+
+    def wavelet_compress(self, n: int, A, num_elems: int, rowptr, col, val) -> int:
+        """Haar compression of the device matrix A (n x n) into device CSR arrays; returns nnz."""
+        nnz = C.c_int64()
+        self._check(lib().cqbem_wavelet_compress(self._h, n, A, num_elems, rowptr, col or None, val or None,
+                                                 C.byref(nnz)))
+        return nnz.value
+
+    def csr_matvec(self, rows: int, rowptr, col, val, x, y):
+        """y = M x for a device CSR matrix."""
+        self._check(lib().cqbem_csr_matvec(self._h, rows, rowptr, col or None, val or None, x, y))
+
+
+def wavelet_padded(n: int) -> int:
+    """The power of two >= n that a compressed n x n matrix is padded to."""
+    np_ = 1
+    while np_ < n:
+        np_ <<= 1
+    return np_
 
 
 def job_footprint(mesh, params, outputs_on_device=True) -> int:

@@ -25,6 +25,7 @@
 
 #include "../../include/cqbem.h"
 #include "cqbem_galerkin.hpp"
+#include "cqbem_wavelet.hpp"
 #include "cqbem_pool.hpp"
 #include "cqbem_kernels.hpp"
 #include "cqbem_rules.hpp"
@@ -702,6 +703,41 @@ int cqbem_matvec(cqbem_context *ctx, int64_t n, const double *A, const double *x
         if (n <= 0 || !A || !x || !y) throw Err{CQBEM_ERR_ARG, "matvec: bad size or null pointer"};
         HIPCHK(hipSetDevice(ctx->device));
         HIPCHK(launch_matvec(A, x, n, y, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    } catch (const Err &e) {
+        ctx->last_error = e.msg;
+        return e.code;
+    }
+    return CQBEM_OK;
+}
+
+int cqbem_wavelet_compress(cqbem_context *ctx, int64_t n, const double *A, int64_t num_elems, int64_t *rowptr, int32_t *col,
+                           double *val, int64_t *nnz) {
+    if (!ctx) return CQBEM_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    try {
+        if (n < 2 || n > (int64_t)1 << 24 || num_elems < 0 || !A || !rowptr || !nnz || (num_elems > 0 && (!col || !val)))
+            throw Err{CQBEM_ERR_ARG, "wavelet_compress: bad size or null pointer"};
+        HIPCHK(hipSetDevice(ctx->device));
+        *nnz = wavelet_compress_into(A, n, num_elems, 1.0, rowptr, col, val, ctx->stream);
+    } catch (const Err &e) {
+        ctx->last_error = e.msg;
+        return e.code;
+    } catch (const std::exception &e) {
+        ctx->last_error = e.what();
+        return CQBEM_ERR_HIP;
+    }
+    return CQBEM_OK;
+}
+
+int cqbem_csr_matvec(cqbem_context *ctx, int64_t rows, const int64_t *rowptr, const int32_t *col, const double *val,
+                     const double *x, double *y) {
+    if (!ctx) return CQBEM_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    try {
+        if (rows <= 0 || !rowptr || !x || !y) throw Err{CQBEM_ERR_ARG, "csr_matvec: bad size or null pointer"};
+        HIPCHK(hipSetDevice(ctx->device));
+        HIPCHK(launch_csr_matvec(rowptr, col, val, x, rows, y, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
     } catch (const Err &e) {
         ctx->last_error = e.msg;
