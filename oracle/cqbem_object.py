@@ -3,7 +3,8 @@
 Only tests/ import this module; the product path never does.
 
 Follows src/elastodynamics_cqbem_object.cpp (estimate_next_state :25-82, update_V_estimate_matrices
-:84-124, confirm_next_state :126-238, uncompressed matrices) and the PhysicsBaseObject parts it uses
+:84-124, confirm_next_state :126-238; with m["compression_ratio"] != 0 the compressed branches, whose
+matrices come from oracle/cqbem_wavelet.py) and the PhysicsBaseObject parts it uses
 (src/physics_base_object.cpp:9-45 init, :85-95 Rodrigues).  The inputs are the collection's matrices:
 G_n, H_n (negated, as the object reads them), B_n, u_p and the 3 x 3nv post-processing products.  The
 solves are restated with LAPACK: H_inv as numpy.linalg.solve on -H_0, B_trans_inv / B_euler_inv as
@@ -72,21 +73,41 @@ class ObjectStep:
         self.V_global = self.V_local @ self.rotation.T + self.translation
         self.hist_acc, self.hist_u, self.hist_p = deque(), deque(), deque()
         self.has_constraints = False
+        self.ratio = m.get("compression_ratio", 0.0)
+        if self.ratio:
+            import cqbem_wavelet as W
+            self.W = W
+            keep = int(n3 * n3 * self.ratio)
+            self.Gc = [None] + [W.compress(m["G"][i], keep) for i in range(1, self.L)]
+            self.Hc = [None] + [W.compress(m["H"][i], keep) for i in range(1, self.L)]
+            self.u_p_c = W.compress(m["u_p"], keep)
+            self.H_inv_c = W.compress(np.linalg.inv(self.H0), keep)
 
     def estimate_next_state(self):
         m = self.m
         self.has_constraints = False
         b = np.zeros(self.n3)
         nh = len(self.hist_acc)
-        for i in range(min(nh, self.max_time_history)):
-            b += m["B"][i + 1] @ self.hist_acc[nh - i - 1]
-            b += m["H"][i + 1] @ self.hist_u[nh - i - 1]
-            b += m["G"][i + 1] @ self.hist_p[nh - i - 1]
+        if self.ratio:
+            bc = 0.0
+            for i in range(min(nh, self.max_time_history)):
+                b += m["B"][i + 1] @ self.hist_acc[nh - i - 1]
+                bc = bc + self.Hc[i + 1] @ self.hist_u[nh - i - 1] + self.Gc[i + 1] @ self.hist_p[nh - i - 1]
+            if nh:
+                b += self.W.reconstruct(bc, self.n3)
+        else:
+            for i in range(min(nh, self.max_time_history)):
+                b += m["B"][i + 1] @ self.hist_acc[nh - i - 1]
+                b += m["H"][i + 1] @ self.hist_u[nh - i - 1]
+                b += m["G"][i + 1] @ self.hist_p[nh - i - 1]
         if not m["update_local_frame"] and self.use_gravity:
             b -= self.Bt0 @ (self.rotation.T @ self.g)
         self.b = b
         self.p = np.zeros(self.n3)
-        self.u_unconstrained = np.linalg.solve(self.H0, b)
+        if self.ratio:
+            self.u_unconstrained = self.W.product(self.H_inv_c, b)
+        else:
+            self.u_unconstrained = np.linalg.solve(self.H0, b)
         u3 = self.u_unconstrained.reshape(-1, 3)
         if self.is_fixed:
             self.V_estimate_global = (self.V + u3) @ self.rotation.T + self.translation
@@ -121,7 +142,7 @@ class ObjectStep:
     def confirm_next_state(self):
         m, dt = self.m, self.dt
         if self.has_constraints:
-            u_p = m["u_p"] @ self.p
+            u_p = self.W.product(self.u_p_c, self.p) if self.ratio else m["u_p"] @ self.p
             u = u_p + self.u_unconstrained
             Vg = np.kron(np.eye(self.V.shape[0]), self.rotation_estimate) @ u_p + self.V_const
             if not self.is_fixed and m["update_local_frame"]:
@@ -176,5 +197,9 @@ class ObjectStep:
             if len(self.hist_acc) >= self.max_time_history:
                 self.hist_acc.popleft(), self.hist_u.popleft(), self.hist_p.popleft()
             self.hist_acc.append(np.concatenate([tacc, racc]))
-            self.hist_u.append(u)
-            self.hist_p.append(self.p.copy())
+            if self.ratio:  # compressed histories (object.cpp:216-222)
+                self.hist_u.append(self.W.decompose_vec(u))
+                self.hist_p.append(self.W.decompose_vec(self.p))
+            else:
+                self.hist_u.append(u)
+                self.hist_p.append(self.p.copy())

@@ -527,6 +527,9 @@ ElastodynamicsCQBEMMatrixCollection::~ElastodynamicsCQBEMMatrixCollection() { fr
 void ElastodynamicsCQBEMMatrixCollection::free_resident() {
     for (double *p : {resident_.G, resident_.H, resident_.B, resident_.u_p, resident_.H_inv})
         if (p) (void)hipFree(p);
+    for (auto &m : resident_.Gc) m.release();
+    for (auto &m : resident_.Hc) m.release();
+    resident_.u_p_c.release(), resident_.H_inv_c.release();
     resident_ = Resident{};
     resident_ready_ = false;
 }
@@ -544,9 +547,47 @@ const ElastodynamicsCQBEMMatrixCollection::Resident &ElastodynamicsCQBEMMatrixCo
     auto fail_up = [&](const char *what) {
         for (double *p : {r.G, r.H, r.B, r.u_p, r.H_inv})
             if (p) (void)hipFree(p);
+        for (auto &m : r.Gc) m.release();
+        for (auto &m : r.Hc) m.release();
+        r.u_p_c.release(), r.H_inv_c.release();
         throw std::runtime_error(std::string("resident matrices: ") + what);
     };
     const size_t mat = sizeof(double) * (size_t)(n3 * n3);
+    if (use_compressed_matrices) {
+        // one dense scratch matrix at a time, compressed on the device (CompressedMatrix(mat, ratio) keeps
+        // (int)(mat.size() * ratio) coefficients, wavelets.hpp:443-445)
+        const int64_t keep = (int64_t)(int)((double)(n3 * n3) * compression_ratio);
+        double *scratch = nullptr;
+        if (hipMalloc(&scratch, mat) != hipSuccess) fail_up("hipMalloc failed");
+        try {
+            r.Gc.resize(L), r.Hc.resize(L);
+            for (size_t n = 1; n < L; n++) {
+                if (hipMemcpy(scratch, G_matrices[n].data.data(), mat, hipMemcpyHostToDevice) != hipSuccess)
+                    throw std::runtime_error("upload failed");
+                wavelet_compress(scratch, n3, keep, 1.0, r.Gc[n], nullptr);
+                if (hipMemcpy(scratch, H_matrices[n].data.data(), mat, hipMemcpyHostToDevice) != hipSuccess)
+                    throw std::runtime_error("upload failed");
+                wavelet_compress(scratch, n3, keep, 1.0, r.Hc[n], nullptr);  // H_matrices are negated already
+            }
+            if (hipMemcpy(scratch, u_p.data.data(), mat, hipMemcpyHostToDevice) != hipSuccess)
+                throw std::runtime_error("upload failed");
+            wavelet_compress(scratch, n3, keep, 1.0, r.u_p_c, nullptr);
+            if (!H_inv.inverse_to_device(scratch, err)) throw std::runtime_error(err);
+            wavelet_compress(scratch, n3, keep, 1.0, r.H_inv_c, nullptr);
+            r.vlen = 2 * wavelet_padded(n3) - 2;
+        } catch (const std::exception &e) {
+            (void)hipFree(scratch);
+            fail_up(e.what());
+        }
+        (void)hipFree(scratch);
+        if (hipMalloc(&r.B, sizeof(double) * (size_t)(n3 * 6) * L) != hipSuccess) fail_up("hipMalloc failed");
+        for (size_t n = 0; n < L; n++)
+            if (hipMemcpy(r.B + n * n3 * 6, B_matrices[n].data.data(), sizeof(double) * n3 * 6, hipMemcpyHostToDevice) != hipSuccess)
+                fail_up("upload failed");
+        resident_ = std::move(r);
+        resident_ready_ = true;
+        return resident_;
+    }
     if (hipMalloc(&r.G, mat * L) != hipSuccess || hipMalloc(&r.H, mat * L) != hipSuccess ||
         hipMalloc(&r.B, sizeof(double) * (size_t)(n3 * 6) * L) != hipSuccess || hipMalloc(&r.u_p, mat) != hipSuccess ||
         hipMalloc(&r.H_inv, mat) != hipSuccess)
@@ -561,7 +602,7 @@ const ElastodynamicsCQBEMMatrixCollection::Resident &ElastodynamicsCQBEMMatrixCo
     }
     if (hipMemcpy(r.u_p, u_p.data.data(), mat, hipMemcpyHostToDevice) != hipSuccess) fail_up("upload failed");
     if (!H_inv.inverse_to_device(r.H_inv, err)) fail_up(err.c_str());
-    resident_ = r;
+    resident_ = std::move(r);
     resident_ready_ = true;
     return resident_;
 }
@@ -673,6 +714,8 @@ bool ElastodynamicsCQBEMMatrixCollection::run(const Json &config) {
         return fail("Unsupported multistep method detected. Valid options are \"BDF1\"/\"BDF2\"");
     use_galerkin = sim.contains("use_galerkin") && sim["use_galerkin"].boolean();
     update_local_frame = !sim.contains("update_local_frame") || sim["update_local_frame"].boolean();
+    compression_ratio = sim.contains("compression_ratio") ? sim["compression_ratio"].number() : 0.0;
+    use_compressed_matrices = compression_ratio != 0.0;
     const int order = (int)sim["gaussian_quadrature_order"].number();
     const int sub = (int)sim["quadrature_subdivision"].number();
     // material constants and history length (matrix_collection.cpp:59-69)

@@ -12,6 +12,7 @@
 
 #include "../../include/cqbem.h"
 #include "cqbem_dense.hpp"
+#include "cqbem_wavelet.hpp"
 
 namespace cqbem {
 
@@ -88,6 +89,10 @@ class ElastodynamicsCQBEMMatrixCollection {
     double c1 = 0, c2 = 0, rho = 0, mu = 0, nu = 0, R = 0;
     bool use_galerkin = false;
     bool update_local_frame = true;
+    // simulation.compression_ratio (elastics_bem_matrix_collection.cpp:22-24): the time stepping then uses
+    // wavelet-compressed G_n, H_n, u_p and H_0^-1 keeping (int)(9 nv^2 ratio) coefficients each
+    double compression_ratio = 0.0;
+    bool use_compressed_matrices = false;
     MultistepMethod multistep_method = BDF2;
     double last_assembly_ms = 0.0, last_factor_ms = 0.0;
     bool loaded_from_cache = false;  // G and H came from simulation.coeffs_folder_path
@@ -105,11 +110,16 @@ class ElastodynamicsCQBEMMatrixCollection {
 
     // Device copies for time stepping on the GPU, on the device of the collection's context: G_n and H_n
     // (un-negated, as cqbem_assemble writes them, for cqbem_history), B_n, u_p, and H_0^-1 (row-major,
-    // from the LU factors: one step's solve becomes a product).  Made on the first call, freed with the
-    // collection; throws std::runtime_error when that fails.
+    // from the LU factors: one step's solve becomes a product).  With use_compressed_matrices, G_n, H_n
+    // (n >= 1, H negated as the object reads it), u_p and H_0^-1 are held wavelet-compressed instead
+    // (matrix_collection.cpp:97-134,229-271) and the dense G, H, u_p, H_inv stay null.  Made on the first
+    // call, freed with the collection; throws std::runtime_error when that fails.
     struct Resident {
         cqbem_context *ctx = nullptr;
         double *G = nullptr, *H = nullptr, *B = nullptr, *u_p = nullptr, *H_inv = nullptr;
+        std::vector<DeviceCsr> Gc, Hc;  // [n], n = 1 .. L-1 (entry 0 unused)
+        DeviceCsr u_p_c, H_inv_c;
+        int64_t vlen = 0;  // length of a compressed vector, 2 np - 2
     };
     const Resident &resident() const;
 

@@ -120,6 +120,31 @@ ElastodynamicsCQBEMObject::ElastodynamicsCQBEMObject(std::shared_ptr<const Elast
 ElastodynamicsCQBEMObject::~ElastodynamicsCQBEMObject() {
     for (double *q : {d_u_, d_p_, d_acc_, d_b_, d_x_, d_y_})
         if (q) (void)hipFree(q);
+    if (d_refs_) (void)hipFree(d_refs_);
+}
+
+void ElastodynamicsCQBEMObject::product(const double *dense, const DeviceCsr *compressed, const std::vector<double> &x,
+                                        std::vector<double> &y) {
+    const ElastodynamicsCQBEMMatrixCollection &mc = *mc_;
+    const auto &res = mc.resident();
+    const int64_t n3 = 3 * mc.num_vertices;
+    y.assign(n3, 0.0);
+    if (!compressed) {
+        if (hipMemcpy(d_x_, x.data(), sizeof(double) * n3, hipMemcpyHostToDevice) != hipSuccess)
+            throw std::runtime_error("hipMemcpy x failed");
+        if (cqbem_matvec(res.ctx, n3, dense, d_x_, d_y_) != CQBEM_OK) throw std::runtime_error(cqbem_last_error(res.ctx));
+        if (hipMemcpy(y.data(), d_y_, sizeof(double) * n3, hipMemcpyDeviceToHost) != hipSuccess)
+            throw std::runtime_error("hipMemcpy y failed");
+        return;
+    }
+    const std::vector<double> xc = wavelet_decompose_vec(x.data(), n3);
+    std::vector<double> yc(xc.size());
+    if (hipMemcpy(d_x_, xc.data(), sizeof(double) * xc.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        launch_csr_matvec(compressed->rowptr, compressed->col, compressed->val, d_x_, compressed->rows, d_y_, nullptr) !=
+            hipSuccess ||
+        hipMemcpy(yc.data(), d_y_, sizeof(double) * yc.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        throw std::runtime_error("compressed product failed");
+    y = wavelet_reconstruct(yc.data(), n3);
 }
 
 bool ElastodynamicsCQBEMObject::fail(const std::string &msg) {
@@ -171,23 +196,47 @@ bool ElastodynamicsCQBEMObject::estimate_next_state() {
     b.assign(n3, 0.0);
     try {
         const auto &res = mc.resident();
+        const bool cmp = mc.use_compressed_matrices;
         if (!d_b_) {
             cap_ = 2 * std::max<int64_t>(1, L - 1);
-            if (hipMalloc(&d_u_, sizeof(double) * cap_ * n3) != hipSuccess ||
-                hipMalloc(&d_p_, sizeof(double) * cap_ * n3) != hipSuccess ||
+            w_ = cmp ? res.vlen : n3;
+            const int64_t vmax = std::max(n3, w_);
+            if (hipMalloc(&d_u_, sizeof(double) * cap_ * w_) != hipSuccess ||
+                hipMalloc(&d_p_, sizeof(double) * cap_ * w_) != hipSuccess ||
                 hipMalloc(&d_acc_, sizeof(double) * cap_ * 6) != hipSuccess || hipMalloc(&d_b_, sizeof(double) * n3) != hipSuccess ||
-                hipMalloc(&d_x_, sizeof(double) * n3) != hipSuccess || hipMalloc(&d_y_, sizeof(double) * n3) != hipSuccess)
+                hipMalloc(&d_x_, sizeof(double) * vmax) != hipSuccess || hipMalloc(&d_y_, sizeof(double) * vmax) != hipSuccess ||
+                hipMalloc(&d_refs_, sizeof(CsrRef) * 2 * std::max(1, L - 1)) != hipSuccess)
                 return fail("hipMalloc of the step buffers failed");
         }
         // b = sum_{i=1..m} (B_i acc + H_i u + G_i p) over the newest m = min(count, L - 1) histories, H_i as the
-        // object reads it (negated): cqbem_history subtracts the assembled H_i (object.cpp:42-49)
+        // object reads it (negated): cqbem_history subtracts the assembled H_i (object.cpp:42-49).  With
+        // compressed matrices B stays dense and the H, G terms are summed in the compressed space, then
+        // mapped back once (object.cpp:31-41).
         if (count_ > 0) {
             const auto t0 = Clock::now();
-            if (cqbem_history(res.ctx, n3, L, res.G, res.H, res.B, d_p_ + start_ * n3, d_u_ + start_ * n3, d_acc_ + start_ * 6,
+            if (cqbem_history(res.ctx, n3, L, cmp ? nullptr : res.G, cmp ? nullptr : res.H, res.B,
+                              cmp ? nullptr : d_p_ + start_ * w_, cmp ? nullptr : d_u_ + start_ * w_, d_acc_ + start_ * 6,
                               (int32_t)count_, d_b_) != CQBEM_OK)
                 return fail(cqbem_last_error(res.ctx));
             if (hipMemcpy(b.data(), d_b_, sizeof(double) * n3, hipMemcpyDeviceToHost) != hipSuccess)
                 return fail("hipMemcpy b failed");
+            if (cmp) {
+                const int64_t m = std::min<int64_t>(count_, L - 1);
+                std::vector<CsrRef> refs;
+                for (int64_t i = 1; i <= m; i++) {
+                    const int64_t row = start_ + count_ - i;
+                    const DeviceCsr &Hc = res.Hc[i], &Gc = res.Gc[i];
+                    refs.push_back({Hc.rowptr, Hc.col, Hc.val, d_u_ + row * w_});
+                    refs.push_back({Gc.rowptr, Gc.col, Gc.val, d_p_ + row * w_});
+                }
+                std::vector<double> bc(w_);
+                if (hipMemcpy(d_refs_, refs.data(), sizeof(CsrRef) * refs.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                    launch_csr_matvec_sum(d_refs_, (int)refs.size(), w_, d_y_, nullptr) != hipSuccess ||
+                    hipMemcpy(bc.data(), d_y_, sizeof(double) * w_, hipMemcpyDeviceToHost) != hipSuccess)
+                    return fail("compressed history sum failed");
+                const std::vector<double> rec = wavelet_reconstruct(bc.data(), n3);
+                for (int64_t r = 0; r < n3; r++) b[r] += rec[r];
+            }
             const double t = since(t0);
             phase_ms[0] += t, t_parts += t;
         }
@@ -207,11 +256,7 @@ bool ElastodynamicsCQBEMObject::estimate_next_state() {
     try {
         const auto t0 = Clock::now();
         const auto &res = mc.resident();
-        if (hipMemcpy(d_x_, b.data(), sizeof(double) * n3, hipMemcpyHostToDevice) != hipSuccess)
-            return fail("hipMemcpy b failed");
-        if (cqbem_matvec(res.ctx, n3, res.H_inv, d_x_, d_y_) != CQBEM_OK) return fail(cqbem_last_error(res.ctx));
-        if (hipMemcpy(u_unconstrained.data(), d_y_, sizeof(double) * n3, hipMemcpyDeviceToHost) != hipSuccess)
-            return fail("hipMemcpy u failed");
+        product(res.H_inv, mc.use_compressed_matrices ? &res.H_inv_c : nullptr, b, u_unconstrained);  // object.cpp:56-60
         const double t = since(t0);
         phase_ms[1] += t, t_parts += t;
     } catch (const std::exception &e) {
@@ -294,15 +339,18 @@ void ElastodynamicsCQBEMObject::push_history(const std::vector<double> &u, const
     if (count_ >= H) start_++, count_--;
     if (start_ + count_ == cap_) {
         if (count_ > 0 &&
-            (hipMemcpy(d_u_, d_u_ + start_ * n3, sizeof(double) * count_ * n3, hipMemcpyDeviceToDevice) != hipSuccess ||
-             hipMemcpy(d_p_, d_p_ + start_ * n3, sizeof(double) * count_ * n3, hipMemcpyDeviceToDevice) != hipSuccess ||
+            (hipMemcpy(d_u_, d_u_ + start_ * w_, sizeof(double) * count_ * w_, hipMemcpyDeviceToDevice) != hipSuccess ||
+             hipMemcpy(d_p_, d_p_ + start_ * w_, sizeof(double) * count_ * w_, hipMemcpyDeviceToDevice) != hipSuccess ||
              hipMemcpy(d_acc_, d_acc_ + start_ * 6, sizeof(double) * count_ * 6, hipMemcpyDeviceToDevice) != hipSuccess))
             throw std::runtime_error("history slide failed");
         start_ = 0;
     }
     const int64_t row = start_ + count_;
-    if (hipMemcpy(d_u_ + row * n3, u.data(), sizeof(double) * n3, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(d_p_ + row * n3, pv.data(), sizeof(double) * n3, hipMemcpyHostToDevice) != hipSuccess ||
+    // compressed matrices keep compressed histories (object.cpp:216-222)
+    const std::vector<double> uc = mc.use_compressed_matrices ? wavelet_decompose_vec(u.data(), n3) : u;
+    const std::vector<double> pc = mc.use_compressed_matrices ? wavelet_decompose_vec(pv.data(), n3) : pv;
+    if (hipMemcpy(d_u_ + row * w_, uc.data(), sizeof(double) * w_, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_p_ + row * w_, pc.data(), sizeof(double) * w_, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_acc_ + row * 6, acc, sizeof(double) * 6, hipMemcpyHostToDevice) != hipSuccess)
         throw std::runtime_error("history upload failed");
     count_++;
@@ -321,11 +369,7 @@ bool ElastodynamicsCQBEMObject::confirm_next_state() {
             // u_p = u_p_mat p on the device (object.cpp:136); V_global from the constraint solve's linearisation
             const auto &res = mc.resident();
             const auto t0 = Clock::now();
-            if (hipMemcpy(d_x_, p.data(), sizeof(double) * n3, hipMemcpyHostToDevice) != hipSuccess)
-                return fail("hipMemcpy p failed");
-            if (cqbem_matvec(res.ctx, n3, res.u_p, d_x_, d_y_) != CQBEM_OK) return fail(cqbem_last_error(res.ctx));
-            if (hipMemcpy(up.data(), d_y_, sizeof(double) * n3, hipMemcpyDeviceToHost) != hipSuccess)
-                return fail("hipMemcpy u_p failed");
+            product(res.u_p, mc.use_compressed_matrices ? &res.u_p_c : nullptr, p, up);  // object.cpp:131-136
             const double t = since(t0);
             phase_ms[2] += t, t_parts += t;
             for (int64_t r = 0; r < n3; r++) u[r] = up[r] + u_unconstrained[r];

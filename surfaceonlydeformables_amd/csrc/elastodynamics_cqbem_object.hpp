@@ -65,12 +65,17 @@ class ElastodynamicsCQBEMObject {
   private:
     bool fail(const std::string &msg);
     void push_history(const std::vector<double> &u, const std::vector<double> &pv, const double acc[6]);
+    // y = M x through the resident matrix: dense (cqbem_matvec), or compressed: reconstruct(M_c decompose(x))
+    // (CompressedMatrixVectorProduct, wavelets.hpp:326-370).  Throws on errors.
+    void product(const double *dense, const DeviceCsr *compressed, const std::vector<double> &x, std::vector<double> &y);
 
     std::shared_ptr<const ElastodynamicsCQBEMMatrixCollection> mc_;
     std::string error_;
     // device histories: rows [start_, start_ + count_) of 2 (L - 1)-row buffers, oldest first
+    // (compressed vectors of 2 np - 2 entries with compressed matrices: w_ is the row width)
     double *d_u_ = nullptr, *d_p_ = nullptr, *d_acc_ = nullptr, *d_b_ = nullptr, *d_x_ = nullptr, *d_y_ = nullptr;
-    int64_t cap_ = 0, start_ = 0, count_ = 0;
+    CsrRef *d_refs_ = nullptr;
+    int64_t cap_ = 0, start_ = 0, count_ = 0, w_ = 0;
 };
 
 // Rodrigues' formula for the rotation by |w| about w / |w| (physics_base_object.cpp:85-95); identity for w = 0.

@@ -7,6 +7,10 @@ matrices the facade dumped.  The product runs the history sums and u_p p on the 
 rocSOLVER and the drift rotation by a polar iteration; the oracle uses LAPACK solves and an SVD.  Bound:
 relative error 1e-9 per quantity and step (the solves' rounding, amplified by cond(H_0) ~ 1e2..1e3).
 
+With simulation.compression_ratio the step runs on wavelet-compressed G_n, H_n, u_p and H_0^-1 (the reference's
+compressed branches); the oracle compresses the same dumped matrices with oracle/cqbem_wavelet.py (its H_0^-1
+from LAPACK), and the bound is the same.
+
 Closed form: with no traction the body stays undeformed, its frame follows the semi-implicit Euler
 free fall, v_k = v_0 + k dt g, t_k = t_0 + k dt v_0 + dt^2 g k (k + 1) / 2, and spins at its constant
 angular velocity w: R_k = exp(k dt [w]_x) (scipy's matrix exponential, independent of the Rodrigues
@@ -53,7 +57,8 @@ def simulate(tmp_path, V, F, obj, tractions, **sim):
          "B": np.fromfile(tmp_path / "out_B.bin").reshape(L, n3, 6),
          "V": post("V", (nv, 3)), "F": np.asarray(F), "original_cm": post("original_cm", (3,)),
          "u_p": post("u_p", (n3, n3)), "rotation_removal_weights": post("rotation_removal_weights", (nv,)),
-         "update_local_frame": sim.get("update_local_frame", True)}
+         "update_local_frame": sim.get("update_local_frame", True),
+         "compression_ratio": sim.get("compression_ratio", 0.0)}
     for k in ("translational_acc_mat", "rotational_acc_mat", "B_trans_inv_H", "B_euler_inv_H",
               "B_trans_inv_G_prime", "B_euler_inv_G_prime"):
         m[k] = post(k, (3, n3))
@@ -144,15 +149,17 @@ def test_free_fall_closed_form(tmp_path, mesh):
         assert np.allclose(s["V_global"], m["V"] @ s["rotation"].T + s["translation"], rtol=0, atol=1e-12)
 
 
-@pytest.mark.parametrize("case", ["local_frame", "no_local_frame", "fixed"])
+@pytest.mark.parametrize("case", ["local_frame", "no_local_frame", "fixed", "compressed", "compressed_no_local_frame"])
 def test_object_step_matches_oracle(tmp_path, mesh, case):
     V, F = mesh
     steps = 48
     tractions = pressure_pulse(V, F, steps, 2, 7, 300.0)
     obj = {"translational_vel": [0.1, 0.0, 0.05], "rotational_vel": [0.0, 2.0, 1.0], "translation": [0.0, 0.5, 0.0]}
     sim = {}
-    if case == "no_local_frame":
+    if case.endswith("no_local_frame"):
         sim["update_local_frame"] = False
+    if case.startswith("compressed"):
+        sim["compression_ratio"] = 0.5  # wavelet-compressed G_n, H_n, u_p and H_0^-1
     if case == "fixed":
         obj["is_fixed"] = True
     m, traj, info = simulate(tmp_path, V, F, obj, tractions, **sim)
