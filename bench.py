@@ -223,7 +223,8 @@ def time_stepping_main(args):
     """CQ time steps of one body on the icosphere-L4 matrices (L = 32) through the C++ facade: the collection
     driver assembles, post-processes, uploads the matrices once, then steps an ElastodynamicsCQBEMObject
     (history sums, u_p p and the H_0^-1 product on the GPU, drift elimination on the host).
-    The first three steps carry a pressure pulse (constrained steps), the rest none."""
+    The first three steps carry a pressure pulse (constrained steps), the rest none.  With --compression-ratio the
+    matrices are wavelet-compressed first (upload_ms then includes the compression of 64 matrices)."""
     import subprocess
     import tempfile
     from surfaceonlydeformables_amd import mesh as M
@@ -236,7 +237,8 @@ def time_stepping_main(args):
         cfg = {"enable_cuda": True, "mesh": {"file": obj},
                "material": {"rho": params.rho, "mu": params.rho * params.c2 ** 2, "nu": params.nu},
                "simulation": {"dt": params.dt, "gaussian_quadrature_order": 5, "quadrature_subdivision": 2,
-                              "multistep_method": "BDF2", "use_galerkin": False, "cq_length": L},
+                              "multistep_method": "BDF2", "use_galerkin": False, "cq_length": L,
+                              "compression_ratio": args.compression_ratio},
                "object": {"translational_vel": [0.1, 0.0, 0.0], "rotational_vel": [0.0, 1.0, 0.0]}}
         scene = os.path.join(tmp, "scene.json")
         with open(scene, "w") as fh:
@@ -265,10 +267,12 @@ def time_stepping_main(args):
     per = (time.perf_counter() - t0) / len(sample)
     print(json.dumps({"metric": "CQ time step of one body on resident matrices (history + H_0 solve + drift "
                                 "elimination)", "value": info["step_ms"], "unit": "ms", "higher_is_better": False,
+                      "compression_ratio": args.compression_ratio,
                       "steps": steps, "step_phases_ms": info["step_phases_ms"], "first_step_ms": info["first_step_ms"],
                       "upload_ms": info["upload_ms"],
                       "assembly_ms": info["ms"], "post_process_ms": info["factor_ms"],
-                      "history_matrices_GB": 2 * (L - 1) * n3 * n3 * 8 / 1e9,
+                      "history_matrices_GB": (2 * (L - 1) * n3 * n3 * 8 / 1e9 if not args.compression_ratio else
+                                              2 * (L - 1) * int(n3 * n3 * args.compression_ratio) * 12 / 1e9),
                       "cpu_history_ms": {"value": per * 2 * (L - 1) * 1e3, "threads": os.cpu_count(),
                                          "sample": f"numpy {n3} x {n3} GEMV, 4 matrices timed, x {2 * (L - 1)}"},
                       "config": {"workload": desc, "nv": nv, "L": L}, "dtype": "f64", "data": "synthetic"}),
@@ -292,6 +296,8 @@ def main():
     ap.add_argument("--duration", type=float, default=10.0, help="serving stream length in seconds")
     ap.add_argument("--time-stepping", action="store_true",
                     help="time steps of one icosphere-L4 body on its resident matrices instead of the assembly line")
+    ap.add_argument("--compression-ratio", type=float, default=0.0,
+                    help="with --time-stepping: wavelet-compressed matrices keeping this fraction of coefficients")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the CQBEM precompute of one object runs on one MI355X "
