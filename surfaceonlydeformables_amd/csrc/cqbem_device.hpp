@@ -14,6 +14,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "cqbem_math_tables.hpp"
+
 namespace cqbem {
 
 struct cd {
@@ -143,6 +145,92 @@ __device__ inline double rsqrt_nr(double d) {
     return fma(0.5 * y, e, y);
 }
 
+// ---------------------------------------------------------------------------
+// Table-driven variants (P. T. P. Tang, "Table-driven implementation of the
+// exponential function in IEEE floating-point arithmetic", ACM TOMS 15, 1989;
+// the same scheme for sin/cos): the reduction goes to |r| <= ln2/128 and
+// |r| <= pi/64 against 64-entry tables of 2^(j/64) and sin, cos(j pi/32) held as
+// double-double (hi, lo) pairs, so short series suffice and the table value is
+// added last: e^{-a}, sin and cos stay within about 0.52 ulp (of 1 for sin, cos).
+// The kernels that use them copy the tables to LDS first (load_math_tables).
+// ---------------------------------------------------------------------------
+struct MathTables {
+    double2 e[64];  // 2^(j/64): hi, lo
+    double4 t[64];  // sin(j pi/32) hi, lo, cos(j pi/32) hi, lo
+};
+
+static __constant__ double kExpTable[64][2] = CQBEM_EXP_TABLE;
+static __constant__ double kTrigTable[64][4] = CQBEM_TRIG_TABLE;
+
+// Copies the tables (constant memory) into `s`; every thread of the block calls it, a barrier must follow.
+__device__ inline void load_math_tables(MathTables *s) {
+    const auto &kExp = kExpTable;
+    const auto &kTrig = kTrigTable;
+    for (int k = threadIdx.x; k < 64; k += blockDim.x) {
+        s->e[k] = make_double2(kExp[k][0], kExp[k][1]);
+        s->t[k] = make_double4(kTrig[k][0], kTrig[k][1], kTrig[k][2], kTrig[k][3]);
+    }
+}
+
+// e^{-a}, a >= -700, as exp_neg.  k = rint(t 64/ln2), t = -a clamped to [-740, 700];
+// r = t - k ln2/64 (the k hi product and the first subtraction exact); e^r - 1 through r^6
+// (next term < 3e-20); 2^(k/64) = 2^(k >> 6) T[k & 63], and T_hi + (T_hi p + T_lo) last.
+__host__ __device__ inline double exp_neg_tab(double a, const double2 *T) {
+#pragma clang fp contract(off)
+    const double t = fmin(fmax(-a, -740.0), 700.0);
+    const double k = rint(t * CQBEM_64_OVER_LN2);
+    const double r1 = fma(-k, CQBEM_LN2_OVER_64_HI, t);
+    const double r = fma(-k, CQBEM_LN2_OVER_64_LO, r1);
+    double q = fma(r, 1.0 / 720.0, 1.0 / 120.0);
+    q = fma(q, r, 1.0 / 24.0);
+    q = fma(q, r, 1.0 / 6.0);
+    q = fma(q, r, 0.5);
+    const double p = fma(r * r, q, r);
+    const int ki = (int)k;
+    const double2 e = T[ki & 63];
+    return ldexp(e.x + fma(e.x, p, e.y), ki >> 6);
+}
+
+// sin y, cos y for |y| < kTrigArgMax, as cq_sincos.  k = rint(32 y/pi); r + rr = y - k pi/32 to
+// ~1e-28 (three-part pi/32, the first two products exact, TwoSum on the subtraction; |rr| reaches
+// 1e-12 at |k| ~ 1e7, so rr enters to first order, sin(r + rr) = sin r + rr cos r and
+// cos(r + rr) - 1 = cos r - 1 - rr sin r, the rr^2 terms below 1e-24); sin r through r^9 and
+// cos r - 1 through r^8 (next terms < 1e-19); then sin(a + r) = S + (S (cos r - 1) + C sin r) and
+// cos(a + r) = C + (C (cos r - 1) - S sin r) with a = (k mod 64) pi/32 and the table's lo parts
+// added inside the brackets.
+__host__ __device__ inline void sincos_tab(double y, const double4 *T, double *s, double *c) {
+#pragma clang fp contract(off)
+    const double k = rint(y * CQBEM_32_OVER_PI);
+    const double r1 = fma(-k, CQBEM_PI_OVER_32_A, y);
+    const double tb = k * CQBEM_PI_OVER_32_B;
+    const double r = r1 - tb;
+    const double bb = r - r1;
+    double rr = (r1 - (r - bb)) - (tb + bb);
+    rr = fma(-k, CQBEM_PI_OVER_32_C, rr);
+    const double x = r * r;
+    double ps = fma(x, 1.0 / 362880.0, -1.0 / 5040.0);
+    ps = fma(ps, x, 1.0 / 120.0);
+    ps = fma(ps, x, -1.0 / 6.0);
+    const double s0 = (r * x) * ps;  // sin r - r
+    double pc = fma(x, 1.0 / 40320.0, -1.0 / 720.0);
+    pc = fma(pc, x, 1.0 / 24.0);
+    pc = fma(pc, x, -0.5);
+    const double c0 = x * pc;                        // cos r - 1
+    const double sr = r + fma(rr, 1.0 + c0, s0);     // sin(r + rr)
+    const double cm1 = fma(-rr, r + s0, c0);         // cos(r + rr) - 1
+    const double4 e = T[(int)k & 63];
+    *s = e.x + fma(e.z, sr, fma(e.x, cm1, e.y));
+    *c = e.z + fma(-e.x, sr, fma(e.z, cm1, e.w));
+}
+
+// exp(-z), Re z >= 0, through the tables
+__host__ __device__ inline cd cexp_neg_tab(cd z, const MathTables *M) {
+    const double m = exp_neg_tab(z.x, M->e);
+    double sn, cs;
+    sincos_tab(z.y, M->t, &sn, &cs);
+    return cd{m * cs, -m * sn};
+}
+
 // exp(-z), Re z >= 0
 __device__ inline cd cexp_neg(cd z) {
     const double m = exp_neg(z.x);
@@ -183,6 +271,18 @@ __device__ inline Expo expo(double r, const Freq &f) {
     const cd z1 = f.sc1 * r, z2 = f.sc2 * r;
     E.e1 = cexp_neg(z1);
     E.e2 = cexp_neg(z2);
+    const cd A1 = (z1 + cd{1.0, 0.0}) * E.e1 * f.is2;
+    E.A2 = (z2 + cd{1.0, 0.0}) * E.e2 * f.is2;
+    E.dA = A1 - E.A2;
+    return E;
+}
+
+// expo with the table-driven exponentials (tables in LDS)
+__device__ inline Expo expo(double r, const Freq &f, const MathTables *M) {
+    Expo E;
+    const cd z1 = f.sc1 * r, z2 = f.sc2 * r;
+    E.e1 = cexp_neg_tab(z1, M);
+    E.e2 = cexp_neg_tab(z2, M);
     const cd A1 = (z1 + cd{1.0, 0.0}) * E.e1 * f.is2;
     E.A2 = (z2 + cd{1.0, 0.0}) * E.e2 * f.is2;
     E.dA = A1 - E.A2;

@@ -182,6 +182,9 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
         x0 = g.V[3 * i], x1 = g.V[3 * i + 1], x2 = g.V[3 * i + 2];
     }
     for (int k = tid; k < kPairWords * kPairPV; k += kPairThreads) acc[k] = 0.0;
+    __shared__ MathTables mtab;  // tables of the exponentials
+    load_math_tables(&mtab);
+    __syncthreads();
 
     for (int pass = patch_pass[pc]; pass < patch_pass[pc + 1]; ++pass) {
         const int2 info = pass_info[pass];
@@ -234,7 +237,7 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
                 const double ir = rsqrt_nr(d2), r = d2 * ir, ir2 = ir * ir;
                 const double h[3] = {r0 * ir, r1 * ir, r2 * ir};
                 const double wL = Qo.w * ar;
-                const Expo E = expo(r, f);
+                const Expo E = expo(r, f, &mtab);
                 cd al, be, c1, c2, c4;
                 if (DG) g_coef(E, ir, ir2, m, al, be);
                 if (DH) h_coef(E, ir, ir2, f, m, c1, c2, c4);
@@ -349,6 +352,9 @@ __global__ void __launch_bounds__(256) k_colloc_b(Geom g, const double4 *__restr
                                                    const double4 *__restrict__ tpts, cd *__restrict__ Bt,
                                                    cd *__restrict__ Ba, long long nv3) {
     __shared__ double red[4][kBWords];
+    __shared__ MathTables mtab;  // tables of the exponentials
+    load_math_tables(&mtab);
+    __syncthreads();
     const int l = blockIdx.x;
     const int i = row0 + blockIdx.y;
     const Freq f = freqs[l0 + l];
@@ -389,7 +395,7 @@ __global__ void __launch_bounds__(256) k_colloc_b(Geom g, const double4 *__restr
             const double ir = rsqrt_nr(d2), r = d2 * ir, ir2 = ir * ir;
             const double h[3] = {r0 * ir, r1 * ir, r2 * ir};
             const double w = Q.w * ar;
-            const Expo E = expo(r, f);
+            const Expo E = expo(r, f, &mtab);
             cd t1, t2, u1, u12, u3;
             b_coef(E, ir, ir2, f, m, t1, t2, u1, u12, u3);
             const double rn = h[0] * n[0] + h[1] * n[1] + h[2] * n[2];
