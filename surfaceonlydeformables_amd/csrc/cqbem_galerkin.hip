@@ -290,12 +290,12 @@ template <int MODE> struct SingAcc {
 
 template <int MODE>
 __device__ __forceinline__ void sing_eval(const double x[3], const double y[3], const double Lx[3], const double Ly[3], double wt,
-                                 const double n[3], const Freq &f, const Mat &m, cd *acc) {
+                                 const double n[3], const Freq &f, const Mat &m, const MathTables *M, cd *acc) {
     const double r0 = y[0] - x[0], r1 = y[1] - x[1], r2 = y[2] - x[2];
     const double d2 = r0 * r0 + r1 * r1 + r2 * r2;
     const double ir = rsqrt_nr(d2), r = d2 * ir, ir2 = ir * ir;
     const double h[3] = {r0 * ir, r1 * ir, r2 * ir};
-    const Expo E = expo(r, f);
+    const Expo E = expo(r, f, M);
     if (MODE == 0) {
         cd al, be;
         g_coef(E, ir, ir2, m, al, be);
@@ -398,6 +398,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
     const int pi = pair0 + blockIdx.x, l = blockIdx.y;
     const Pair P = pairs[pi];
     const Freq f = freqs[l];
+    __shared__ MathTables mtab;  // tables of the exponentials
+    load_math_tables(&mtab);
+    __syncthreads();
     const double n[3] = {g.N[3 * P.f2], g.N[3 * P.f2 + 1], g.N[3 * P.f2 + 2]};
     // x(xh) = px + dx0 xh0 + dx1 xh1 and y likewise; barycentrics (1-h0, h0-h1, h1)
     double px[3], dx0[3], dx1[3], dy0[3], dy1[3], jac;
@@ -438,7 +441,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
             #pragma unroll
             for (int k = 0; k < 3; k++) x[k] = px[k] + dx0[k] * a0 + dx1[k] * a1, y[k] = px[k] + dy0[k] * b0 + dy1[k] * b1;
             const double Lx[3] = {1.0 - a0, a0 - a1, a1}, Ly[3] = {1.0 - b0, b0 - b1, b1};
-            sing_eval<MODE>(x, y, Lx, Ly, wt, n, f, m, acc);
+            sing_eval<MODE>(x, y, Lx, Ly, wt, n, f, m, &mtab, acc);
         }
     }
     // wave reduction; lane 0 expands and writes
@@ -524,12 +527,13 @@ template <int MODE> struct QuadK {
 
 template <int MODE>
 __device__ __forceinline__ void quad_point(const double x[3], const double y[3], const double Lx[3], const double Ly[3], double wt,
-                                           const double n[3], const Freq &f, const Mat &m, QuadK<MODE> &K) {
+                                           const double n[3], const Freq &f, const Mat &m, const MathTables *M,
+                                           QuadK<MODE> &K) {
     const double r0 = y[0] - x[0], r1 = y[1] - x[1], r2 = y[2] - x[2];
     const double d2 = r0 * r0 + r1 * r1 + r2 * r2;
     const double ir = rsqrt_nr(d2), r = d2 * ir, ir2 = ir * ir;
     const double h[3] = {r0 * ir, r1 * ir, r2 * ir};
-    const Expo E = expo(r, f);
+    const Expo E = expo(r, f, M);
     if (MODE == 0) {
         cd al, be;
         g_coef(E, ir, ir2, m, al, be);
@@ -586,6 +590,9 @@ __global__ void __launch_bounds__(64) k_gal_sing_quad(Geom g, const Pair *__rest
     const int pi = pair0 + blockIdx.x, l = blockIdx.y;
     const Pair P = pairs[pi];
     const Freq f = freqs[l];
+    __shared__ MathTables mtab;  // tables of the exponentials
+    load_math_tables(&mtab);
+    __syncthreads();
     const double n[3] = {g.N[3 * P.f2], g.N[3 * P.f2 + 1], g.N[3 * P.f2 + 2]};
     double px[3], dx0[3], dx1[3], dy0[3], dy1[3], jac;
     {
@@ -634,7 +641,7 @@ __global__ void __launch_bounds__(64) k_gal_sing_quad(Geom g, const Pair *__rest
             for (int k = 0; k < 3; k++) x[k] = px[k] + dx0[k] * a0 + dx1[k] * a1, y[k] = px[k] + dy0[k] * b0 + dy1[k] * b1;
             const double Lx[3] = {1.0 - a0, a0 - a1, a1}, Ly[3] = {1.0 - b0, b0 - b1, b1};
             QuadK<MODE> K;
-            quad_point<MODE>(x, y, Lx, Ly, wt, n, f, m, K);
+            quad_point<MODE>(x, y, Lx, Ly, wt, n, f, m, &mtab, K);
             quad_acc<MODE>(K, ci, cj, cv, acc);
 #pragma unroll 1
             for (int rot = 0; rot < 3; rot++) {  // the next lane's point, then the one after, then the last
