@@ -279,6 +279,71 @@ def time_stepping_main(args):
           flush=True)
 
 
+REFERENCE_SCENES = ("bunny_friction", "cube_drop", "cube_collide", "glued_cubes")
+
+
+def reference_scene_main(args):
+    """The elastodynamicsCQBEM groups of one of the reference's shipped scenes (data/<name>.json, meshes from
+    tests/golden) through the C++ facade: init (mesh preparation, assembly, post-processing), the upload of the
+    resident matrices (and their wavelet compression when the scene sets compression_ratio), then time steps of one
+    body with a 3-step pressure pulse.  The scene's matrix cache folder is dropped, so every run assembles."""
+    import subprocess
+    import tempfile
+    from surfaceonlydeformables_amd import mesh as M
+    golden = os.path.join(ROOT, "tests", "golden")
+    with open(os.path.join(golden, "scenes", args.reference_scene + ".json")) as fh:
+        scene = json.load(fh)
+    drv = os.path.join(ROOT, "surfaceonlydeformables_amd", "collection_driver")
+    results = []
+    for grp in scene["physics_object_groups"]:
+        cfg = grp["physics_object_group"]
+        if cfg.get("type") != "elastodynamicsCQBEM":
+            continue
+        cfg = json.loads(json.dumps(cfg))
+        cfg["mesh"]["file"] = os.path.join(golden, os.path.basename(cfg["mesh"]["file"]))
+        cfg["simulation"].pop("coeffs_folder_path", None)
+        cfg["enable_cuda"] = True
+        cfg["gravity"] = scene.get("gravitational_constant", [0.0, -9.8, 0.0])
+        with tempfile.TemporaryDirectory() as tmp:
+            path = os.path.join(tmp, "group.json")
+            with open(path, "w") as fh:
+                json.dump(cfg, fh)
+            # vertex count after the scene's mesh preparation (split included)
+            r = subprocess.run([drv, "--mesh", path], capture_output=True, text=True)
+            if r.returncode != 0:
+                sys.stderr.write(r.stderr)
+                sys.exit(1)
+            Vp = np.array(json.loads(r.stdout)["V"]).reshape(-1, 3)
+            nv = len(Vp)
+            steps = max(16, 4 * args.steps)
+            Vc = Vp - Vp.mean(axis=0)
+            p = np.zeros((steps, nv, 3))
+            mask = Vc[:, 1] < Vc[:, 1].min() + 0.2 * (Vc[:, 1].max() - Vc[:, 1].min())
+            p[:3, mask, 1] = 1e3
+            tr = os.path.join(tmp, "tractions.bin")
+            p.reshape(steps, 3 * nv).tofile(tr)
+            print(f"reference scene {args.reference_scene}: group with {nv} vertices, {steps} steps", file=sys.stderr,
+                  flush=True)
+            r = subprocess.run([drv, "--simulate", path, str(steps), tr, "-"], capture_output=True, text=True)
+            if r.returncode != 0:
+                sys.stderr.write(r.stderr)
+                sys.exit(1)
+            info = json.loads(r.stdout.strip().splitlines()[-1])
+        sim = cfg["simulation"]
+        results.append({"mesh": os.path.basename(cfg["mesh"]["file"]), "nv": nv, "L": info["L"],
+                        "formulation": "Galerkin" if sim.get("use_galerkin") else "collocation",
+                        "multistep_method": sim.get("multistep_method"),
+                        "traction_discontinuity": bool(cfg["mesh"].get("enable_traction_discontinuity", False)),
+                        "compression_ratio": sim.get("compression_ratio", 0.0),
+                        "init_assembly_ms": info["ms"], "init_post_process_ms": info["factor_ms"],
+                        "resident_upload_ms": info["upload_ms"], "step_ms": info["step_ms"],
+                        "step_phases_ms": info["step_phases_ms"],
+                        "G+H_entries_per_s": 2.0 * (3 * nv) ** 2 * info["L"] / (info["ms"] / 1e3)})
+    print(json.dumps({"metric": f"reference scene {args.reference_scene}: facade init and time step of its "
+                                "elastodynamicsCQBEM bodies", "groups": results, "dtype": "f64",
+                      "data": "the reference's scene file and mesh"}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -296,6 +361,8 @@ def main():
     ap.add_argument("--duration", type=float, default=10.0, help="serving stream length in seconds")
     ap.add_argument("--time-stepping", action="store_true",
                     help="time steps of one icosphere-L4 body on its resident matrices instead of the assembly line")
+    ap.add_argument("--reference-scene", choices=REFERENCE_SCENES,
+                    help="the deformable bodies of one of the reference's shipped scenes through the C++ facade")
     ap.add_argument("--compression-ratio", type=float, default=0.0,
                     help="with --time-stepping: wavelet-compressed matrices keeping this fraction of coefficients")
     args = ap.parse_args()
@@ -310,6 +377,8 @@ def main():
         return serving_main(args)
     if args.time_stepping:
         return time_stepping_main(args)
+    if args.reference_scene:
+        return reference_scene_main(args)
 
     import torch
     from surfaceonlydeformables_amd import abi
