@@ -2,6 +2,8 @@
 // solve on the host.  See cqbem_dense.hpp.
 #include "cqbem_dense.hpp"
 
+#include "host_parallel.hpp"
+
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -60,10 +62,12 @@ bool hip_ok(hipError_t e, const char *what, std::string &err) {
 // Row-major (rows x cols) -> column-major, i.e. a transpose into `out`.
 void to_colmajor(const double *in, int64_t rows, int64_t cols, double *out) {
     constexpr int64_t kB = 64;
-    for (int64_t r0 = 0; r0 < rows; r0 += kB)
-        for (int64_t c0 = 0; c0 < cols; c0 += kB)
-            for (int64_t r = r0; r < std::min(rows, r0 + kB); r++)
-                for (int64_t c = c0; c < std::min(cols, c0 + kB); c++) out[c * rows + r] = in[r * cols + c];
+    parallel_for((rows + kB - 1) / kB, [&](int64_t b0, int64_t b1) {
+        for (int64_t r0 = b0 * kB; r0 < std::min(rows, b1 * kB); r0 += kB)
+            for (int64_t c0 = 0; c0 < cols; c0 += kB)
+                for (int64_t r = r0; r < std::min(rows, r0 + kB); r++)
+                    for (int64_t c = c0; c < std::min(cols, c0 + kB); c++) out[c * rows + r] = in[r * cols + c];
+    });
 }
 
 }  // namespace
@@ -185,27 +189,28 @@ std::vector<double> ThinLeastSquares::solve(const double *Y, int64_t r) const {
             Q[j * k + c] -= s;
             for (int64_t i = j + 1; i < m; i++) Q[i * k + c] -= s * qr_[i * k + j];
         }
-    // Z = Q^T Y (k x r), streamed over the rows of Y
+    // Z = Q^T Y (k x r), streamed over the rows of Y; then X = R^-1 Z.  Columns of Y are split over threads.
     std::vector<double> X((size_t)(k * r), 0.0);
-    for (int64_t i = 0; i < m; i++) {
-        const double *y = Y + i * r;
-        for (int c = 0; c < k; c++) {
-            const double q = Q[i * k + c];
-            double *z = X.data() + c * r;
-            for (int64_t t = 0; t < r; t++) z[t] += q * y[t];
+    parallel_for(r, [&](int64_t t0, int64_t t1) {
+        for (int64_t i = 0; i < m; i++) {
+            const double *y = Y + i * r;
+            for (int c = 0; c < k; c++) {
+                const double q = Q[i * k + c];
+                double *z = X.data() + c * r;
+                for (int64_t t = t0; t < t1; t++) z[t] += q * y[t];
+            }
         }
-    }
-    // X = R^-1 Z
-    for (int c = k - 1; c >= 0; c--) {
-        double *x = X.data() + c * r;
-        for (int d = c + 1; d < k; d++) {
-            const double rcd = qr_[c * k + d];
-            const double *xd = X.data() + d * r;
-            for (int64_t t = 0; t < r; t++) x[t] -= rcd * xd[t];
+        for (int c = k - 1; c >= 0; c--) {
+            double *x = X.data() + c * r;
+            for (int d = c + 1; d < k; d++) {
+                const double rcd = qr_[c * k + d];
+                const double *xd = X.data() + d * r;
+                for (int64_t t = t0; t < t1; t++) x[t] -= rcd * xd[t];
+            }
+            const double inv = 1.0 / qr_[c * k + c];
+            for (int64_t t = t0; t < t1; t++) x[t] *= inv;
         }
-        const double inv = 1.0 / qr_[c * k + c];
-        for (int64_t t = 0; t < r; t++) x[t] *= inv;
-    }
+    });
     return X;
 }
 

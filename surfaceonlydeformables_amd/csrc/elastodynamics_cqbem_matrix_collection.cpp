@@ -4,6 +4,7 @@
 #include "elastodynamics_cqbem_matrix_collection.hpp"
 
 #include "cqbem_rules.hpp"
+#include "host_parallel.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -594,7 +595,9 @@ const ElastodynamicsCQBEMMatrixCollection::Resident &ElastodynamicsCQBEMMatrixCo
         fail_up("hipMalloc failed");
     std::vector<double> h((size_t)(n3 * n3));
     for (size_t n = 0; n < L; n++) {
-        for (size_t e = 0; e < h.size(); e++) h[e] = -H_matrices[n].data[e];  // back to the assembled sign
+        parallel_for((int64_t)h.size(), [&](int64_t a, int64_t b) {  // back to the assembled sign
+            for (int64_t e = a; e < b; e++) h[e] = -H_matrices[n].data[e];
+        });
         if (hipMemcpy(r.G + n * n3 * n3, G_matrices[n].data.data(), mat, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(r.H + n * n3 * n3, h.data(), mat, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(r.B + n * n3 * 6, B_matrices[n].data.data(), sizeof(double) * n3 * 6, hipMemcpyHostToDevice) != hipSuccess)
@@ -780,12 +783,19 @@ bool ElastodynamicsCQBEMMatrixCollection::run(const Json &config) {
     last_assembly_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     for (size_t n = 0; n < L; n++) {
         if (!loaded_from_cache) {
-            std::copy(g.begin() + n * n3 * n3, g.begin() + (n + 1) * n3 * n3, G_matrices[n].data.begin());
-            std::copy(h.begin() + n * n3 * n3, h.begin() + (n + 1) * n3 * n3, H_matrices[n].data.begin());
-            if (!folder.empty() && (!save_matrix(H_matrices[n], folder + "/H_" + std::to_string(n) + ".mat") ||
-                                    !save_matrix(G_matrices[n], folder + "/G_" + std::to_string(n) + ".mat")))
-                return fail("cannot write the matrix cache in " + folder);
-            for (double &x : H_matrices[n].data) x = -x;  // matrix_collection.cpp:140
+            const double *gs = g.data() + n * n3 * n3, *hs = h.data() + n * n3 * n3;
+            double *gd = G_matrices[n].data.data(), *hd = H_matrices[n].data.data();
+            if (!folder.empty()) {
+                std::copy(hs, hs + n3 * n3, hd);
+                std::copy(gs, gs + n3 * n3, gd);
+                if (!save_matrix(H_matrices[n], folder + "/H_" + std::to_string(n) + ".mat") ||
+                    !save_matrix(G_matrices[n], folder + "/G_" + std::to_string(n) + ".mat"))
+                    return fail("cannot write the matrix cache in " + folder);
+            }
+            parallel_for(n3 * n3, [&](int64_t a, int64_t e1) {
+                std::copy(gs + a, gs + e1, gd + a);
+                for (int64_t e = a; e < e1; e++) hd[e] = -hs[e];  // matrix_collection.cpp:140
+            });
         }
         std::copy(b.begin() + n * n3 * 6, b.begin() + (n + 1) * n3 * 6, B_matrices[n].data.begin());
     }
@@ -876,15 +886,20 @@ bool ElastodynamicsCQBEMMatrixCollection::post_process() {
     for (int64_t r = 0; r < n3; r++)
         for (int k = 0; k < 3; k++) Bt[3 * r + k] = B_matrices[0](r, k), Be[3 * r + k] = B_matrices[0](r, 3 + k);
     std::vector<double> H0(H_matrices[0].data.size());
-    for (size_t e = 0; e < H0.size(); e++) H0[e] = -H_matrices[0].data[e];
+    parallel_for((int64_t)H0.size(), [&](int64_t a, int64_t b) {
+        for (int64_t e = a; e < b; e++) H0[e] = -H_matrices[0].data[e];
+    });
     u_p = G_matrices[0];  // becomes G', then H_0^-1 G'
     if (update_local_frame)
-        for (int64_t r = 0; r < n3; r++)
-            for (int64_t c = 0; c < n3; c++) {
-                double v = 0;
-                for (int k = 0; k < 3; k++) v += Bt[3 * r + k] * translational_acc_mat(k, c) + Be[3 * r + k] * rotational_acc_mat(k, c);
-                u_p(r, c) += v;
-            }
+        parallel_for(n3, [&](int64_t r0, int64_t r1) {
+            for (int64_t r = r0; r < r1; r++)
+                for (int64_t c = 0; c < n3; c++) {
+                    double v = 0;
+                    for (int k = 0; k < 3; k++)
+                        v += Bt[3 * r + k] * translational_acc_mat(k, c) + Be[3 * r + k] * rotational_acc_mat(k, c);
+                    u_p(r, c) += v;
+                }
+        });
     std::string err;
     if (!B_trans_inv.compute(Bt, n3, 3, err) || !B_euler_inv.compute(Be, n3, 3, err)) return fail(err);
     auto as3 = [&](std::vector<double> x, int64_t cols) {
