@@ -464,6 +464,69 @@ bool load_matrix(Dense &m, const std::string &path) {
     return (bool)in;
 }
 
+// ----------------------------------------------------------------- .cpmat --
+bool save_cpmat(const DeviceCsr &m, const std::string &path) {
+    std::vector<int64_t> rp((size_t)m.rows + 1);
+    std::vector<int32_t> col((size_t)m.nnz);
+    std::vector<double> val((size_t)m.nnz);
+    if (hipMemcpy(rp.data(), m.rowptr, sizeof(int64_t) * rp.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+        (m.nnz && (hipMemcpy(col.data(), m.col, sizeof(int32_t) * m.nnz, hipMemcpyDeviceToHost) != hipSuccess ||
+                   hipMemcpy(val.data(), m.val, sizeof(double) * m.nnz, hipMemcpyDeviceToHost) != hipSuccess)))
+        return false;
+    std::ofstream out(path, std::ios::binary);
+    if (!out) return false;
+    const int32_t N = (int32_t)m.n;
+    const int64_t head[5] = {m.rows, m.rows, m.nnz, m.rows, m.rows};  // rows, cols, nonZeros, outerSize, innerSize
+    out.write(reinterpret_cast<const char *>(&N), sizeof N);
+    out.write(reinterpret_cast<const char *>(head), sizeof head);
+    out.write(reinterpret_cast<const char *>(val.data()), sizeof(double) * m.nnz);
+    std::vector<int32_t> outer(rp.begin(), rp.end() - 1);
+    out.write(reinterpret_cast<const char *>(outer.data()), sizeof(int32_t) * outer.size());
+    out.write(reinterpret_cast<const char *>(col.data()), sizeof(int32_t) * m.nnz);
+    return (bool)out;
+}
+
+bool load_cpmat(DeviceCsr &m, int64_t n, const std::string &path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) return false;
+    int32_t N = 0;
+    int64_t head[5];
+    in.read(reinterpret_cast<char *>(&N), sizeof N);
+    in.read(reinterpret_cast<char *>(head), sizeof head);
+    const int64_t rows = 2 * wavelet_padded(n) - 2, nnz = head[2];
+    if (!in || N != n || head[0] != rows || head[1] != rows || head[3] != rows || head[4] != rows || nnz < 0 ||
+        nnz > rows * rows)
+        return false;
+    std::vector<double> val((size_t)nnz);
+    std::vector<int32_t> outer((size_t)rows), col((size_t)nnz);
+    in.read(reinterpret_cast<char *>(val.data()), sizeof(double) * nnz);
+    in.read(reinterpret_cast<char *>(outer.data()), sizeof(int32_t) * rows);
+    in.read(reinterpret_cast<char *>(col.data()), sizeof(int32_t) * nnz);
+    if (!in) return false;
+    std::vector<int64_t> rp((size_t)rows + 1);
+    for (int64_t r = 0; r < rows; r++) rp[r] = outer[r];
+    rp[rows] = nnz;
+    for (int64_t r = 0; r < rows; r++)
+        if (rp[r] < 0 || rp[r] > rp[r + 1]) return false;
+    for (int32_t c : col)
+        if (c < 0 || c >= rows) return false;
+    m.release();
+    if (hipMalloc(&m.rowptr, sizeof(int64_t) * (rows + 1)) != hipSuccess ||
+        hipMalloc(&m.col, sizeof(int32_t) * std::max<int64_t>(nnz, 1)) != hipSuccess ||
+        hipMalloc(&m.val, sizeof(double) * std::max<int64_t>(nnz, 1)) != hipSuccess) {
+        m.release();
+        return false;
+    }
+    m.n = n, m.rows = rows, m.nnz = nnz;
+    if (hipMemcpy(m.rowptr, rp.data(), sizeof(int64_t) * (rows + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        (nnz && (hipMemcpy(m.col, col.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice) != hipSuccess ||
+                 hipMemcpy(m.val, val.data(), sizeof(double) * nnz, hipMemcpyHostToDevice) != hipSuccess))) {
+        m.release();
+        return false;
+    }
+    return true;
+}
+
 // ------------------------------------------------------------------- OBJ --
 bool load_obj(const std::string &path, std::vector<double> &V, std::vector<int32_t> &F, std::string &err) {
     std::ifstream in(path);
@@ -556,25 +619,37 @@ const ElastodynamicsCQBEMMatrixCollection::Resident &ElastodynamicsCQBEMMatrixCo
     const size_t mat = sizeof(double) * (size_t)(n3 * n3);
     if (use_compressed_matrices) {
         // one dense scratch matrix at a time, compressed on the device (CompressedMatrix(mat, ratio) keeps
-        // (int)(mat.size() * ratio) coefficients, wavelets.hpp:443-445)
+        // (int)(mat.size() * ratio) coefficients, wavelets.hpp:443-445).  With a cache folder, each compressed
+        // matrix is read from its .cpmat file when present and written there otherwise, under the reference's
+        // names (matrix_collection.cpp:97-134,229-271).
         const int64_t keep = (int64_t)(int)((double)(n3 * n3) * compression_ratio);
+        const std::string tag = "_" + std::to_string(compression_ratio) + ".cpmat";
         double *scratch = nullptr;
         if (hipMalloc(&scratch, mat) != hipSuccess) fail_up("hipMalloc failed");
+        auto get = [&](DeviceCsr &out, const std::string &name, const std::function<void()> &fill) {
+            const std::string path = cache_folder.empty() ? "" : cache_folder + "/" + name + tag;
+            if (!path.empty() && load_cpmat(out, n3, path)) {
+                compressed_loaded++;
+                return;
+            }
+            fill();
+            wavelet_compress(scratch, n3, keep, 1.0, out, nullptr);
+            if (!path.empty() && !save_cpmat(out, path)) throw std::runtime_error("cannot write " + path);
+        };
+        auto upload_host = [&](const Dense &m) {
+            if (hipMemcpy(scratch, m.data.data(), mat, hipMemcpyHostToDevice) != hipSuccess)
+                throw std::runtime_error("upload failed");
+        };
         try {
             r.Gc.resize(L), r.Hc.resize(L);
-            for (size_t n = 1; n < L; n++) {
-                if (hipMemcpy(scratch, G_matrices[n].data.data(), mat, hipMemcpyHostToDevice) != hipSuccess)
-                    throw std::runtime_error("upload failed");
-                wavelet_compress(scratch, n3, keep, 1.0, r.Gc[n], nullptr);
-                if (hipMemcpy(scratch, H_matrices[n].data.data(), mat, hipMemcpyHostToDevice) != hipSuccess)
-                    throw std::runtime_error("upload failed");
-                wavelet_compress(scratch, n3, keep, 1.0, r.Hc[n], nullptr);  // H_matrices are negated already
+            for (size_t n = 0; n < L; n++) {
+                get(r.Hc[n], "H_" + std::to_string(n), [&] { upload_host(H_matrices[n]); });  // negated already
+                get(r.Gc[n], "G_" + std::to_string(n), [&] { upload_host(G_matrices[n]); });
             }
-            if (hipMemcpy(scratch, u_p.data.data(), mat, hipMemcpyHostToDevice) != hipSuccess)
-                throw std::runtime_error("upload failed");
-            wavelet_compress(scratch, n3, keep, 1.0, r.u_p_c, nullptr);
-            if (!H_inv.inverse_to_device(scratch, err)) throw std::runtime_error(err);
-            wavelet_compress(scratch, n3, keep, 1.0, r.H_inv_c, nullptr);
+            get(r.u_p_c, update_local_frame ? "u_p" : "u_p_no_frame_update", [&] { upload_host(u_p); });
+            get(r.H_inv_c, "H_inv", [&] {
+                if (!H_inv.inverse_to_device(scratch, err)) throw std::runtime_error(err);
+            });
             r.vlen = 2 * wavelet_padded(n3) - 2;
         } catch (const std::exception &e) {
             (void)hipFree(scratch);
@@ -748,6 +823,7 @@ bool ElastodynamicsCQBEMMatrixCollection::run(const Json &config) {
         std::filesystem::create_directories(folder);
         std::ofstream(folder + "/config.json") << stripped.dump(4);
     }
+    cache_folder = folder;
     const int64_t n3 = 3 * num_vertices;
     G_matrices.assign(L, Dense(n3, n3));
     H_matrices.assign(L, Dense(n3, n3));

@@ -96,6 +96,10 @@ class ElastodynamicsCQBEMMatrixCollection {
     MultistepMethod multistep_method = BDF2;
     double last_assembly_ms = 0.0, last_factor_ms = 0.0;
     bool loaded_from_cache = false;  // G and H came from simulation.coeffs_folder_path
+    // compressed matrices read from .cpmat files of the cache folder by resident() (the rest were compressed
+    // on the GPU and written there)
+    mutable int compressed_loaded = 0;
+    std::string cache_folder;  // elastodynamicsCQBEM_<key> under simulation.coeffs_folder_path, or empty
 
     // --- after the assembly (matrix_collection.cpp:174-215, physics_base_matrix_collection.cpp:40-50) ---
     Dense translational_acc_mat, rotational_acc_mat;  // 3 x 3 nv
@@ -144,6 +148,13 @@ class ElastodynamicsCQBEMMatrixCollection {
 int split_traction_discontinuity(const double *V, int nv, const int32_t *F, int nf, double threshold_deg,
                                  std::vector<double> &V_out, std::vector<int32_t> &F_out, std::vector<int32_t> &vmap_out,
                                  std::string &err);
+
+// The reference's compressed-matrix files (include/wavelets.hpp:451-504, SparseMatrix row-major, int32 indices):
+// int N; int64 rows, cols, nnz, outerSize, innerSize; nnz doubles; outerSize row offsets (the last, nnz, is not
+// stored); nnz column indices.  save_cpmat copies the device CSR to the host; load_cpmat checks the layout
+// against an n x n compression and uploads it (false when the file is missing or does not match).
+bool save_cpmat(const DeviceCsr &m, const std::string &path);
+bool load_cpmat(DeviceCsr &m, int64_t n, const std::string &path);
 
 // Loads an OBJ (v / f lines, polygons fanned into triangles).
 bool load_obj(const std::string &path, std::vector<double> &V, std::vector<int32_t> &F, std::string &err);
