@@ -760,7 +760,11 @@ __global__ void __launch_bounds__(256) k_transform(const cd *__restrict__ spec, 
         for (int l0 = 0; l0 < Lf; l0 += LC) {
             cd X[LC];
 #pragma unroll
-            for (int c = 0; c < LC; c++) X[c] = l0 + c < Lf ? spec[(long long)(l0 + c) * spec_ls + e] : cd{0.0, 0.0};
+            for (int c = 0; c < LC; c++) {
+                // each spectrum value is read once: streaming (non-temporal) loads
+                const double *src = reinterpret_cast<const double *>(spec + (long long)(l0 + c) * spec_ls + e);
+                X[c] = l0 + c < Lf ? cd{__builtin_nontemporal_load(src), __builtin_nontemporal_load(src + 1)} : cd{0.0, 0.0};
+            }
 #pragma unroll
             for (int c = 0; c < LC; c++) {
                 if (l0 + c >= Lf) break;
@@ -772,7 +776,7 @@ __global__ void __launch_bounds__(256) k_transform(const cd *__restrict__ spec, 
 #pragma unroll
         for (int k = 0; k < NC; k++) {
             const int n = n0 + k;
-            if (n < L) out[(long long)n * out_nstride + e] = acc[k];
+            if (n < L) __builtin_nontemporal_store(acc[k], out + (long long)n * out_nstride + e);  // written once
         }
     }
 }
