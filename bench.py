@@ -97,11 +97,25 @@ def cpu_baseline(V, F, params, seconds_target=15.0):
             model = next((l.split(":", 1)[1].strip() for l in fh if l.startswith("model name")), "")
     except OSError:
         pass
+    # the same quadrature on one core (reference precision), and on all cores in FP64 (the oracle's mode), each
+    # on a smaller row sample, extrapolated the same way
+    def quad_rate(mode, nthreads, nrows):
+        q = O.params(params.c1, params.c2, params.rho, params.nu, params.dt, params.R, L, bdf=params.bdf,
+                     order=params.order, subdiv=params.subdivision, mode=mode, nthreads=nthreads)
+        t = time.perf_counter()
+        O.laplace(V, F, q, s[ls[0]], mask=15, rows=(0, nrows))
+        return (time.perf_counter() - t) * (nv / nrows) * Lf + t_tr * (2 * n3 * n3 / M_sample)
+    full_1 = quad_rate(O.MODE_REFMIXED, 1, 2)
+    full_64 = quad_rate(O.MODE_FP64, threads, rows)
     return {"value": entries / full, "unit": "entries/s", "cores": threads, "kind": "port", "cpu_model": model,
-            "host_cpus": os.cpu_count(),
+            "host_cpus": os.cpu_count(), "one_core_value": entries / full_1,
+            "fp64_all_cores_value": entries / full_64,
             "sample": (f"oracle (reference float/double typedefs), G+H+B_trans+B_angular Laplace rows 0..{rows - 1} "
                        f"of {nv} at frequencies {ls} of {Lf} ({t_quad:.1f} s) + inverse transform of "
-                       f"{M_sample} entries ({t_tr:.1f} s), extrapolated to the full job: {full:.0f} s")}
+                       f"{M_sample} entries ({t_tr:.1f} s), extrapolated to the full job: {full:.0f} s; "
+                       f"one_core_value: rows 0..1 at frequency 0 on one thread; fp64_all_cores_value: rows "
+                       f"0..{rows - 1} at frequency 0 in the oracle's FP64 mode; both add the same (all-thread) "
+                       f"transform sample")}
 
 
 def scene_jobs(n_objects=16, seed=0):
