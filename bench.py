@@ -496,7 +496,7 @@ def history_step(ctx, ptrs, n3, L, torch, dev):
         ctx.history(*hist_args)
     hist_ms = (time.perf_counter() - t0) * 1e3 / reps
     hist_bytes = 2.0 * nh * n3 * n3 * 8 + nh * n3 * 6 * 8
-    return {"kernel": "k_history", "ms": hist_ms, "matrices_read_GB": hist_bytes / 1e9,
+    return {"kernel": "k_history_v2 (4 rows per wavefront, 16-byte loads)", "ms": hist_ms, "matrices_read_GB": hist_bytes / 1e9,
             "achieved_TBps": hist_bytes / (hist_ms * 1e-3) / 1e12, "peak_TBps": 8.0,
             "note": f"b = sum_i G_i p + H_i u + B_i acc over the {nh} resident history weights (one CQ time step)"}
 

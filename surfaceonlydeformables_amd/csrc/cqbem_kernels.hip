@@ -14,6 +14,9 @@
 // of src/elastodynamics_cqbem_{G,H,B_trans,B_angular}.cu.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+#include <cstdlib>
+
 #include "cqbem_device.hpp"
 #include "cqbem_kernels.hpp"
 
@@ -869,6 +872,93 @@ __global__ void __launch_bounds__(256) k_history(const double *__restrict__ G, c
     }
 }
 
+// The same history sum for an even n3 with 16-byte aligned matrices and vectors (every row then starts on a 16-byte
+// boundary).  Each wavefront takes R consecutive rows, so a chunk of p and u loaded once serves R matrix rows
+// (the vectors are re-read from L2 by every wavefront: with one row per wavefront that traffic equals the matrix
+// stream).  The matrix rows are streamed with 16-byte non-temporal loads: each weight is read once per step.  The
+// B terms are spread over the lanes before the wave reduction.  Matrix i starts at G + i * mat_stride (n3^2 for the
+// history; 0 with m = nh = 1 makes it y = A x).
+typedef double hist_d2 __attribute__((ext_vector_type(2)));
+
+template <bool HG, bool HH, int R>
+__global__ void __launch_bounds__(256) k_history_v2(const double *__restrict__ G, const double *__restrict__ H,
+                                                     const double *__restrict__ B, const double *__restrict__ p,
+                                                     const double *__restrict__ u, const double *__restrict__ acc,
+                                                     long long n3, long long mat_stride, int m, int nh,
+                                                     double *__restrict__ b) {
+    const long long r0 = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * R;
+    const int lane = threadIdx.x & 63;
+    if (r0 >= n3) return;
+    const long long nc = n3 >> 1;  // 16-byte columns
+    long long rr[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) rr[k] = r0 + k < n3 ? r0 + k : n3 - 1;  // rows past the end repeat the last (unused)
+    double s[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) s[k] = 0.0;
+    for (int i = 1; i <= m; i++) {
+        const long long mo = (long long)i * mat_stride;
+        const hist_d2 *xp = HG ? reinterpret_cast<const hist_d2 *>(p + (long long)(nh - i) * n3) : nullptr;
+        const hist_d2 *xu = HH ? reinterpret_cast<const hist_d2 *>(u + (long long)(nh - i) * n3) : nullptr;
+        const hist_d2 *gr[R], *hr[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            gr[k] = HG ? reinterpret_cast<const hist_d2 *>(G + mo + rr[k] * n3) : nullptr;
+            hr[k] = HH ? reinterpret_cast<const hist_d2 *>(H + mo + rr[k] * n3) : nullptr;
+        }
+        long long c = lane;
+        for (; c + 64 * (kHistUnroll - 1) < nc; c += 64 * kHistUnroll) {
+            hist_d2 x[kHistUnroll], y[kHistUnroll], g[R][kHistUnroll], h[R][kHistUnroll];
+#pragma unroll
+            for (int j = 0; j < kHistUnroll; j++) {
+                if (HG) x[j] = xp[c + 64 * j];
+                if (HH) y[j] = xu[c + 64 * j];
+#pragma unroll
+                for (int k = 0; k < R; k++) {
+                    if (HG) g[k][j] = __builtin_nontemporal_load(gr[k] + c + 64 * j);
+                    if (HH) h[k][j] = __builtin_nontemporal_load(hr[k] + c + 64 * j);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kHistUnroll; j++)
+#pragma unroll
+                for (int k = 0; k < R; k++) {
+                    if (HG) s[k] = fma(g[k][j].y, x[j].y, fma(g[k][j].x, x[j].x, s[k]));
+                    if (HH) s[k] = fma(-h[k][j].y, y[j].y, fma(-h[k][j].x, y[j].x, s[k]));
+                }
+        }
+        for (; c < nc; c += 64) {
+            const hist_d2 x = HG ? xp[c] : hist_d2{0.0, 0.0}, y = HH ? xu[c] : hist_d2{0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                if (HG) {
+                    const hist_d2 g = __builtin_nontemporal_load(gr[k] + c);
+                    s[k] = fma(g.y, x.y, fma(g.x, x.x, s[k]));
+                }
+                if (HH) {
+                    const hist_d2 h = __builtin_nontemporal_load(hr[k] + c);
+                    s[k] = fma(-h.y, y.y, fma(-h.x, y.x, s[k]));
+                }
+            }
+        }
+    }
+    if (B)
+        for (int j = lane; j < 6 * m; j += 64) {
+            const int i = j / 6 + 1, q = j - 6 * (i - 1);
+            const double a = acc[(long long)(nh - i) * 6 + q];
+#pragma unroll
+            for (int k = 0; k < R; k++) s[k] = fma(B[((long long)i * n3 + rr[k]) * 6 + q], a, s[k]);
+        }
+#pragma unroll
+    for (int k = 0; k < R; k++)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s[k] += __shfl_xor(s[k], o, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < R; k++)
+            if (r0 + k < n3) b[r0 + k] = s[k];
+}
+
 // y = A x for a row-major n x n matrix (u_p p of a constrained time step); one wavefront per row, as k_history.
 __global__ void __launch_bounds__(256) k_matvec(const double *__restrict__ A, const double *__restrict__ x, long long n,
                                                 double *__restrict__ y) {
@@ -1010,8 +1100,46 @@ hipError_t launch_transform(const cd *spec, long long E, long long spec_ls, int 
     return hipSuccess;
 }
 
+template <int R>
+static void launch_history_v2(const double *G, const double *H, const double *B, const double *p, const double *u,
+                              const double *acc, long long n3, long long mat_stride, int m, int nh, double *b,
+                              hipStream_t st) {
+    const long long waves = (n3 + R - 1) / R;
+    const dim3 grid((unsigned)((waves + 3) / 4));
+    if (G && H)
+        hipLaunchKernelGGL((k_history_v2<true, true, R>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, mat_stride, m,
+                           nh, b);
+    else if (G)
+        hipLaunchKernelGGL((k_history_v2<true, false, R>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, mat_stride, m,
+                           nh, b);
+    else if (H)
+        hipLaunchKernelGGL((k_history_v2<false, true, R>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, mat_stride, m,
+                           nh, b);
+    else
+        hipLaunchKernelGGL((k_history_v2<false, false, R>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, mat_stride,
+                           m, nh, b);
+}
+
+// Rows per wavefront of the 16-byte history and matvec kernel: CQBEM_HISTORY_ROWS = 1 selects the 8-byte one-row
+// kernel, 2 or 4 (default) the vectorized one.  Read at every call (a few hundred ns against a launch).
+static int history_rows() {
+    const char *e = std::getenv("CQBEM_HISTORY_ROWS");
+    const int v = e ? std::atoi(e) : 4;
+    return v == 1 || v == 2 ? v : 4;
+}
+
+static bool aligned16(const double *q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; }
+
 hipError_t launch_history(const double *G, const double *H, const double *B, const double *p, const double *u,
                           const double *acc, long long n3, int m, int nh, double *b, hipStream_t st) {
+    const int rows = history_rows();
+    if (rows > 1 && n3 % 2 == 0 && (!G || (aligned16(G) && aligned16(p))) && (!H || (aligned16(H) && aligned16(u)))) {
+        if (rows == 4)
+            launch_history_v2<4>(G, H, B, p, u, acc, n3, n3 * n3, m, nh, b, st);
+        else
+            launch_history_v2<2>(G, H, B, p, u, acc, n3, n3 * n3, m, nh, b, st);
+        return hipGetLastError();
+    }
     const dim3 grid((unsigned)((n3 + 3) / 4));
     if (G && H)
         hipLaunchKernelGGL((k_history<true, true>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, m, nh, b);
@@ -1025,6 +1153,14 @@ hipError_t launch_history(const double *G, const double *H, const double *B, con
 }
 
 hipError_t launch_matvec(const double *A, const double *x, long long n, double *y, hipStream_t st) {
+    const int rows = history_rows();
+    if (rows > 1 && n % 2 == 0 && aligned16(A) && aligned16(x)) {
+        if (rows == 4)
+            launch_history_v2<4>(A, nullptr, nullptr, x, nullptr, nullptr, n, 0, 1, 1, y, st);
+        else
+            launch_history_v2<2>(A, nullptr, nullptr, x, nullptr, nullptr, n, 0, 1, 1, y, st);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_matvec, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, A, x, n, y);
     return hipGetLastError();
 }

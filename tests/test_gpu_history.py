@@ -80,16 +80,52 @@ def test_history_on_assembled_matrices(ctx):
     assert np.linalg.norm(b.cpu().numpy() - want) <= 1e-13 * np.linalg.norm(want)
 
 
-@pytest.mark.parametrize("n", [1, 63, 258, 1031])
-def test_matvec(ctx, n):
-    """cqbem_matvec (the u_p p product of a constrained step) against numpy; relative error <= 1e-14."""
+@pytest.mark.parametrize("rows", ["1", "2", "4"])
+@pytest.mark.parametrize("n3,L,nh", [(2, 3, 2), (10, 4, 3), (130, 6, 9), (390, 5, 4), (1026, 9, 12)])
+def test_history_even_sizes_all_terms(ctx, monkeypatch, rows, n3, L, nh):
+    """Even 3 nv with 16-byte aligned operands takes the vectorized kernel (CQBEM_HISTORY_ROWS = 2 or 4 rows per
+    wavefront, 4 the default; 1 is the one-row kernel), with rows past the end when n3 is not a multiple of the row count; each
+    of G, H may be NULL.  A vector 8 bytes off alignment falls back to the one-row kernel."""
     import torch
+    monkeypatch.setenv("CQBEM_HISTORY_ROWS", rows)
+    rng = np.random.default_rng(n3 + 1)
+    G, H = rng.standard_normal((L, n3, n3)), rng.standard_normal((L, n3, n3))
+    B = rng.standard_normal((L, n3, 6))
+    p, u, acc = rng.standard_normal((nh, n3)), rng.standard_normal((nh, n3)), rng.standard_normal((nh, 6))
+    dev = torch.device("cuda:0")
+    t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in dict(G=G, H=H, B=B, p=p, u=u, acc=acc).items()}
+    b = torch.full((n3,), float("nan"), dtype=torch.float64, device=dev)
+    zero = np.zeros_like(G)
+    for use_g, use_h in ((True, True), (True, False), (False, True), (False, False)):
+        print(f"history rows={rows} n3={n3} L={L} nh={nh} G={use_g} H={use_h}", flush=True)
+        ctx.history(n3, L, t["G"].data_ptr() if use_g else 0, t["H"].data_ptr() if use_h else 0, t["B"].data_ptr(),
+                    t["p"].data_ptr() if use_g else 0, t["u"].data_ptr() if use_h else 0, t["acc"].data_ptr(), nh,
+                    b.data_ptr())
+        want = reference(G if use_g else zero, H if use_h else zero, B, p, u, acc, L)
+        assert np.linalg.norm(b.cpu().numpy() - want) <= 1e-13 * np.linalg.norm(want)
+    # p one double into a larger buffer: 8 bytes off a 16-byte boundary
+    pp = torch.zeros(nh * n3 + 1, dtype=torch.float64, device=dev)
+    pp[1:] = t["p"].reshape(-1)
+    b.fill_(float("nan"))
+    ctx.history(n3, L, t["G"].data_ptr(), t["H"].data_ptr(), t["B"].data_ptr(), pp.data_ptr() + 8, t["u"].data_ptr(),
+                t["acc"].data_ptr(), nh, b.data_ptr())
+    want = reference(G, H, B, p, u, acc, L)
+    assert np.linalg.norm(b.cpu().numpy() - want) <= 1e-13 * np.linalg.norm(want)
+
+
+@pytest.mark.parametrize("rows", ["1", "2", "4"])
+@pytest.mark.parametrize("n", [1, 2, 63, 258, 1030, 1031])
+def test_matvec(ctx, monkeypatch, rows, n):
+    """cqbem_matvec (the u_p p and H_0^-1 b products of a time step) against numpy; relative error <= 1e-14.  Even n
+    takes the vectorized kernel with 2 or 4 rows per wavefront (CQBEM_HISTORY_ROWS), odd n the one-row kernel."""
+    import torch
+    monkeypatch.setenv("CQBEM_HISTORY_ROWS", rows)
     rng = np.random.default_rng(n)
     A, x = rng.standard_normal((n, n)), rng.standard_normal(n)
     dev = torch.device("cuda:0")
     tA, tx = torch.from_numpy(A).to(dev), torch.from_numpy(x).to(dev)
     y = torch.full((n,), float("nan"), dtype=torch.float64, device=dev)
-    print(f"matvec n={n}", flush=True)
+    print(f"matvec rows={rows} n={n}", flush=True)
     ctx.matvec(n, tA.data_ptr(), tx.data_ptr(), y.data_ptr())
     want = A @ x
     assert np.linalg.norm(y.cpu().numpy() - want) <= 1e-14 * np.linalg.norm(A) * np.linalg.norm(x)
