@@ -120,6 +120,29 @@ __global__ void __launch_bounds__(256) k_row_fill(const double *__restrict__ can
     }
 }
 
+// One lane's part of a CSR row, s += val[e] x[col[e]] for e = e0 + lane, e0 + lane + 64, ... < e1, in that order.
+// Four (col, val) pairs and their gathers are in flight per lane: with one, a wavefront per row leaves the memory
+// system mostly idle.  The matrix arrays are streamed once per product (non-temporal); x is gathered through L2.
+__device__ inline double csr_row_part(const int32_t *__restrict__ col, const double *__restrict__ val,
+                                     const double *__restrict__ x, int64_t e0, int64_t e1, int lane, double s) {
+    int64_t e = e0 + lane;
+    for (; e + 64 * 3 < e1; e += 256) {
+        int32_t c[4];
+        double v[4], xv[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            c[k] = __builtin_nontemporal_load(col + e + 64 * k);
+            v[k] = __builtin_nontemporal_load(val + e + 64 * k);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) xv[k] = x[c[k]];
+#pragma unroll
+        for (int k = 0; k < 4; k++) s = fma(v[k], xv[k], s);
+    }
+    for (; e < e1; e += 64) s = fma(val[e], x[col[e]], s);
+    return s;
+}
+
 __global__ void __launch_bounds__(256) k_csr_matvec_sum(const CsrRef *__restrict__ refs, int m, int64_t rows,
                                                         double *__restrict__ y) {
     const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -128,8 +151,7 @@ __global__ void __launch_bounds__(256) k_csr_matvec_sum(const CsrRef *__restrict
     double s = 0.0;
     for (int i = 0; i < m; i++) {
         const CsrRef R = refs[i];
-        const int64_t e1 = R.rowptr[r + 1];
-        for (int64_t e = R.rowptr[r] + lane; e < e1; e += 64) s = fma(R.val[e], R.x[R.col[e]], s);
+        s = csr_row_part(R.col, R.val, R.x, R.rowptr[r], R.rowptr[r + 1], lane, s);
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
@@ -142,9 +164,7 @@ __global__ void __launch_bounds__(256) k_csr_matvec(const int64_t *__restrict__ 
     const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (r >= rows) return;
-    double s = 0.0;
-    const int64_t e1 = rowptr[r + 1];
-    for (int64_t e = rowptr[r] + lane; e < e1; e += 64) s = fma(val[e], x[col[e]], s);
+    double s = csr_row_part(col, val, x, rowptr[r], rowptr[r + 1], lane, 0.0);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if (lane == 0) y[r] = s;
