@@ -488,6 +488,7 @@ def history_step(ctx, ptrs, n3, L, torch, dev):
     hu = torch.randn((nh, n3), dtype=torch.float64, device=dev)
     ha = torch.randn((nh, 6), dtype=torch.float64, device=dev)
     hb = torch.empty(n3, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()  # the histories come from torch's stream; the library runs on its own
     hist_args = (n3, L, ptrs["G"], ptrs["H"], ptrs["B"], hp.data_ptr(), hu.data_ptr(), ha.data_ptr(), nh, hb.data_ptr())
     ctx.history(*hist_args)
     reps = 10

@@ -78,7 +78,10 @@ typedef struct {
     int32_t on_device;
 } cqbem_outputs;
 
-/* Creates a context on HIP device `device`; hbm_budget_bytes = 0 means 90% of free device memory. */
+/* Creates a context on HIP device `device`; hbm_budget_bytes = 0 means 90% of free device memory.
+ * A context works on streams of its own.  Device buffers passed to it must hold their data when the call is
+ * made: work the caller queued on another stream to produce them (or to fill an output buffer) must have
+ * completed, e.g. by synchronizing that stream first.  Calls that return results complete before returning. */
 int cqbem_create(cqbem_context **ctx, int device, uint64_t hbm_budget_bytes);
 int cqbem_destroy(cqbem_context *ctx);
 const char *cqbem_last_error(const cqbem_context *ctx);

@@ -101,6 +101,7 @@ def test_matrix_selection_and_host_device_outputs_agree(ctx):
     dev = {k: torch.zeros(shape, dtype=torch.float64, device="cuda") for k, shape in
            (("G", (L, n3, n3)), ("H", (L, n3, n3)), ("B_trans", (L, n3, 3)), ("B_angular", (L, n3, 3, 3)),
             ("B", (L, n3, 6)))}
+    torch.cuda.synchronize()  # buffers filled on torch's stream; the library runs on its own
     ctx.assemble_device(mesh, abi.make_params(c1, c2, RHO, 0.3, 1e-3, R, L), {k: t.data_ptr() for k, t in dev.items()})
     torch.cuda.synchronize()
     for k in dev:
@@ -205,6 +206,7 @@ def test_row_slabs_are_bitwise_identical():
         res = []
         for ctx in (full, small):
             outs = {k: torch.full(s, float("nan"), dtype=torch.float64, device=dev) for k, s in shapes.items()}
+            torch.cuda.synchronize()  # buffers filled on torch's stream; the library runs on its own
             ctx.assemble_device(mesh, params, {k: t.data_ptr() for k, t in outs.items()})
             res.append({k: t.cpu().numpy() for k, t in outs.items()})
         for k in shapes:

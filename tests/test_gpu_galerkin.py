@@ -129,6 +129,7 @@ def test_galerkin_chunks_are_bitwise_identical():
         res = []
         for ctx in (full, small):
             outs = {k: torch.full(s, float("nan"), dtype=torch.float64, device=dev) for k, s in shapes.items()}
+            torch.cuda.synchronize()  # buffers filled on torch's stream; the library runs on its own
             ctx.assemble_device(m, params, {k: t.data_ptr() for k, t in outs.items()})
             res.append({k: t.cpu().numpy() for k, t in outs.items()})
         for k in shapes:

@@ -40,6 +40,7 @@ def test_history_random(ctx, n3, L, nh):
          dict(G=G, H=H, B=B, p=p if nh else np.zeros((1, n3)), u=u if nh else np.zeros((1, n3)),
               acc=acc if nh else np.zeros((1, 6))).items()}
     b = torch.full((n3,), float("nan"), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()  # buffers filled on torch's stream; the library runs on its own
     print(f"history n3={n3} L={L} nh={nh}", flush=True)
     ctx.history(n3, L, t["G"].data_ptr(), t["H"].data_ptr(), t["B"].data_ptr(), t["p"].data_ptr(),
                 t["u"].data_ptr(), t["acc"].data_ptr(), nh, b.data_ptr())
@@ -96,6 +97,7 @@ def test_history_even_sizes_all_terms(ctx, monkeypatch, rows, n3, L, nh):
     t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in dict(G=G, H=H, B=B, p=p, u=u, acc=acc).items()}
     b = torch.full((n3,), float("nan"), dtype=torch.float64, device=dev)
     zero = np.zeros_like(G)
+    torch.cuda.synchronize()  # buffers filled on torch's stream; the library runs on its own
     for use_g, use_h in ((True, True), (True, False), (False, True), (False, False)):
         print(f"history rows={rows} n3={n3} L={L} nh={nh} G={use_g} H={use_h}", flush=True)
         ctx.history(n3, L, t["G"].data_ptr() if use_g else 0, t["H"].data_ptr() if use_h else 0, t["B"].data_ptr(),
@@ -107,10 +109,42 @@ def test_history_even_sizes_all_terms(ctx, monkeypatch, rows, n3, L, nh):
     pp = torch.zeros(nh * n3 + 1, dtype=torch.float64, device=dev)
     pp[1:] = t["p"].reshape(-1)
     b.fill_(float("nan"))
+    torch.cuda.synchronize()  # buffers filled on torch's stream; the library runs on its own
     ctx.history(n3, L, t["G"].data_ptr(), t["H"].data_ptr(), t["B"].data_ptr(), pp.data_ptr() + 8, t["u"].data_ptr(),
                 t["acc"].data_ptr(), nh, b.data_ptr())
     want = reference(G, H, B, p, u, acc, L)
     assert np.linalg.norm(b.cpu().numpy() - want) <= 1e-13 * np.linalg.norm(want)
+
+
+def test_history_full_size_against_device_gemv(ctx):
+    """At the icosphere-L4 size (3 nv = 7686; L = 8 to keep it at 7.6 GB), b against torch's FP64 matrix-vector
+    products on the device (rocBLAS), summed in another order: relative error <= 1e-13.  H_0^-1 b's matvec too."""
+    import torch
+    n3, L, nh = 7686, 8, 10
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(11)
+    G = torch.randn((L, n3, n3), dtype=torch.float64, device=dev, generator=g)
+    H = torch.randn((L, n3, n3), dtype=torch.float64, device=dev, generator=g)
+    B = torch.randn((L, n3, 6), dtype=torch.float64, device=dev, generator=g)
+    p = torch.randn((nh, n3), dtype=torch.float64, device=dev, generator=g)
+    u = torch.randn((nh, n3), dtype=torch.float64, device=dev, generator=g)
+    a = torch.randn((nh, 6), dtype=torch.float64, device=dev, generator=g)
+    b = torch.full((n3,), float("nan"), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()  # the inputs come from torch's stream; the library runs on its own
+    print(f"history full size n3={n3} L={L} nh={nh}", flush=True)
+    ctx.history(n3, L, G.data_ptr(), H.data_ptr(), B.data_ptr(), p.data_ptr(), u.data_ptr(), a.data_ptr(), nh, b.data_ptr())
+    want = torch.zeros(n3, dtype=torch.float64, device=dev)
+    for i in range(1, L):
+        want += torch.mv(G[i], p[nh - i]) - torch.mv(H[i], u[nh - i]) + torch.mv(B[i], a[nh - i])
+    assert (torch.linalg.norm(b - want) / torch.linalg.norm(want)).item() <= 1e-13
+    y = torch.full((n3,), float("nan"), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    print(f"matvec full size n={n3}", flush=True)
+    ctx.matvec(n3, G[0].data_ptr(), p[0].data_ptr(), y.data_ptr())
+    want = torch.mv(G[0], p[0])
+    assert (torch.linalg.norm(y - want) / torch.linalg.norm(want)).item() <= 1e-13
+    del G, H
+    torch.cuda.empty_cache()
 
 
 @pytest.mark.parametrize("rows", ["1", "2", "4"])
@@ -125,6 +159,7 @@ def test_matvec(ctx, monkeypatch, rows, n):
     dev = torch.device("cuda:0")
     tA, tx = torch.from_numpy(A).to(dev), torch.from_numpy(x).to(dev)
     y = torch.full((n,), float("nan"), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()  # buffers filled on torch's stream; the library runs on its own
     print(f"matvec rows={rows} n={n}", flush=True)
     ctx.matvec(n, tA.data_ptr(), tx.data_ptr(), y.data_ptr())
     want = A @ x

@@ -44,7 +44,9 @@ def device_outputs(torch, nv, L):
     n3 = 3 * nv
     dev = torch.device("cuda:0")
     shapes = {"G": (L, n3, n3), "H": (L, n3, n3), "B_trans": (L, n3, 3), "B_angular": (L, n3, 3, 3), "B": (L, n3, 6)}
-    return {k: torch.full(s, float("nan"), dtype=torch.float64, device=dev) for k, s in shapes.items()}
+    outs = {k: torch.full(s, float("nan"), dtype=torch.float64, device=dev) for k, s in shapes.items()}
+    torch.cuda.synchronize()  # buffers filled on torch's stream; the library runs on its own
+    return outs
 
 
 def test_concurrent_jobs_match_isolated_runs():

@@ -32,6 +32,7 @@ def gpu_compress(ctx, A, k):
     cap = max(k, 1)
     col = torch.full((cap,), -1, dtype=torch.int32, device=dev)
     val = torch.full((cap,), float("nan"), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()  # buffers filled on torch's stream; the library runs on its own
     print(f"wavelet compress n={n} k={k}", flush=True)
     nnz = ctx.wavelet_compress(n, tA.data_ptr(), k, rowptr.data_ptr(), col.data_ptr(), val.data_ptr())
     return nnz, rowptr, col, val
@@ -90,6 +91,7 @@ def test_compress_assembled_G_and_product(ctx):
     x = np.random.default_rng(2).standard_normal(rows)
     tx = torch.from_numpy(x).to(dev)
     y = torch.full((rows,), float("nan"), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()  # buffers filled on torch's stream; the library runs on its own
     ctx.csr_matvec(rows, rowptr.data_ptr(), col.data_ptr(), val.data_ptr(), tx.data_ptr(), y.data_ptr())
     ref = want @ x
     assert np.linalg.norm(y.cpu().numpy() - ref) <= 1e-14 * np.linalg.norm(want.data) * np.linalg.norm(x)
