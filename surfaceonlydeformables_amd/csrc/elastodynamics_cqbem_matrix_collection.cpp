@@ -586,7 +586,16 @@ ElastodynamicsCQBEMMatrixCollection::ElastodynamicsCQBEMMatrixCollection(double 
                                                                          unsigned int cuda_thread_per_block)
     : dt_(dt), enable_cuda_(enable_cuda), cuda_thread_per_block_(cuda_thread_per_block) {}
 
-ElastodynamicsCQBEMMatrixCollection::~ElastodynamicsCQBEMMatrixCollection() { free_resident(); }
+ElastodynamicsCQBEMMatrixCollection::~ElastodynamicsCQBEMMatrixCollection() {
+    free_resident();
+    free_assembled();
+}
+
+void ElastodynamicsCQBEMMatrixCollection::free_assembled() const {
+    for (double *p : {assembled_G_, assembled_H_, assembled_B_})
+        if (p) (void)hipFree(p);
+    assembled_G_ = assembled_H_ = assembled_B_ = nullptr;
+}
 
 void ElastodynamicsCQBEMMatrixCollection::free_resident() {
     for (double *p : {resident_.G, resident_.H, resident_.B, resident_.u_p, resident_.H_inv})
@@ -626,29 +635,39 @@ const ElastodynamicsCQBEMMatrixCollection::Resident &ElastodynamicsCQBEMMatrixCo
         const std::string tag = "_" + std::to_string(compression_ratio) + ".cpmat";
         double *scratch = nullptr;
         if (hipMalloc(&scratch, mat) != hipSuccess) fail_up("hipMalloc failed");
-        auto get = [&](DeviceCsr &out, const std::string &name, const std::function<void()> &fill) {
+        // fill() puts the dense matrix somewhere on the device and returns it with the factor for its kept values
+        using Source = std::pair<const double *, double>;
+        auto get = [&](DeviceCsr &out, const std::string &name, const std::function<Source()> &fill) {
             const std::string path = cache_folder.empty() ? "" : cache_folder + "/" + name + tag;
             if (!path.empty() && load_cpmat(out, n3, path)) {
                 compressed_loaded++;
                 return;
             }
-            fill();
-            wavelet_compress(scratch, n3, keep, 1.0, out, nullptr);
+            const Source src = fill();
+            wavelet_compress(src.first, n3, keep, src.second, out, nullptr);
             if (!path.empty() && !save_cpmat(out, path)) throw std::runtime_error("cannot write " + path);
         };
         auto upload_host = [&](const Dense &m) {
             if (hipMemcpy(scratch, m.data.data(), mat, hipMemcpyHostToDevice) != hipSuccess)
                 throw std::runtime_error("upload failed");
+            return Source{scratch, 1.0};
         };
         try {
             r.Gc.resize(L), r.Hc.resize(L);
             for (size_t n = 0; n < L; n++) {
-                get(r.Hc[n], "H_" + std::to_string(n), [&] { upload_host(H_matrices[n]); });  // negated already
-                get(r.Gc[n], "G_" + std::to_string(n), [&] { upload_host(G_matrices[n]); });
+                // init's device copies hold H_n un-negated: compressed with factor -1, the kept set and values
+                // are those of the negated host matrix (negation commutes exactly with the Haar steps)
+                get(r.Hc[n], "H_" + std::to_string(n), [&] {
+                    return assembled_H_ ? Source{assembled_H_ + n * n3 * n3, -1.0} : upload_host(H_matrices[n]);
+                });
+                get(r.Gc[n], "G_" + std::to_string(n), [&] {
+                    return assembled_G_ ? Source{assembled_G_ + n * n3 * n3, 1.0} : upload_host(G_matrices[n]);
+                });
             }
-            get(r.u_p_c, update_local_frame ? "u_p" : "u_p_no_frame_update", [&] { upload_host(u_p); });
+            get(r.u_p_c, update_local_frame ? "u_p" : "u_p_no_frame_update", [&] { return upload_host(u_p); });
             get(r.H_inv_c, "H_inv", [&] {
                 if (!H_inv.inverse_to_device(scratch, err)) throw std::runtime_error(err);
+                return Source{scratch, 1.0};
             });
             r.vlen = 2 * wavelet_padded(n3) - 2;
         } catch (const std::exception &e) {
@@ -656,10 +675,26 @@ const ElastodynamicsCQBEMMatrixCollection::Resident &ElastodynamicsCQBEMMatrixCo
             fail_up(e.what());
         }
         (void)hipFree(scratch);
-        if (hipMalloc(&r.B, sizeof(double) * (size_t)(n3 * 6) * L) != hipSuccess) fail_up("hipMalloc failed");
-        for (size_t n = 0; n < L; n++)
-            if (hipMemcpy(r.B + n * n3 * 6, B_matrices[n].data.data(), sizeof(double) * n3 * 6, hipMemcpyHostToDevice) != hipSuccess)
-                fail_up("upload failed");
+        if (assembled_B_) {
+            r.B = assembled_B_;
+            assembled_B_ = nullptr;
+        } else {
+            if (hipMalloc(&r.B, sizeof(double) * (size_t)(n3 * 6) * L) != hipSuccess) fail_up("hipMalloc failed");
+            for (size_t n = 0; n < L; n++)
+                if (hipMemcpy(r.B + n * n3 * 6, B_matrices[n].data.data(), sizeof(double) * n3 * 6, hipMemcpyHostToDevice) != hipSuccess)
+                    fail_up("upload failed");
+        }
+        free_assembled();  // the dense G_n and H_n are not stepped on
+        resident_ = std::move(r);
+        resident_ready_ = true;
+        return resident_;
+    }
+    if (assembled_G_ && assembled_H_ && assembled_B_) {  // init's device copies, in the layout cqbem_history reads
+        r.G = assembled_G_, r.H = assembled_H_, r.B = assembled_B_;
+        assembled_G_ = assembled_H_ = assembled_B_ = nullptr;
+        if (hipMalloc(&r.u_p, mat) != hipSuccess || hipMalloc(&r.H_inv, mat) != hipSuccess) fail_up("hipMalloc failed");
+        if (hipMemcpy(r.u_p, u_p.data.data(), mat, hipMemcpyHostToDevice) != hipSuccess) fail_up("upload failed");
+        if (!H_inv.inverse_to_device(r.H_inv, err)) fail_up(err.c_str());
         resident_ = std::move(r);
         resident_ready_ = true;
         return resident_;
@@ -844,19 +879,70 @@ bool ElastodynamicsCQBEMMatrixCollection::run(const Json &config) {
     std::string err;
     cqbem_context *ctx = context(err);
     if (!ctx) return fail(err);
-    std::vector<double> g, h, b((size_t)L * n3 * 6);
-    if (!loaded_from_cache) g.resize((size_t)L * n3 * n3), h.resize((size_t)L * n3 * n3);
+    free_resident();
+    free_assembled();
     cqbem_mesh mesh{V.data(), num_vertices, F.data(), (int64_t)(F.size() / 3),
                     vertex_map_inverse.empty() ? nullptr : vertex_map_inverse.data()};
     // with cached G and H only B is assembled (the reference computes B on every init, :146-177)
     cqbem_params prm{c1, c2, rho, nu, dt_, R, (int32_t)L, multistep_method == BDF1 ? CQBEM_BDF1 : CQBEM_BDF2,
                      order, sub, use_galerkin ? 1 : 0, CQBEM_FP64,
                      loaded_from_cache ? (CQBEM_B_TRANS | CQBEM_B_ANGULAR) : CQBEM_ALL};
+    const auto t0 = std::chrono::steady_clock::now();
+    // Assembled into device buffers of the collection when they can be allocated: the host matrices are copied
+    // from there and the time stepping keeps them (resident()).  The library's own host-output path needs the
+    // same device memory, so a failed allocation means that path too would hold the outputs only while staging.
+    const size_t mbytes = sizeof(double) * (size_t)(n3 * n3), bmat = sizeof(double) * (size_t)(n3 * 6);
+    {
+        double *dg = nullptr, *dh = nullptr, *db = nullptr;
+        const bool ok = (loaded_from_cache || (hipMalloc(&dg, mbytes * L) == hipSuccess && hipMalloc(&dh, mbytes * L) == hipSuccess)) &&
+                        hipMalloc(&db, bmat * L) == hipSuccess;
+        if (ok) {
+            assembled_G_ = dg, assembled_H_ = dh, assembled_B_ = db;
+        } else {
+            for (double *q : {dg, dh, db})
+                if (q) (void)hipFree(q);
+            (void)hipGetLastError();
+        }
+    }
+    if (assembled_B_) {
+        cqbem_outputs out{assembled_G_, assembled_H_, nullptr, nullptr, assembled_B_, 1};
+        if (cqbem_assemble(ctx, &mesh, &prm, &out) != CQBEM_OK) {
+            free_assembled();
+            return fail(cqbem_last_error(ctx));
+        }
+        for (size_t n = 0; n < L; n++) {
+            if (!loaded_from_cache) {
+                double *gd = G_matrices[n].data.data(), *hd = H_matrices[n].data.data();
+                if (hipMemcpy(gd, assembled_G_ + n * n3 * n3, mbytes, hipMemcpyDeviceToHost) != hipSuccess ||
+                    hipMemcpy(hd, assembled_H_ + n * n3 * n3, mbytes, hipMemcpyDeviceToHost) != hipSuccess) {
+                    free_assembled();
+                    return fail("device-to-host copy of the matrices failed");
+                }
+                if (!folder.empty() && (!save_matrix(H_matrices[n], folder + "/H_" + std::to_string(n) + ".mat") ||
+                                        !save_matrix(G_matrices[n], folder + "/G_" + std::to_string(n) + ".mat")))
+                    return fail("cannot write the matrix cache in " + folder);
+                parallel_for(n3 * n3, [&](int64_t a, int64_t e1) {
+                    for (int64_t e = a; e < e1; e++) hd[e] = -hd[e];  // matrix_collection.cpp:140
+                });
+            }
+            if (hipMemcpy(B_matrices[n].data.data(), assembled_B_ + n * n3 * 6, bmat, hipMemcpyDeviceToHost) != hipSuccess) {
+                free_assembled();
+                return fail("device-to-host copy of B failed");
+            }
+        }
+        if (loaded_from_cache) free_assembled();  // G and H come from the files: resident() uploads them
+        last_assembly_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        (void)cuda_thread_per_block_;  // launch shapes are chosen by the library
+        const auto t1 = std::chrono::steady_clock::now();
+        if (!post_process()) return false;
+        last_factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+        return true;
+    }
+    std::vector<double> g, h, b((size_t)L * n3 * 6);
+    if (!loaded_from_cache) g.resize((size_t)L * n3 * n3), h.resize((size_t)L * n3 * n3);
     cqbem_outputs out{loaded_from_cache ? nullptr : g.data(), loaded_from_cache ? nullptr : h.data(), nullptr, nullptr,
                       b.data(), 0};
-    const auto t0 = std::chrono::steady_clock::now();
     if (cqbem_assemble(ctx, &mesh, &prm, &out) != CQBEM_OK) return fail(cqbem_last_error(ctx));
-    last_assembly_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     for (size_t n = 0; n < L; n++) {
         if (!loaded_from_cache) {
             const double *gs = g.data() + n * n3 * n3, *hs = h.data() + n * n3 * n3;
@@ -875,6 +961,7 @@ bool ElastodynamicsCQBEMMatrixCollection::run(const Json &config) {
         }
         std::copy(b.begin() + n * n3 * 6, b.begin() + (n + 1) * n3 * 6, B_matrices[n].data.begin());
     }
+    last_assembly_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     (void)cuda_thread_per_block_;  // launch shapes are chosen by the library
     const auto t1 = std::chrono::steady_clock::now();
     if (!post_process()) return false;

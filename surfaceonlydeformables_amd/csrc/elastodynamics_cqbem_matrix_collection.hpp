@@ -137,6 +137,12 @@ class ElastodynamicsCQBEMMatrixCollection {
     mutable Resident resident_;
     mutable bool resident_ready_ = false;
     void free_resident();
+    // G_n, H_n (un-negated) and B_n as cqbem_assemble wrote them on the device during init, handed to resident()
+    // so that the time stepping does not upload them again (dense matrices), or compressed from there.  Null when
+    // init assembled through host memory (the device buffers could not be allocated) or read G and H from the
+    // cache folder.
+    mutable double *assembled_G_ = nullptr, *assembled_H_ = nullptr, *assembled_B_ = nullptr;
+    void free_assembled() const;
 
     double dt_;
     bool enable_cuda_;
