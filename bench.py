@@ -285,6 +285,7 @@ def time_stepping_main(args):
                       "steps": steps, "step_phases_ms": info["step_phases_ms"], "first_step_ms": info["first_step_ms"],
                       "upload_ms": info["upload_ms"],
                       "assembly_ms": info["ms"], "post_process_ms": info["factor_ms"],
+                      "post_process_phases_ms": info.get("factor_phases_ms"),
                       "history_matrices_GB": (2 * (L - 1) * n3 * n3 * 8 / 1e9 if not args.compression_ratio else
                                               2 * (L - 1) * int(n3 * n3 * args.compression_ratio) * 12 / 1e9),
                       "cpu_history_ms": {"value": per * 2 * (L - 1) * 1e3, "threads": os.cpu_count(),

@@ -194,9 +194,12 @@ int main(int argc, char **argv) {
     std::printf("{\"L\": %zu, \"nv\": %lld, \"R\": %.17g, \"c1\": %.17g, \"c2\": %.17g, \"mass\": %.17g, "
                 "\"ms\": %.3f, \"factor_ms\": %.3f, \"upload_ms\": %.3f, \"first_step_ms\": %.3f, \"step_ms\": %.3f, "
                 "\"step_phases_ms\": {\"history\": %.3f, \"solve\": %.3f, \"u_p_p\": %.3f, \"history_upload\": %.3f, "
-                "\"host\": %.3f}, \"cached\": %s, \"compressed_loaded\": %d}\n",
+                "\"host\": %.3f}, \"factor_phases_ms\": {\"geometry\": %.3f, \"g_prime\": %.3f, \"least_squares\": %.3f, "
+                "\"lu\": %.3f, \"lu_solves\": %.3f}, \"cached\": %s, \"compressed_loaded\": %d}\n",
                 col.max_time_history + 1, (long long)col.num_vertices, col.R, col.c1, col.c2, col.mass, col.last_assembly_ms,
                 col.last_factor_ms, upload_ms, first_step_ms, step_ms, phase_step_ms[0], phase_step_ms[1], phase_step_ms[2],
-                phase_step_ms[3], phase_step_ms[4], col.loaded_from_cache ? "true" : "false", col.compressed_loaded);
+                phase_step_ms[3], phase_step_ms[4], col.post_process_phase_ms[0], col.post_process_phase_ms[1],
+                col.post_process_phase_ms[2], col.post_process_phase_ms[3], col.post_process_phase_ms[4],
+                col.loaded_from_cache ? "true" : "false", col.compressed_loaded);
     return 0;
 }

@@ -72,6 +72,31 @@ void to_colmajor(const double *in, int64_t rows, int64_t cols, double *out) {
 
 }  // namespace
 
+void warm_up_lu() {
+    const RocLib &lib = roclib();
+    if (!lib.error.empty()) return;
+    rocblas_handle h = nullptr;
+    if (lib.create_handle(&h) != rocblas_status_success) return;
+    constexpr int n = 2048;  // large enough for the blocked factorisation and its trailing-update products
+    std::vector<double> eye((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) eye[(size_t)i * n + i] = 2.0;
+    double *a = nullptr, *b = nullptr;
+    int32_t *piv = nullptr;
+    if (hipMalloc(&a, sizeof(double) * eye.size()) == hipSuccess && hipMalloc(&b, sizeof(double) * eye.size()) == hipSuccess &&
+        hipMalloc(&piv, sizeof(int32_t) * (n + 1)) == hipSuccess &&
+        hipMemcpy(a, eye.data(), sizeof(double) * eye.size(), hipMemcpyHostToDevice) == hipSuccess &&
+        hipMemcpy(b, eye.data(), sizeof(double) * eye.size(), hipMemcpyHostToDevice) == hipSuccess &&
+        lib.dgetrf(h, n, n, a, n, piv, piv + n) == rocblas_status_success) {
+        (void)lib.dgetrs(h, rocblas_operation_transpose, n, 3, a, n, piv, b, n);
+        (void)lib.dgetrs(h, rocblas_operation_transpose, n, n, a, n, piv, b, n);
+        (void)hipStreamSynchronize(nullptr);  // the handle's (null) stream only
+    }
+    for (void *p : {(void *)a, (void *)b, (void *)piv})
+        if (p) (void)hipFree(p);
+    (void)hipGetLastError();
+    lib.destroy_handle(h);
+}
+
 DeviceLU::~DeviceLU() {
     if (lu_) (void)hipFree(lu_);
     if (ipiv_) (void)hipFree(ipiv_);

@@ -13,6 +13,11 @@
 
 namespace cqbem {
 
+// Opens rocBLAS and rocSOLVER and runs a small LU and solves, so that their first-use cost (library load,
+// handle set-up, kernel code objects: seconds in a fresh process) is not paid inside a later DeviceLU::factor.
+// Callable from any thread, beside other GPU work on non-blocking streams; failures are left to factor().
+void warm_up_lu();
+
 // LU factors of one n x n row-major matrix, resident in device memory.
 class DeviceLU {
   public:

@@ -95,6 +95,9 @@ class ElastodynamicsCQBEMMatrixCollection {
     bool use_compressed_matrices = false;
     MultistepMethod multistep_method = BDF2;
     double last_assembly_ms = 0.0, last_factor_ms = 0.0;
+    // last_factor_ms by part: [0] geometry, inertia and acceleration matrices, [1] G', [2] the least-squares
+    // products, [3] the LU of H_0 (GPU), [4] its solves for u_p, H_inv_B_trans and H_inv_B_euler
+    double post_process_phase_ms[5] = {0, 0, 0, 0, 0};
     bool loaded_from_cache = false;  // G and H came from simulation.coeffs_folder_path
     // compressed matrices read from .cpmat files of the cache folder by resident() (the rest were compressed
     // on the GPU and written there)
