@@ -1133,17 +1133,24 @@ bool ElastodynamicsCQBEMMatrixCollection::post_process() {
     parallel_for((int64_t)H0.size(), [&](int64_t a, int64_t b) {
         for (int64_t e = a; e < b; e++) H0[e] = -H_matrices[0].data[e];
     });
-    u_p = G_matrices[0];  // becomes G', then H_0^-1 G'
-    if (update_local_frame)
-        parallel_for(n3, [&](int64_t r0, int64_t r1) {
-            for (int64_t r = r0; r < r1; r++)
-                for (int64_t c = 0; c < n3; c++) {
-                    double v = 0;
-                    for (int k = 0; k < 3; k++)
-                        v += Bt[3 * r + k] * translational_acc_mat(k, c) + Be[3 * r + k] * rotational_acc_mat(k, c);
-                    u_p(r, c) += v;
-                }
-        });
+    // u_p becomes G' = G_0 (+ B_trans T + B_euler R), then H_0^-1 G'; one threaded pass over the rows
+    u_p = Dense(n3, n3);
+    parallel_for(n3, [&](int64_t r0, int64_t r1) {
+        for (int64_t r = r0; r < r1; r++) {
+            const double *g0 = &G_matrices[0].data[(size_t)(r * n3)];
+            double *up = &u_p.data[(size_t)(r * n3)];
+            if (!update_local_frame) {
+                std::memcpy(up, g0, sizeof(double) * (size_t)n3);
+                continue;
+            }
+            for (int64_t c = 0; c < n3; c++) {
+                double v = 0;
+                for (int k = 0; k < 3; k++)
+                    v += Bt[3 * r + k] * translational_acc_mat(k, c) + Be[3 * r + k] * rotational_acc_mat(k, c);
+                up[c] = g0[c] + v;
+            }
+        }
+    });
     phase(1);
     std::string err;
     if (!B_trans_inv.compute(Bt, n3, 3, err) || !B_euler_inv.compute(Be, n3, 3, err)) return fail(err);
