@@ -23,6 +23,7 @@ struct RocLib {
     decltype(&rocblas_destroy_handle) destroy_handle = nullptr;
     decltype(&rocsolver_dgetrf) dgetrf = nullptr;
     decltype(&rocsolver_dgetrs) dgetrs = nullptr;
+    decltype(&rocblas_dgeam) dgeam = nullptr;  // optional: device transposes of many right-hand sides
     std::string error;
 };
 
@@ -48,6 +49,7 @@ const RocLib &roclib() {
         lib.destroy_handle = reinterpret_cast<decltype(lib.destroy_handle)>(dlsym(blas, "rocblas_destroy_handle"));
         lib.dgetrf = reinterpret_cast<decltype(lib.dgetrf)>(dlsym(solver, "rocsolver_dgetrf"));
         lib.dgetrs = reinterpret_cast<decltype(lib.dgetrs)>(dlsym(solver, "rocsolver_dgetrs"));
+        lib.dgeam = reinterpret_cast<decltype(lib.dgeam)>(dlsym(blas, "rocblas_dgeam"));
         if (!lib.create_handle || !lib.destroy_handle || !lib.dgetrf || !lib.dgetrs) lib.error = "rocSOLVER symbols missing";
     });
     return lib;
@@ -135,6 +137,31 @@ bool DeviceLU::solve(double *B, int64_t nrhs, std::string &err) const {
     std::lock_guard<std::mutex> lk(mu_);
     const RocLib &lib = roclib();
     if (!lu_) return err = "LU not computed", false;
+    if (nrhs > 1 && lib.dgeam) {
+        // The row-major n x nrhs B is a column-major nrhs x n matrix: transposed on the device (rocblas_dgeam)
+        // into the column-major layout getrs solves in, and back; the host only copies.
+        rocblas_handle h = static_cast<rocblas_handle>(handle_);
+        const double one = 1.0, zero = 0.0;
+        double *d = nullptr, *t = nullptr;
+        bool ok = hip_ok(hipMalloc(&d, sizeof(double) * n_ * nrhs), "hipMalloc rhs", err) &&
+                  hip_ok(hipMalloc(&t, sizeof(double) * n_ * nrhs), "hipMalloc rhs", err) &&
+                  hip_ok(hipMemcpy(t, B, sizeof(double) * n_ * nrhs, hipMemcpyHostToDevice), "hipMemcpy rhs", err) &&
+                  // dgeam's B operand is d itself with factor 0: zeroed so that no stale NaN survives 0 * x
+                  hip_ok(hipMemset(d, 0, sizeof(double) * n_ * nrhs), "hipMemset rhs", err);
+        if (ok && lib.dgeam(h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)n_, (rocblas_int)nrhs, &one, t,
+                            (rocblas_int)nrhs, &zero, d, (rocblas_int)n_, d, (rocblas_int)n_) != rocblas_status_success)
+            ok = false, err = "rocblas_dgeam failed";
+        if (ok && lib.dgetrs(h, rocblas_operation_transpose, (rocblas_int)n_, (rocblas_int)nrhs, lu_, (rocblas_int)n_, ipiv_, d,
+                             (rocblas_int)n_) != rocblas_status_success)
+            ok = false, err = "rocsolver_dgetrs failed";
+        if (ok && lib.dgeam(h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)nrhs, (rocblas_int)n_, &one, d,
+                            (rocblas_int)n_, &zero, t, (rocblas_int)nrhs, t, (rocblas_int)nrhs) != rocblas_status_success)
+            ok = false, err = "rocblas_dgeam failed";
+        if (ok) ok = hip_ok(hipMemcpy(B, t, sizeof(double) * n_ * nrhs, hipMemcpyDeviceToHost), "hipMemcpy x", err);
+        if (d) (void)hipFree(d);
+        if (t) (void)hipFree(t);
+        return ok;
+    }
     std::vector<double> cm;
     const double *src = B;
     if (nrhs > 1) {
