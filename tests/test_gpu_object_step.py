@@ -239,8 +239,9 @@ def test_cube_trajectory_200_steps(tmp_path):
 def test_compressed_matrix_cache(tmp_path, mesh):
     """With simulation.coeffs_folder_path the compressed matrices go to the reference's .cpmat files
     (H_<n>_<ratio>.cpmat, G_<n>_..., u_p_..., H_inv_...; wavelets.hpp:451-504 layout): the first run writes them,
-    a second run reads all of them and steps identically.  The written G_1 is the oracle's compression of the
-    dumped G_1, bit for bit."""
+    a second run reads all of them and steps identically.  The written G_1 and H_1 are the oracle's compressions
+    of the dumped (host) G_1 and negated H_1, bit for bit: H_n is compressed from the un-negated device copy with
+    the factor -1."""
     import glob
     from cqbem_wavelet import compress
     V, F = mesh
@@ -257,18 +258,20 @@ def test_compressed_matrix_cache(tmp_path, mesh):
     want = sorted([f"{k}_{n}_0.500000.cpmat" for k in ("G", "H") for n in range(L)] +
                   ["u_p_0.500000.cpmat", "H_inv_0.500000.cpmat"])
     assert names == want and info1["compressed_loaded"] == 0
-    with open(folder[0] + "/G_1_0.500000.cpmat", "rb") as fh:
-        raw = fh.read()
-    N = np.frombuffer(raw[:4], np.int32)[0]
-    rows, cols, nnz, outer, inner = np.frombuffer(raw[4:44], np.int64)
-    off = 44
-    val = np.frombuffer(raw[off:off + 8 * nnz], np.float64); off += 8 * nnz
-    rp = np.frombuffer(raw[off:off + 4 * outer], np.int32); off += 4 * outer
-    col = np.frombuffer(raw[off:off + 4 * nnz], np.int32); off += 4 * nnz
-    assert off == len(raw) and N == n3 and rows == cols == outer == inner
-    ref = compress(m1["G"][1], int(n3 * n3 * 0.5))
-    assert ref.shape[0] == rows and ref.nnz == nnz
-    assert np.array_equal(rp, ref.indptr[:-1]) and np.array_equal(col, ref.indices) and np.array_equal(val, ref.data)
+    for name, dense in (("G_1", m1["G"][1]), ("H_1", m1["H"][1])):
+        with open(folder[0] + f"/{name}_0.500000.cpmat", "rb") as fh:
+            raw = fh.read()
+        N = np.frombuffer(raw[:4], np.int32)[0]
+        rows, cols, nnz, outer, inner = np.frombuffer(raw[4:44], np.int64)
+        off = 44
+        val = np.frombuffer(raw[off:off + 8 * nnz], np.float64); off += 8 * nnz
+        rp = np.frombuffer(raw[off:off + 4 * outer], np.int32); off += 4 * outer
+        col = np.frombuffer(raw[off:off + 4 * nnz], np.int32); off += 4 * nnz
+        assert off == len(raw) and N == n3 and rows == cols == outer == inner, name
+        ref = compress(dense, int(n3 * n3 * 0.5))
+        assert ref.shape[0] == rows and ref.nnz == nnz, name
+        assert np.array_equal(rp, ref.indptr[:-1]) and np.array_equal(col, ref.indices), name
+        assert np.array_equal(val, ref.data), name
     m2, traj2, info2 = simulate(tmp_path, V, F, obj, tractions, **sim)  # same scene, same cache key
     assert info2["cached"] and info2["compressed_loaded"] == 2 * L + 2
     assert np.array_equal(traj1, traj2)
