@@ -155,7 +155,9 @@ int cqbem_matvec(cqbem_context *ctx, int64_t n, const double *A, const double *x
  * caller-provided device arrays rowptr (2np - 1 entries), col and val (num_elems entries).  *nnz (host)
  * receives the count kept (num_elems, or fewer when fewer coefficients are nonzero).  Coefficients tied
  * with the smallest kept magnitude are kept in row-major order.  The reference keeps
- * (int)(n * n * compression_ratio) (wavelets.hpp:443-445). */
+ * (int)(n * n * compression_ratio) (wavelets.hpp:443-445); when that count is <= 0 it keeps every nonzero
+ * coefficient (decompose_mat, wavelets.hpp:168-184), and so does this call: col and val then need np * np
+ * entries. */
 int cqbem_wavelet_compress(cqbem_context *ctx, int64_t n, const double *A, int64_t num_elems, int64_t *rowptr, int32_t *col,
                            double *val, int64_t *nnz);
 /* y = M x for a CSR device matrix of `rows` rows (the product in the compressed space,

@@ -58,7 +58,7 @@ def compress(A, num_elems: int):
     r, c, v = candidates(A)
     nz = v != 0
     r, c, v = r[nz], c[nz], v[nz]
-    if num_elems < len(v):
+    if 0 < num_elems < len(v):  # num_elems <= 0 keeps every nonzero coefficient (wavelets.hpp:168-184)
         keep = np.zeros(len(v), dtype=bool)
         if num_elems > 0:
             mag = np.abs(v)

@@ -93,6 +93,7 @@ void check_inputs(const cqbem_mesh *mesh, const cqbem_params *p) {
     if (p->order < 1 || p->order > 5) throw Err{CQBEM_ERR_ARG, "order must be in [1, 5]"};
     if (p->subdivision < 1 || p->subdivision > 16) throw Err{CQBEM_ERR_ARG, "subdivision must be in [1, 16]"};
     if (p->precision != CQBEM_FP64 && p->precision != CQBEM_MIXED) throw Err{CQBEM_ERR_ARG, "unknown precision"};
+    if (p->precision == CQBEM_MIXED) throw Err{CQBEM_ERR_UNSUPPORTED, "CQBEM_MIXED is not implemented"};
     if ((p->which & ~CQBEM_ALL) != 0) throw Err{CQBEM_ERR_ARG, "unknown matrix bits in `which`"};
     // the device sincos reduces |Im(s) r / c| <= max|s_l| diam / min(c1, c2) exactly below kTrigArgMax
     double lo[3] = {mesh->V[0], mesh->V[1], mesh->V[2]}, hi[3] = {lo[0], lo[1], lo[2]};
@@ -716,7 +717,7 @@ int cqbem_wavelet_compress(cqbem_context *ctx, int64_t n, const double *A, int64
     if (!ctx) return CQBEM_ERR_ARG;
     std::lock_guard<std::mutex> lk(ctx->mu);
     try {
-        if (n < 2 || n > (int64_t)1 << 24 || num_elems < 0 || !A || !rowptr || !nnz || (num_elems > 0 && (!col || !val)))
+        if (n < 2 || n > (int64_t)1 << 24 || !A || !rowptr || !nnz || !col || !val)
             throw Err{CQBEM_ERR_ARG, "wavelet_compress: bad size or null pointer"};
         HIPCHK(hipSetDevice(ctx->device));
         *nnz = wavelet_compress_into(A, n, num_elems, 1.0, rowptr, col, val, ctx->stream);
@@ -938,13 +939,20 @@ int cqbem_poll(cqbem_context *ctx, uint64_t job_id, int32_t *done) {
 int cqbem_wait(cqbem_context *ctx, uint64_t job_id) {
     if (!ctx) return CQBEM_ERR_ARG;
     std::shared_ptr<JobRecord> job;
+    std::thread worker;
     {
+        // the first waiter takes the worker thread out of the record and joins it; later waiters on the same
+        // job wait for its completion flag instead (a std::thread is joined once)
         std::lock_guard<std::mutex> lk(ctx->mu);
         auto it = ctx->jobs.find(job_id);
         if (it == ctx->jobs.end()) return CQBEM_ERR_NOTFOUND;
         job = it->second;
+        worker = std::move(job->worker);
     }
-    if (job->worker.joinable()) job->worker.join();
+    if (worker.joinable())
+        worker.join();
+    else
+        while (!job->done.load()) std::this_thread::sleep_for(std::chrono::microseconds(100));
     std::lock_guard<std::mutex> lk(ctx->mu);
     for (int k = 0; k < (int)ctx->slot_job.size(); k++)
         if (ctx->slot_job[k] == job_id) ctx->slot_job[k] = 0;

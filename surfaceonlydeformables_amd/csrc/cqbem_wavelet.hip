@@ -251,7 +251,7 @@ int64_t wavelet_compress_into(const double *A, int64_t n, int64_t num_elems, dou
         if (p == 0) {
             int64_t nonzero = 0;
             for (int b = 0; b < nb; b++) nonzero += h[b];
-            if (num_elems >= nonzero) {  // everything nonzero is kept
+            if (num_elems <= 0 || num_elems >= nonzero) {  // everything nonzero is kept (num_elems <= 0: wavelets.hpp:180-184)
                 T = 0, remaining = 0;
                 break;
             }
@@ -267,7 +267,6 @@ int64_t wavelet_compress_into(const double *A, int64_t n, int64_t num_elems, dou
         prefix = (prefix << widths[p]) | (uint64_t)d;
         if (p == 5) T = prefix;
     }
-    if (num_elems <= 0) T = ~0ull >> 1, remaining = 0;  // nothing kept
     // per-row counts, the row pointers and the tie quota of each row (ties kept in row-major order)
     Dev<int64_t> d_rstart((size_t)rows), d_rcol((size_t)rows);
     Dev<int32_t> d_rlen((size_t)rows), d_gt((size_t)rows), d_eq((size_t)rows), d_quota((size_t)rows);
@@ -289,7 +288,7 @@ int64_t wavelet_compress_into(const double *A, int64_t n, int64_t num_elems, dou
         rp[r + 1] = rp[r] + gt[r] + quota[r];
     }
     const int64_t nnz = rp[rows];
-    if (nnz > std::max<int64_t>(num_elems, 0)) throw std::runtime_error("wavelet selection kept more than requested");
+    if (nnz > (num_elems > 0 ? num_elems : total)) throw std::runtime_error("wavelet selection kept more than requested");
     check(hipMemcpyAsync(rowptr, rp.data(), sizeof(int64_t) * (rows + 1), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
     check(hipMemcpyAsync(d_quota.p, quota.data(), sizeof(int32_t) * rows, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
     hipLaunchKernelGGL(k_row_fill, dim3(rgrid), dim3(256), 0, st, cand.p, d_rstart.p, d_rlen.p, d_rcol.p, d_quota.p, rowptr,
@@ -302,12 +301,13 @@ int64_t wavelet_compress_into(const double *A, int64_t n, int64_t num_elems, dou
 void wavelet_compress(const double *A, int64_t n, int64_t num_elems, double scale, DeviceCsr &out, hipStream_t st) {
     out.release();
     const int64_t np = wavelet_padded(n), rows = 2 * np - 2;
-    const int64_t cap = std::max<int64_t>(1, std::min<int64_t>(num_elems, np * np));
+    const int64_t cap = std::max<int64_t>(1, num_elems > 0 ? std::min<int64_t>(num_elems, np * np) : np * np);
     check(hipMalloc(&out.rowptr, sizeof(int64_t) * (rows + 1)), "hipMalloc rowptr");
     check(hipMalloc(&out.col, sizeof(int32_t) * cap), "hipMalloc col");
     check(hipMalloc(&out.val, sizeof(double) * cap), "hipMalloc val");
     out.n = n, out.rows = rows;
-    out.nnz = wavelet_compress_into(A, n, std::min<int64_t>(num_elems, np * np), scale, out.rowptr, out.col, out.val, st);
+    out.nnz = wavelet_compress_into(A, n, num_elems > 0 ? std::min<int64_t>(num_elems, np * np) : 0, scale, out.rowptr, out.col,
+                                    out.val, st);
 }
 
 hipError_t launch_csr_matvec(const int64_t *rowptr, const int32_t *col, const double *val, const double *x, int64_t rows,

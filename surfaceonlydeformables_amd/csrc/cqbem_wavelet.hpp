@@ -38,12 +38,14 @@ struct DeviceCsr {
 // np for an n x n matrix: the power of two >= n (std::round(std::pow(2, std::ceil(std::log2(n))))).
 int64_t wavelet_padded(int64_t n);
 
-// Compresses the row-major n x n device matrix A (n >= 2) keeping num_elems coefficients; `scale` multiplies
+// Compresses the row-major n x n device matrix A (n >= 2) keeping num_elems coefficients (num_elems <= 0: every
+// nonzero coefficient, as decompose_mat does without a count, wavelets.hpp:180-184); `scale` multiplies
 // the kept values (exact for -1, which gives the compression of -A).  Allocates out's arrays.  Runs on
 // stream st and returns after it has finished.  Throws std::runtime_error on HIP errors.
 void wavelet_compress(const double *A, int64_t n, int64_t num_elems, double scale, DeviceCsr &out, hipStream_t st);
 
-// Same, into caller-provided arrays: rowptr (rows + 1), col and val (capacity num_elems); returns nnz.
+// Same, into caller-provided arrays: rowptr (rows + 1), col and val (capacity num_elems, or np^2 when
+// num_elems <= 0); returns nnz.
 int64_t wavelet_compress_into(const double *A, int64_t n, int64_t num_elems, double scale, int64_t *rowptr, int32_t *col,
                               double *val, hipStream_t st);
 

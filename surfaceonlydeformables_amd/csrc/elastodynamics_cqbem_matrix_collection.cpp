@@ -809,6 +809,13 @@ bool prepare_mesh(const Json &m, std::vector<double> &v, std::vector<int32_t> &f
         return err = "mesh.subdivision_level is not supported: subdivide the mesh before loading it", false;
     if (m.contains("decimation_scale") && m["decimation_scale"].number() < 1.0)
         return err = "mesh.decimation_scale is not supported: decimate the mesh before loading it", false;
+    // lines 158-196: a build with OpenCCL reorders the vertices (default on); without OpenCCL the reference
+    // ignores the flag.  The cache key does not record the order, so an explicit request is refused rather
+    // than silently producing (or reading cached) matrices in another vertex order.
+    if (m.contains("enable_vertex_reordering") && m["enable_vertex_reordering"].boolean())
+        return err = "mesh.enable_vertex_reordering = true is not supported: reorder the OBJ beforehand (the shipped "
+                     "scenes ship pre-reordered meshes and set it to false)",
+               false;
     if (m.contains("flip_normal") && m["flip_normal"].boolean())  // line 152
         for (size_t t = 0; t < f.size(); t += 3) std::swap(f[t + 1], f[t + 2]);
     if (m.contains("enable_traction_discontinuity") && m["enable_traction_discontinuity"].boolean()) {
@@ -938,13 +945,19 @@ bool ElastodynamicsCQBEMMatrixCollection::run(const Json &config) {
     if (!ctx) return fail(err);
     free_resident();
     free_assembled();
-    // rocSOLVER's first-use cost, paid beside the assembly and the download instead of inside post_process()
+    // rocSOLVER's first-use cost (its workspace allocations included), paid beside the download of the
+    // assembled matrices instead of inside post_process().  It starts only after cqbem_assemble has returned,
+    // so its allocations never compete with the assembly's, which size the spectra slabs from free memory.
     struct Joined {
         std::thread t;
-        ~Joined() {
+        void start() {
+            if (!t.joinable()) t = std::thread(warm_up_lu);
+        }
+        void join() {
             if (t.joinable()) t.join();
         }
-    } warm{std::thread(warm_up_lu)};
+        ~Joined() { join(); }
+    } warm;
     cqbem_mesh mesh{V.data(), num_vertices, F.data(), (int64_t)(F.size() / 3),
                     vertex_map_inverse.empty() ? nullptr : vertex_map_inverse.data()};
     // with cached G and H only B is assembled (the reference computes B on every init, :146-177)
@@ -974,6 +987,7 @@ bool ElastodynamicsCQBEMMatrixCollection::run(const Json &config) {
             free_assembled();
             return fail(cqbem_last_error(ctx));
         }
+        warm.start();
         DownloadStager stager;
         for (size_t n = 0; n < L; n++) {
             if (!loaded_from_cache) {
@@ -1005,7 +1019,7 @@ bool ElastodynamicsCQBEMMatrixCollection::run(const Json &config) {
         if (loaded_from_cache) free_assembled();  // G and H come from the files: resident() uploads them
         last_assembly_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         (void)cuda_thread_per_block_;  // launch shapes are chosen by the library
-        warm.t.join();
+        warm.join();
         const auto t1 = std::chrono::steady_clock::now();
         if (!post_process()) return false;
         last_factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
@@ -1016,6 +1030,7 @@ bool ElastodynamicsCQBEMMatrixCollection::run(const Json &config) {
     cqbem_outputs out{loaded_from_cache ? nullptr : g.data(), loaded_from_cache ? nullptr : h.data(), nullptr, nullptr,
                       b.data(), 0};
     if (cqbem_assemble(ctx, &mesh, &prm, &out) != CQBEM_OK) return fail(cqbem_last_error(ctx));
+    warm.start();
     for (size_t n = 0; n < L; n++) {
         if (!loaded_from_cache) {
             const double *gs = g.data() + n * n3 * n3, *hs = h.data() + n * n3 * n3;
@@ -1036,7 +1051,7 @@ bool ElastodynamicsCQBEMMatrixCollection::run(const Json &config) {
     }
     last_assembly_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     (void)cuda_thread_per_block_;  // launch shapes are chosen by the library
-    warm.t.join();
+    warm.join();
     const auto t1 = std::chrono::steady_clock::now();
     if (!post_process()) return false;
     last_factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();

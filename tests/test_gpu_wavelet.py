@@ -29,7 +29,7 @@ def gpu_compress(ctx, A, k):
     dev = torch.device("cuda:0")
     tA = torch.from_numpy(np.ascontiguousarray(A)).to(dev)
     rowptr = torch.full((rows + 1,), -1, dtype=torch.int64, device=dev)
-    cap = max(k, 1)
+    cap = k if k > 0 else npad * npad  # k <= 0 keeps every nonzero coefficient (at most np^2)
     col = torch.full((cap,), -1, dtype=torch.int32, device=dev)
     val = torch.full((cap,), float("nan"), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()  # buffers filled on torch's stream; the library runs on its own
@@ -55,6 +55,21 @@ def check_same(ctx, A, k):
 def test_compress_random(ctx, n, frac):
     A = np.random.default_rng(n).standard_normal((n, n))
     check_same(ctx, A, int(n * n * frac))
+
+
+@pytest.mark.parametrize("ratio", [1e-9, 0.1 / (37 * 37)])
+def test_compress_count_below_one_keeps_everything(ctx, ratio):
+    """(int)(n^2 ratio) = 0 keeps every nonzero coefficient, as the reference's decompose_mat does without a
+    positive count (include/wavelets.hpp:168-184): the product is then the dense product."""
+    from cqbem_wavelet import padded, product
+    n = 37
+    A = np.random.default_rng(11).standard_normal((n, n))
+    k = int(n * n * ratio)
+    assert k == 0
+    nnz, rowptr, col, val, want = check_same(ctx, A, k)
+    assert nnz == np.count_nonzero(want.data) and nnz > n * n // 2
+    x = np.random.default_rng(12).standard_normal(n)
+    assert np.allclose(product(want, x), A @ x, rtol=0, atol=1e-12 * np.abs(A @ x).max())
 
 
 def test_compress_ties_and_zeros(ctx):

@@ -82,3 +82,19 @@ def test_ties_keep_row_major_order():
     ties = sorted((ri, ci) for ri, ci, vi in zip(r, c, v) if vi != 0 and abs(vi) == T)
     kept_ties = sorted((ri, ci) for ri, ci, vi in zip(coo.row, coo.col, coo.data) if abs(vi) == T)
     assert kept_ties == ties[:len(kept_ties)]
+
+
+@pytest.mark.parametrize("k", [0, -1])
+def test_nonpositive_count_keeps_every_nonzero(k):
+    """decompose_mat keeps every nonzero coefficient when it gets no positive count (wavelets.hpp:168-184), as a
+    compression ratio with (int)(n^2 ratio) = 0 gives; the product is then the dense product."""
+    n = 13
+    A = np.random.default_rng(2).standard_normal((n, n))
+    A[3:5, :] = 0.0
+    M = compress(A, k)
+    full = compress(A, padded(n) ** 2)
+    assert M.nnz == full.nnz > 0
+    assert np.array_equal(M.indptr, full.indptr) and np.array_equal(M.indices, full.indices)
+    assert np.array_equal(M.data, full.data)
+    x = np.random.default_rng(4).standard_normal(n)
+    assert np.allclose(product(M, x), A @ x, rtol=0, atol=1e-12 * np.abs(A @ x).max())
