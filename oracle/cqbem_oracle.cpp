@@ -248,7 +248,7 @@ template <class T> cx<T> k_u_r(const v3<T> &rh, T r, int i, int j, const KParam<
             (d / (k.c2 * k.c2)) * e2);
 }
 
-// p_hat_ij: Laplace-domain traction kernel (H_collocation.cu:203-234)
+// p_hat_ij: Laplace-domain traction kernel (H_collocation.cu:45-76)
 template <class T> cx<T> k_p(const v3<T> &rh, T r, const v3<T> &n, int i, int j, const KParam<T> &k) {
     const T d = (i == j) ? T(1) : T(0);
     const cx<T> s = k.s;
@@ -266,7 +266,7 @@ template <class T> cx<T> k_p(const v3<T> &rh, T r, const v3<T> &n, int i, int j,
 }
 
 // Static, strongly singular part of p_hat removed by the Guiggiani rule
-// (H_collocation.cu:236-241).
+// (H_collocation.cu:78-83).
 template <class T> T k_p_sing(const v3<T> &rh, T r, const v3<T> &n, int i, int j, const KParam<T> &k) {
     return inv4pi<T>() * (k.c2 * k.c2 / (k.c1 * k.c1)) * (rh[i] * n[j] - rh[j] * n[i]) / (r * r);
 }
@@ -282,7 +282,7 @@ template <class T> cx<T> k_bt_r(const v3<T> &rh, T r, const v3<T> &n, int i, int
            (((rh[i] * n[j] + rh[j] * n[i]) / (T(2) * r)) * (A2 - A1) + (d * dot(rh, n) / r) * (T(1) / (s * s) - A2));
 }
 
-// r * b_hat_ijk, angular fictitious-force kernel (B_angular_collocation.cu:185-204)
+// r * b_hat_ijk, angular fictitious-force kernel (B_angular_collocation.cu:44-63)
 template <class T> cx<T> k_ba_r(const v3<T> &rh, T r, const v3<T> &n, int i, int j, int kk, const KParam<T> &k) {
     auto dl = [](int a, int b) { return a == b ? T(1) : T(0); };
     const cx<T> s = k.s;
@@ -672,7 +672,7 @@ void colloc_Ba_row(const Mesh &m, int i, double c1, double c2, cx<double> s, con
             const int j[3] = {m.F[3 * f], m.F[3 * f + 1], m.F[3 * f + 2]};
             const v3<T> n = cast3<T>(m.N.data() + 3 * f);
             const v3<T> y1 = cast3<T>(m.V + 3 * j[0]), y2 = cast3<T>(m.V + 3 * j[1]), y3 = cast3<T>(m.V + 3 * j[2]);
-            for (int kk = 0; kk < 3; kk++) {  // three separate integrations (B_angular_collocation.cu:238-246)
+            for (int kk = 0; kk < 3; kk++) {  // three separate integrations (B_angular_collocation.cu:97-105)
                 cx<T> acc[9];
                 for (auto &b : acc) b = cx<T>(0);
                 far_rule<T>(y1, y2, y3, q, [&](const v3<T> &y, const v3<T> &, T w) {

@@ -2,8 +2,8 @@
 // Laplace-domain elastodynamic kernels in factored form.
 //
 // The reference evaluates each kernel entry (i,j) separately
-// (G_collocation.cu:44-60, H_collocation.cu:203-234, B_trans_collocation.cu:41-58,
-// B_angular_collocation.cu:185-204).  Here every kernel is written as a few
+// (G_collocation.cu:44-60, H_collocation.cu:45-76, B_trans_collocation.cu:41-58,
+// B_angular_collocation.cu:44-63).  Here every kernel is written as a few
 // complex scalar coefficients times frequency-independent geometric tensors, so
 // one point/frequency evaluation costs two complex exponentials plus ~30 complex
 // operations for all four matrices:

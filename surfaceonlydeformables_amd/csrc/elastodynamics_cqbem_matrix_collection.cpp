@@ -203,6 +203,24 @@ namespace {
 // floating-point numbers quickly and accurately with integers", PLDI 2010) with the scaled boundaries narrowed
 // by one unit, so that the digits read back to the same double; most of the time they are the shortest
 // such digits, and where Grisu2 emits a longer string the cache key must contain that same string.
+//
+// The digit generation, the cached-power table layout and the rounding loop below follow the Grisu2
+// implementation of nlohmann/json 3.9.1 (the reference's vendored include/json.hpp:14600-15300), because the
+// cache key must be byte-identical to the one that library prints.  That code carries this notice:
+//   JSON for Modern C++, https://github.com/nlohmann/json
+//   Licensed under the MIT License <http://opensource.org/licenses/MIT>.  SPDX-License-Identifier: MIT
+//   Copyright (c) 2013-2019 Niels Lohmann <http://nlohmann.me>.
+//   Permission is hereby granted, free of charge, to any person obtaining a copy of this software and
+//   associated documentation files (the "Software"), to deal in the Software without restriction, including
+//   without limitation the rights to use, copy, modify, merge, publish, distribute, sublicense, and/or sell
+//   copies of the Software, and to permit persons to whom the Software is furnished to do so, subject to the
+//   following conditions: The above copyright notice and this permission notice shall be included in all
+//   copies or substantial portions of the Software.
+//   THE SOFTWARE IS PROVIDED "AS IS", WITHOUT WARRANTY OF ANY KIND, EXPRESS OR IMPLIED, INCLUDING BUT NOT
+//   LIMITED TO THE WARRANTIES OF MERCHANTABILITY, FITNESS FOR A PARTICULAR PURPOSE AND NONINFRINGEMENT. IN NO
+//   EVENT SHALL THE AUTHORS OR COPYRIGHT HOLDERS BE LIABLE FOR ANY CLAIM, DAMAGES OR OTHER LIABILITY, WHETHER IN
+//   AN ACTION OF CONTRACT, TORT OR OTHERWISE, ARISING FROM, OUT OF OR IN CONNECTION WITH THE SOFTWARE OR THE
+//   USE OR OTHER DEALINGS IN THE SOFTWARE.
 struct DiyFp {
     uint64_t f;
     int e;

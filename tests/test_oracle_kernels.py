@@ -1,5 +1,5 @@
-"""Pins the oracle's Laplace-domain kernels (G_collocation.cu:44-60, H_collocation.cu:203-234,
-B_trans_collocation.cu:41-58, B_angular_collocation.cu:185-204 of the reference) against physics
+"""Pins the oracle's Laplace-domain kernels (G_collocation.cu:44-60, H_collocation.cu:45-76,
+B_trans_collocation.cu:41-58, B_angular_collocation.cu:44-63 of the reference) against physics
 that does not come from the reference's formulas: the Kelvin static limits, the Laplace-domain
 Navier equation, the traction operator, and the divergence identities of the fictitious-force kernels.
 All derivative checks use the long-double oracle mode and 4th-order central differences."""
