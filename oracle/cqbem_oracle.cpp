@@ -236,33 +236,54 @@ template <class T> struct KParam {
     cx<T> s;
 };
 
-// r * u_hat_ij (G_collocation.cu:44-60)
-template <class T> cx<T> k_u_r(const v3<T> &rh, T r, int i, int j, const KParam<T> &k) {
-    const T d = (i == j) ? T(1) : T(0);
+// Frequency- and distance-dependent factors shared by every entry of one point pair.  The reference's
+// integrands evaluate them inside each entry (the same expressions on the same operands); computing them once
+// per point pair gives bit-identical entries.
+template <class T> struct KExp {
+    cx<T> e1, e2, A1, A2;   // e_k = exp(-r s / c_k), A_k = (s r / c_k + 1) / s^2 e_k
+    cx<T> q, e1r2, e2r2, e1s, e2s;  // traction-kernel factors (H_collocation.cu:45-76)
+    cx<T> A2mA1, is2mA2, ec;        // B kernels: A2 - A1, 1/s^2 - A2, e2/c2 - e1/c1
+};
+template <class T> KExp<T> k_exp(T r, const KParam<T> &k) {
+    KExp<T> x;
     const cx<T> s = k.s;
-    const cx<T> e1 = std::exp(-r * s / k.c1), e2 = std::exp(-r * s / k.c2);
-    const cx<T> A1 = ((s * r / k.c1 + T(1)) / (s * s)) * e1;
-    const cx<T> A2 = ((s * r / k.c2 + T(1)) / (s * s)) * e2;
+    const T c1 = k.c1, c2 = k.c2;
+    x.e1 = std::exp(-r * s / c1), x.e2 = std::exp(-r * s / c2);
+    x.A1 = ((s * r / c1 + T(1)) / (s * s)) * x.e1;
+    x.A2 = ((s * r / c2 + T(1)) / (s * s)) * x.e2;
+    const cx<T> rs = r * s;
+    x.q = (x.e1 / rs) * (T(1) / c1 + T(1) / rs) - (x.e2 / rs) * (T(1) / c2 + T(1) / rs);
+    x.e1r2 = x.e1 / (r * r), x.e2r2 = x.e2 / (r * r);
+    x.e1s = x.e1 * s / (r * c1), x.e2s = x.e2 * s / (r * c2);
+    x.A2mA1 = x.A2 - x.A1, x.is2mA2 = T(1) / (s * s) - x.A2, x.ec = x.e2 / c2 - x.e1 / c1;
+    return x;
+}
+
+// r * u_hat_ij (G_collocation.cu:44-60)
+template <class T> cx<T> k_u_r(const v3<T> &rh, T r, int i, int j, const KParam<T> &k, const KExp<T> &x) {
+    const T d = (i == j) ? T(1) : T(0);
     return (inv4pi<T>() / k.rho) *
-           (((T(3) * rh[i] * rh[j] - d) / (r * r)) * (A1 - A2) + (rh[i] * rh[j]) * (e1 / (k.c1 * k.c1) - e2 / (k.c2 * k.c2)) +
-            (d / (k.c2 * k.c2)) * e2);
+           (((T(3) * rh[i] * rh[j] - d) / (r * r)) * (x.A1 - x.A2) +
+            (rh[i] * rh[j]) * (x.e1 / (k.c1 * k.c1) - x.e2 / (k.c2 * k.c2)) + (d / (k.c2 * k.c2)) * x.e2);
+}
+template <class T> cx<T> k_u_r(const v3<T> &rh, T r, int i, int j, const KParam<T> &k) {
+    return k_u_r(rh, r, i, j, k, k_exp(r, k));
 }
 
 // p_hat_ij: Laplace-domain traction kernel (H_collocation.cu:45-76)
-template <class T> cx<T> k_p(const v3<T> &rh, T r, const v3<T> &n, int i, int j, const KParam<T> &k) {
+template <class T> cx<T> k_p(const v3<T> &rh, T r, const v3<T> &n, int i, int j, const KParam<T> &k, const KExp<T> &x) {
     const T d = (i == j) ? T(1) : T(0);
-    const cx<T> s = k.s;
     const T c1 = k.c1, c2 = k.c2, kap = c2 * c2 / (c1 * c1);
-    const cx<T> e1 = std::exp(-r * s / c1), e2 = std::exp(-r * s / c2);
     const T rn = dot(rh, n);
-    const cx<T> rs = r * s;
-    const cx<T> t1 = ((T(6) * c2 * c2 / (r * r)) * (rh[i] * n[j] + rh[j] * n[i] + (d - T(5) * rh[i] * rh[j]) * rn)) *
-                     ((e1 / rs) * (T(1) / c1 + T(1) / rs) - (e2 / rs) * (T(1) / c2 + T(1) / rs));
-    const cx<T> t2 = (e1 / (r * r)) * (T(2) * kap * (T(2) * rh[i] * n[j] + rh[j] * n[i] - (T(6) * rh[i] * rh[j] - d) * rn) - rh[i] * n[j]);
-    const cx<T> t3 = (e2 / (r * r)) * (T(12) * rh[i] * rh[j] * rn - T(2) * rh[i] * n[j] - T(3) * rh[j] * n[i] - T(3) * d * rn);
-    const cx<T> t4 = (e1 * s / (r * c1)) * (rh[i] * n[j] + T(2) * kap * (rh[i] * rh[j] * rn - rh[i] * n[j]));
-    const cx<T> t5 = (e2 * s / (r * c2)) * (T(2) * rh[i] * rh[j] * rn - d * rn - rh[j] * n[i]);
+    const cx<T> t1 = ((T(6) * c2 * c2 / (r * r)) * (rh[i] * n[j] + rh[j] * n[i] + (d - T(5) * rh[i] * rh[j]) * rn)) * x.q;
+    const cx<T> t2 = x.e1r2 * (T(2) * kap * (T(2) * rh[i] * n[j] + rh[j] * n[i] - (T(6) * rh[i] * rh[j] - d) * rn) - rh[i] * n[j]);
+    const cx<T> t3 = x.e2r2 * (T(12) * rh[i] * rh[j] * rn - T(2) * rh[i] * n[j] - T(3) * rh[j] * n[i] - T(3) * d * rn);
+    const cx<T> t4 = x.e1s * (rh[i] * n[j] + T(2) * kap * (rh[i] * rh[j] * rn - rh[i] * n[j]));
+    const cx<T> t5 = x.e2s * (T(2) * rh[i] * rh[j] * rn - d * rn - rh[j] * n[i]);
     return inv4pi<T>() * (t1 + t2 + t3 - t4 + t5);
+}
+template <class T> cx<T> k_p(const v3<T> &rh, T r, const v3<T> &n, int i, int j, const KParam<T> &k) {
+    return k_p(rh, r, n, i, j, k, k_exp(r, k));
 }
 
 // Static, strongly singular part of p_hat removed by the Guiggiani rule
@@ -272,27 +293,25 @@ template <class T> T k_p_sing(const v3<T> &rh, T r, const v3<T> &n, int i, int j
 }
 
 // r * b_hat_ij, translational fictitious-force kernel (B_trans_collocation.cu:41-58)
-template <class T> cx<T> k_bt_r(const v3<T> &rh, T r, const v3<T> &n, int i, int j, const KParam<T> &k) {
+template <class T> cx<T> k_bt_r(const v3<T> &rh, T r, const v3<T> &n, int i, int j, const KParam<T> &k, const KExp<T> &x) {
     const T d = (i == j) ? T(1) : T(0);
-    const cx<T> s = k.s;
-    const cx<T> e1 = std::exp(-r * s / k.c1), e2 = std::exp(-r * s / k.c2);
-    const cx<T> A1 = ((s * r / k.c1 + T(1)) / (s * s)) * e1;
-    const cx<T> A2 = ((s * r / k.c2 + T(1)) / (s * s)) * e2;
-    return -inv4pi<T>() *
-           (((rh[i] * n[j] + rh[j] * n[i]) / (T(2) * r)) * (A2 - A1) + (d * dot(rh, n) / r) * (T(1) / (s * s) - A2));
+    return -inv4pi<T>() * (((rh[i] * n[j] + rh[j] * n[i]) / (T(2) * r)) * x.A2mA1 + (d * dot(rh, n) / r) * x.is2mA2);
+}
+template <class T> cx<T> k_bt_r(const v3<T> &rh, T r, const v3<T> &n, int i, int j, const KParam<T> &k) {
+    return k_bt_r(rh, r, n, i, j, k, k_exp(r, k));
 }
 
 // r * b_hat_ijk, angular fictitious-force kernel (B_angular_collocation.cu:44-63)
-template <class T> cx<T> k_ba_r(const v3<T> &rh, T r, const v3<T> &n, int i, int j, int kk, const KParam<T> &k) {
+template <class T>
+cx<T> k_ba_r(const v3<T> &rh, T r, const v3<T> &n, int i, int j, int kk, const KParam<T> &k, const KExp<T> &x) {
     auto dl = [](int a, int b) { return a == b ? T(1) : T(0); };
     const cx<T> s = k.s;
-    const cx<T> e1 = std::exp(-r * s / k.c1), e2 = std::exp(-r * s / k.c2);
-    const cx<T> A1 = ((s * r / k.c1 + T(1)) / (s * s)) * e1;
-    const cx<T> A2 = ((s * r / k.c2 + T(1)) / (s * s)) * e2;
     const T X = rh[i] * rh[kk] * n[j] + rh[j] * rh[kk] * n[i] + dl(j, kk) * n[i] + dl(i, kk) * n[j];
     const T Y = dl(j, kk) * n[i] + dl(i, kk) * n[j];
-    return -inv4pi<T>() * ((X / T(2)) * (A2 - A1) - (r * Y) / (T(2) * s) * (e2 / k.c2 - e1 / k.c1) -
-                           (r * dl(i, j) * n[kk]) / (k.c2 * s) * e2);
+    return -inv4pi<T>() * ((X / T(2)) * x.A2mA1 - (r * Y) / (T(2) * s) * x.ec - (r * dl(i, j) * n[kk]) / (k.c2 * s) * x.e2);
+}
+template <class T> cx<T> k_ba_r(const v3<T> &rh, T r, const v3<T> &n, int i, int j, int kk, const KParam<T> &k) {
+    return k_ba_r(rh, r, n, i, j, kk, k, k_exp(r, k));
 }
 
 // ------------------------------------------------------------------ mesh --
@@ -461,9 +480,10 @@ template <class T> void guiggiani_rule(const v3<T> &y1, const v3<T> &y2, const v
             const v3<T> rv = rho * Av;
             const T r = norm(rv);
             const v3<T> rh = unit(rv);
+            const KExp<T> e = k_exp(r, kp);
             for (int p = 0; p < 3; p++)
                 for (int qq = 0; qq < 3; qq++) {
-                    const cx<T> F = k_p(rh, r, n, p, qq, kp);
+                    const cx<T> F = k_p(rh, r, n, p, qq, kp, e);
                     for (int k = 0; k < 3; k++) {
                         cx<T> val = F * (L[k] * J * rho);
                         if (k == 0) val -= fpq[p * 3 + qq] / rho;
@@ -513,10 +533,11 @@ void colloc_G_row(const Mesh &m, int i, double c1, double c2, double rho, cx<dou
                 const v3<T> rv = y - x;
                 const T r = norm(rv);
                 const v3<T> rh = unit(rv);
+                const KExp<T> e = k_exp(r, kt);
                 cx<T> u[9];
                 for (int a = 0; a < 3; a++)
                     for (int b = 0; b <= a; b++) {
-                        u[a * 3 + b] = k_u_r(rh, r, a, b, kt) / r;
+                        u[a * 3 + b] = k_u_r(rh, r, a, b, kt, e) / r;
                         u[b * 3 + a] = u[a * 3 + b];
                     }
                 for (int k = 0; k < 3; k++)
@@ -535,9 +556,10 @@ void colloc_G_row(const Mesh &m, int i, double c1, double c2, double rho, cx<dou
                 for (auto &b : a) b = cx<S>(0);
             polar_rule<S>(y1, y2, y3, q, [&](const v3<S> &y, const v3<S> &rh, S r, S w) {
                 const v3<S> L = bary_clamped(y, y1, y2, y3);
+                const KExp<S> e = k_exp(r, ks);
                 for (int a = 0; a < 3; a++)
                     for (int b = 0; b <= a; b++) {
-                        cx<S> v = k_u_r(rh, r, a, b, ks);
+                        cx<S> v = k_u_r(rh, r, a, b, ks, e);
                         for (int k = 0; k < 3; k++) acc[k][a * 3 + b] += w * (v * L[k]);
                     }
             });
@@ -577,9 +599,10 @@ void colloc_H_row(const Mesh &m, int i, double c1, double c2, cx<double> s, cons
                 const v3<T> rv = y - x;
                 const T r = norm(rv);
                 const v3<T> rh = unit(rv);
+                const KExp<T> e = k_exp(r, kt);
                 cx<T> pv[9];
                 for (int a = 0; a < 3; a++)
-                    for (int b = 0; b < 3; b++) pv[a * 3 + b] = k_p(rh, r, n, a, b, kt);
+                    for (int b = 0; b < 3; b++) pv[a * 3 + b] = k_p(rh, r, n, a, b, kt, e);
                 for (int k = 0; k < 3; k++)
                     for (int e = 0; e < 9; e++) acc[k][e] += (pv[e] * L[k]) * w;
             });
@@ -626,9 +649,10 @@ void colloc_Bt_row(const Mesh &m, int i, double c1, double c2, cx<double> s, con
                 const v3<T> rv = y - x;
                 const T r = norm(rv);
                 const v3<T> rh = unit(rv);
+                const KExp<T> e = k_exp(r, kt);
                 for (int a = 0; a < 3; a++)
                     for (int b = 0; b <= a; b++) {
-                        cx<T> v = (k_bt_r(rh, r, n, a, b, kt) / r) * w;
+                        cx<T> v = (k_bt_r(rh, r, n, a, b, kt, e) / r) * w;
                         acc[a * 3 + b] += v;
                         if (a != b) acc[b * 3 + a] += v;
                     }
@@ -641,8 +665,9 @@ void colloc_Bt_row(const Mesh &m, int i, double c1, double c2, cx<double> s, con
             cx<S> acc[9];
             for (auto &b : acc) b = cx<S>(0);
             polar_rule<S>(y1, y2, y3, q, [&](const v3<S> &, const v3<S> &rh, S r, S w) {
+                const KExp<S> e = k_exp(r, ks);
                 for (int a = 0; a < 3; a++)
-                    for (int b = 0; b <= a; b++) acc[a * 3 + b] += w * k_bt_r(rh, r, n, a, b, ks);
+                    for (int b = 0; b <= a; b++) acc[a * 3 + b] += w * k_bt_r(rh, r, n, a, b, ks, e);
             });
             for (int a = 0; a < 3; a++)
                 for (int b = 0; b <= a; b++) {
@@ -679,9 +704,10 @@ void colloc_Ba_row(const Mesh &m, int i, double c1, double c2, cx<double> s, con
                     const v3<T> rv = y - x;
                     const T r = norm(rv);
                     const v3<T> rh = unit(rv);
+                    const KExp<T> e = k_exp(r, kt);
                     for (int a = 0; a < 3; a++)
                         for (int b = 0; b <= a; b++) {
-                            cx<T> v = (k_ba_r(rh, r, n, a, b, kk, kt) / r) * w;
+                            cx<T> v = (k_ba_r(rh, r, n, a, b, kk, kt, e) / r) * w;
                             acc[a * 3 + b] += v;
                             if (a != b) acc[b * 3 + a] += v;
                         }
@@ -696,9 +722,10 @@ void colloc_Ba_row(const Mesh &m, int i, double c1, double c2, cx<double> s, con
             cx<S> acc[27];
             for (auto &b : acc) b = cx<S>(0);
             polar_rule<S>(y1, y2, y3, q, [&](const v3<S> &, const v3<S> &rh, S r, S w) {
+                const KExp<S> e = k_exp(r, ks);
                 for (int a = 0; a < 3; a++)
                     for (int b = 0; b <= a; b++)
-                        for (int kk = 0; kk < 3; kk++) acc[a * 9 + b * 3 + kk] += w * k_ba_r(rh, r, n, a, b, kk, ks);
+                        for (int kk = 0; kk < 3; kk++) acc[a * 9 + b * 3 + kk] += w * k_ba_r(rh, r, n, a, b, kk, ks, e);
             });
             for (int a = 0; a < 3; a++)
                 for (int b = 0; b <= a; b++)
@@ -723,20 +750,21 @@ template <class T> struct GalKernels {
         const v3<T> rv = y - x;
         const T r = norm(rv);
         const v3<T> rh = unit(rv);
+        const KExp<T> e = k_exp(r, k);
         if (w.G)
             for (int a = 0; a < 3; a++)
-                for (int b = 0; b <= a; b++) u[a * 3 + b] = u[b * 3 + a] = k_u_r(rh, r, a, b, k) / r;
+                for (int b = 0; b <= a; b++) u[a * 3 + b] = u[b * 3 + a] = k_u_r(rh, r, a, b, k, e) / r;
         if (w.H)
             for (int a = 0; a < 3; a++)
-                for (int b = 0; b < 3; b++) p[a * 3 + b] = k_p(rh, r, n, a, b, k);
+                for (int b = 0; b < 3; b++) p[a * 3 + b] = k_p(rh, r, n, a, b, k, e);
         if (w.Bt)
             for (int a = 0; a < 3; a++)
-                for (int b = 0; b <= a; b++) bt[a * 3 + b] = bt[b * 3 + a] = k_bt_r(rh, r, n, a, b, k) / r;
+                for (int b = 0; b <= a; b++) bt[a * 3 + b] = bt[b * 3 + a] = k_bt_r(rh, r, n, a, b, k, e) / r;
         if (w.Ba)
             for (int kk = 0; kk < 3; kk++)
                 for (int a = 0; a < 3; a++)
                     for (int b = 0; b <= a; b++)
-                        ba[a * 9 + 3 * kk + b] = ba[b * 9 + 3 * kk + a] = k_ba_r(rh, r, n, a, b, kk, k) / r;
+                        ba[a * 9 + 3 * kk + b] = ba[b * 9 + 3 * kk + a] = k_ba_r(rh, r, n, a, b, kk, k, e) / r;
     }
 };
 
@@ -1075,8 +1103,11 @@ void laplace_collocation(const Mesh &m, const orc_params &p, cx<double> s, const
     }
 }
 
+// Rows [row0, row1) (vertex range): only the test faces with a corner in the range are integrated, so the rows
+// of those vertices are complete and every other row holds partial sums (row-sample checks at full mesh sizes).
 template <class Pol>
-void laplace_galerkin(const Mesh &m, const orc_params &p, cx<double> s, const Which &w, Laplace<typename Pol::Acc> &out) {
+void laplace_galerkin(const Mesh &m, const orc_params &p, cx<double> s, const Which &w, int row0, int row1,
+                      Laplace<typename Pol::Acc> &out) {
     using T = typename Pol::GalNS;
     using S = typename Pol::Sing;
     using A = typename Pol::Acc;
@@ -1085,44 +1116,97 @@ void laplace_galerkin(const Mesh &m, const orc_params &p, cx<double> s, const Wh
     GalKernels<T> KT{w, KParam<T>{(T)p.c1, (T)p.c2, (T)p.rho, cx<T>((T)s.real(), (T)s.imag())}};
     GalKernels<S> KS{w, KParam<S>{(S)p.c1, (S)p.c2, (S)p.rho, cx<S>((S)s.real(), (S)s.imag())}};
     out.zero();
-    // Test faces f1 own their rows except for vertices shared with other test
-    // faces, so the scatter is serialized per f1 (the reference serializes with
-    // omp critical, G_galerkin_non_singular.cu:84).
+    // One test face f1 at a time: its source faces are split over the threads, each summing into a row block
+    // of its own (the 9 rows of f1's corners); the blocks are then added into the output rows in thread order.
+    // The reference serializes the same scatter with omp critical (G_galerkin_non_singular.cu:84).
     const int nth =
 #ifdef _OPENMP
         omp_get_max_threads();
 #else
         1;
 #endif
-    std::vector<Laplace<A>> part(nth);
+    const int n3 = out.n3;
+    struct RowBlock {
+        std::vector<cx<A>> U, P, Bt, Ba;  // 9 x n3, 9 x n3, 9 x 3, 9 x 9 (row 3c + a for corner c of f1)
+    };
+    std::vector<RowBlock> blk(nth);
+    for (auto &b : blk) {
+        if (w.G) b.U.resize(9 * (size_t)n3);
+        if (w.H) b.P.resize(9 * (size_t)n3);
+        if (w.Bt) b.Bt.resize(27);
+        if (w.Ba) b.Ba.resize(81);
+    }
+    for (int f1 = 0; f1 < m.nf; f1++) {
+        bool in_range = false;
+        for (int c = 0; c < 3; c++) in_range |= m.F[3 * f1 + c] >= row0 && m.F[3 * f1 + c] < row1;
+        if (!in_range) continue;
 #pragma omp parallel
-    {
-        int tid =
+        {
+            int tid =
 #ifdef _OPENMP
-            omp_get_thread_num();
+                omp_get_thread_num();
 #else
-            0;
+                0;
 #endif
-        Laplace<A> &mine = part[tid];
-        mine.alloc(m.nv, w);
-        PairAcc<T> accT;
-        PairAcc<S> accS;
-#pragma omp for schedule(dynamic, 1)
-        for (int f1 = 0; f1 < m.nf; f1++)
+            RowBlock &mine = blk[tid];
+            for (auto *v : {&mine.U, &mine.P, &mine.Bt, &mine.Ba}) std::fill(v->begin(), v->end(), cx<A>(0));
+            PairAcc<T> accT;
+            PairAcc<S> accS;
+            // the corners of f1 in the order a rule reports them (singular rules permute them) -> local block rows
+            auto add = [&](const auto &acc, const int j1[3], const int j2[3]) {
+                for (int i = 0; i < 3; i++) {
+                    int c = 0;
+                    while (m.F[3 * f1 + c] != j1[i]) c++;
+                    for (int a = 0; a < 3; a++) {
+                        const size_t row = (size_t)(3 * c + a);
+                        if (w.Bt)
+                            for (int b = 0; b < 3; b++) {
+                                const auto v = acc.Bt[(3 * i + a) * 3 + b];
+                                mine.Bt[row * 3 + b] += cx<A>((A)v.real(), (A)v.imag());
+                            }
+                        if (w.Ba)
+                            for (int kk = 0; kk < 3; kk++)
+                                for (int b = 0; b < 3; b++) {
+                                    const auto v = acc.Ba[(3 * i + a) * 9 + 3 * kk + b];
+                                    mine.Ba[row * 9 + b * 3 + kk] += cx<A>((A)v.real(), (A)v.imag());
+                                }
+                        for (int j = 0; j < 3; j++)
+                            for (int b = 0; b < 3; b++) {
+                                if (w.G) {
+                                    const auto v = acc.U[(3 * i + a) * 9 + 3 * j + b];
+                                    mine.U[row * n3 + 3 * j2[j] + b] += cx<A>((A)v.real(), (A)v.imag());
+                                }
+                                if (w.H) {
+                                    const auto v = acc.P[(3 * i + a) * 9 + 3 * j + b];
+                                    mine.P[row * n3 + 3 * j2[j] + b] += cx<A>((A)v.real(), (A)v.imag());
+                                }
+                            }
+                    }
+                }
+            };
+#pragma omp for schedule(static, 1)  // cyclic: the expensive touching pairs spread over the threads
             for (int f2 = 0; f2 < m.nf; f2++) {
                 if (galerkin_nonsingular_pair<T>(m, f1, f2, KT, q, ref, accT)) {
-                    scatter_pair<T, A>(w, accT, m.F + 3 * f1, m.F + 3 * f2, mine);
+                    add(accT, m.F + 3 * f1, m.F + 3 * f2);
                 } else {
                     int j1[3], j2[3];
-                    if (galerkin_singular_pair<S>(m, f1, f2, KS, q, accS, j1, j2)) scatter_pair<S, A>(w, accS, j1, j2, mine);
+                    if (galerkin_singular_pair<S>(m, f1, f2, KS, q, accS, j1, j2)) add(accS, j1, j2);
                 }
             }
-    }
-    for (auto &pt : part) {
-        for (size_t k = 0; k < out.U.size(); k++) out.U[k] += pt.U[k];
-        for (size_t k = 0; k < out.P.size(); k++) out.P[k] += pt.P[k];
-        for (size_t k = 0; k < out.Bt.size(); k++) out.Bt[k] += pt.Bt[k];
-        for (size_t k = 0; k < out.Ba.size(); k++) out.Ba[k] += pt.Ba[k];
+        }
+        for (const auto &b : blk)
+            for (int c = 0; c < 3; c++)
+                for (int a = 0; a < 3; a++) {
+                    const size_t lr = (size_t)(3 * c + a), gr = (size_t)(3 * m.F[3 * f1 + c] + a);
+                    if (w.G)
+                        for (int k = 0; k < n3; k++) out.U[gr * n3 + k] += b.U[lr * n3 + k];
+                    if (w.H)
+                        for (int k = 0; k < n3; k++) out.P[gr * n3 + k] += b.P[lr * n3 + k];
+                    if (w.Bt)
+                        for (int k = 0; k < 3; k++) out.Bt[gr * 3 + k] += b.Bt[lr * 3 + k];
+                    if (w.Ba)
+                        for (int k = 0; k < 9; k++) out.Ba[gr * 9 + k] += b.Ba[lr * 9 + k];
+                }
     }
 }
 
@@ -1130,7 +1214,7 @@ template <class Pol>
 void laplace_any(const Mesh &m, const orc_params &p, cx<double> s, const Which &w, int row0, int row1,
                  Laplace<typename Pol::Acc> &out) {
     if (p.galerkin)
-        laplace_galerkin<Pol>(m, p, s, w, out);
+        laplace_galerkin<Pol>(m, p, s, w, row0, row1, out);
     else
         laplace_collocation<Pol>(m, p, s, w, row0, row1, out);
 }
@@ -1345,9 +1429,9 @@ int orc_transform(const double *spec, long long M, int L, double R, double *W) {
     return 0;
 }
 
-// Laplace-domain matrices for one complex frequency s.  Collocation computes rows
-// [row0,row1) (vertex range); Galerkin ignores the range.  Outputs complex interleaved,
-// full size (3nv x 3nv, 3nv x 3, 3nv x 9); rows outside the range are left untouched.
+// Laplace-domain matrices for one complex frequency s, complete in the rows of vertices [row0,row1)
+// (collocation: those rows only, the others left untouched; Galerkin: the test faces touching the range,
+// the other rows then hold partial sums).  Outputs complex interleaved, full size (3nv x 3nv, 3nv x 3, 3nv x 9).
 int orc_laplace(const double *V, int nv, const int *F, int nf, const int *vmap, const orc_params *p, double s_re, double s_im,
                 int mask, int row0, int row1, double *U, double *P, double *Bt, double *Ba) {
     int rc = check_params(p, nv, nf);

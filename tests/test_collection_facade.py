@@ -10,6 +10,7 @@ import subprocess
 import numpy as np
 import pytest
 
+from post_numpy import tetra_inertia as _tetra_inertia
 from surfaceonlydeformables_amd import mesh as M
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -76,25 +77,6 @@ def test_facade_matches_oracle(tmp_path, oracle):
     assert rel(G, ref["G"]) < 1e-9
     assert rel(-H, ref["H"]) < 1e-8
     assert rel(B, ref["B6"]) < 1e-9
-
-
-def _tetra_inertia(V, F, rho):
-    """Exact inertia tensor of the closed polyhedron about its centroid (signed tetrahedra from the origin)."""
-    vol, first, second = 0.0, np.zeros(3), np.zeros((3, 3))
-    for f in F:
-        a, b, c = V[f[0]], V[f[1]], V[f[2]]
-        det = np.dot(a, np.cross(b, c))
-        vol += det / 6
-        first += det / 24 * (a + b + c)
-        s = a + b + c
-        second += det / 120 * (np.outer(a, a) + np.outer(b, b) + np.outer(c, c) + np.outer(s, s))
-    cm = first / vol
-    C = second - vol * np.outer(cm, cm)
-    return rho * (np.trace(C) * np.eye(3) - C), vol
-
-
-def _cross(v):
-    return np.array([[0, -v[2], v[1]], [v[2], 0, -v[0]], [-v[1], v[0], 0]])
 
 
 @pytest.mark.gpu

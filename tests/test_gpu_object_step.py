@@ -169,29 +169,33 @@ def test_object_step_matches_oracle(tmp_path, mesh, case):
     assert np.linalg.norm(o.V_local - m["V"]) > 1e-6
 
 
-def test_cube_trajectory_200_steps(tmp_path):
-    """The k = 8 cube (386 vertices, side 1 m) for 200 steps: a 10-step upward pressure pulse of 4e4 Pa on its
-    bottom face (what a floor contact applies), gravity, an initial spin.  Every vertex of V_global stays within
-    1e-8 m of the oracle's at every step.  E = 1e7 Pa (nu = 0.3, rho = 1000, dt = 1e-3 s, BDF2) with the
-    mesh-derived L = 30: with E = 1e5 Pa waves need ~280 steps to cross the cube, and L = 32 truncates the
-    convolution so that the step diverges (both implementations do; the LU-solve and explicit-inverse variants
-    of the oracle then part by 1e-3 m at step 150 while here they agree to 2e-15 m)."""
+CUBE_E, CUBE_NU, CUBE_RHO, CUBE_DT, CUBE_STEPS = 1e7, 0.3, 1000.0, 1e-3, 200
+CUBE_OBJ = {"translational_vel": [0.0, -0.5, 0.0], "rotational_vel": [0.3, 0.0, 0.2], "translation": [0.0, 1.0, 0.0]}
+
+
+def cube_run(tmp_path):
+    """The k = 8 cube (386 vertices, side 1 m) through the facade and the GPU object for 200 steps: a 10-step
+    upward pressure pulse of 4e4 Pa on its bottom face (what a floor contact applies), gravity, an initial spin.
+    E = 1e7 Pa (nu = 0.3, rho = 1000, dt = 1e-3 s, BDF2) with the mesh-derived L = 30: with E = 1e5 Pa waves need
+    ~280 steps to cross the cube, and L = 32 truncates the convolution so that the step diverges (both
+    implementations do; the LU-solve and explicit-inverse variants of the oracle then part by 1e-3 m at step 150
+    while here they agree to 2e-15 m).  Returns the facade's dumped matrices (as ObjectStep reads them), the
+    trajectory, the tractions and the run info."""
     V, F = M.cube(8, 0.5)
-    steps, nv = 200, len(V)
+    steps, nv = CUBE_STEPS, len(V)
     Vc = M.center(V, F)
     p = np.zeros((steps, nv, 3))
     bottom = Vc[:, 1] < Vc[:, 1].min() + 1e-9
     p[20:30, bottom, 1] = 4e4
     tractions = p.reshape(steps, 3 * nv)
-    obj = {"translational_vel": [0.0, -0.5, 0.0], "rotational_vel": [0.3, 0.0, 0.2], "translation": [0.0, 1.0, 0.0]}
-    mu = 1e7 / (2 * 1.3)
+    mu = CUBE_E / (2 * (1 + CUBE_NU))
     obj_path = tmp_path / "cube.obj"
     M.write_obj(str(obj_path), V, F)
     cfg = {"enable_cuda": True, "mesh": {"file": str(obj_path)},
-           "material": {"rho": 1000.0, "mu": mu, "nu": 0.3},
-           "simulation": {"dt": 1e-3, "gaussian_quadrature_order": 5, "quadrature_subdivision": 2,
+           "material": {"rho": CUBE_RHO, "mu": mu, "nu": CUBE_NU},
+           "simulation": {"dt": CUBE_DT, "gaussian_quadrature_order": 5, "quadrature_subdivision": 2,
                           "multistep_method": "BDF2", "use_galerkin": False},
-           "object": obj, "gravity": GRAVITY}
+           "object": CUBE_OBJ, "gravity": GRAVITY}
     path = tmp_path / "scene.json"
     path.write_text(json.dumps(cfg))
     tr = tmp_path / "tractions.bin"
@@ -217,10 +221,16 @@ def test_cube_trajectory_200_steps(tmp_path):
     m["H_inv_B_trans"] = post("H_inv_B_trans", (n3, 3))
     m["H_inv_B_euler"] = post("H_inv_B_euler", (n3, 3))
     traj = np.fromfile(tmp_path / "out_traj.bin").reshape(steps, 24 + 6 * nv)
+    return V, F, m, traj, tractions, info
+
+
+def step_against(m, traj, tractions, nv, tol=1e-8):
+    """Steps oracle/cqbem_object.ObjectStep on the matrices m beside the recorded trajectory; every vertex of
+    V_global within tol (m) of it at every step.  Returns the worst difference and the final object."""
     from cqbem_object import ObjectStep
-    o = ObjectStep(m, 1e-3, GRAVITY, obj)
+    o = ObjectStep(m, CUBE_DT, GRAVITY, CUBE_OBJ)
     worst = 0.0
-    for k in range(steps):
+    for k in range(len(tractions)):
         o.estimate_next_state()
         if np.any(tractions[k] != 0):
             o.update_V_estimate_matrices()
@@ -229,11 +239,45 @@ def test_cube_trajectory_200_steps(tmp_path):
         o.confirm_next_state()
         got = split(traj[k], nv)
         worst = max(worst, np.abs(got["V_global"] - o.V_global).max())
-        assert np.abs(got["V_global"] - o.V_global).max() < 1e-8, (k, worst)
+        assert np.abs(got["V_global"] - o.V_global).max() < tol, (k, worst)
+    return worst, o
+
+
+def test_cube_trajectory_200_steps(tmp_path):
+    """The time stepping alone: the oracle's object steps on the matrices the facade assembled on the GPU."""
+    V, F, m, traj, tractions, info = cube_run(tmp_path)
+    worst, o = step_against(m, traj, tractions, len(V))
     deform = np.abs(o.V_local - m["V"]).max()
     print(f"cube 200 steps: worst vertex difference {worst:.2e} m, final deformation {deform:.2e} m, "
           f"step {info['step_ms']:.3f} ms")
     assert deform > 1e-6  # the pulse left the cube ringing
+
+
+@pytest.mark.timeout(600)
+def test_cube_trajectory_200_steps_oracle_assembled(tmp_path, oracle):
+    """End to end against the CPU precompute: the oracle assembles G, H and B of the cube in FP64 (same L and R
+    as the facade derives, order 5 x subdivision 2), numpy derives u_p, H_0^-1 B and the least-squares products
+    from them (tests/post_numpy.py, as the reference's init does), and the oracle's object steps on those.  The
+    facade's GPU assembly and GPU object must stay within 1e-8 m of that trajectory at every vertex and step."""
+    from post_numpy import time_stepping_matrices
+    V, F, m_gpu, traj, tractions, info = cube_run(tmp_path)
+    nv, L = len(V), info["L"]
+    Vc = M.center(V, F)
+    mu = CUBE_E / (2 * (1 + CUBE_NU))
+    lam = 2 * CUBE_NU * mu / (1 - 2 * CUBE_NU)
+    c1, c2 = float(np.sqrt((lam + 2 * mu) / CUBE_RHO)), float(np.sqrt(mu / CUBE_RHO))
+    R = M.contour_radius(L - 1)
+    print(f"oracle assembly of the cube: nv={nv} L={L}", flush=True)
+    ref = oracle.assemble(Vc, F, oracle.params(c1, c2, CUBE_RHO, CUBE_NU, CUBE_DT, R, L, nthreads=0))
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    errs = {"G": rel(m_gpu["G"], ref["G"]), "H": rel(-m_gpu["H"], ref["H"]), "B": rel(m_gpu["B"], ref["B6"])}
+    print("  GPU vs oracle matrices: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    m = time_stepping_matrices(Vc, F, ref["G"], ref["H"], ref["B6"], CUBE_RHO, M.volume_centroid(V, F)[1])
+    worst, o = step_against(m, traj, tractions, nv)
+    deform = np.abs(o.V_local - m["V"]).max()
+    print(f"cube 200 steps, oracle-assembled: worst vertex difference {worst:.2e} m, final deformation "
+          f"{deform:.2e} m", flush=True)
+    assert deform > 1e-6
 
 
 def test_compressed_matrix_cache(tmp_path, mesh):
