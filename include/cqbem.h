@@ -89,6 +89,20 @@ const char *cqbem_last_error(const cqbem_context *ctx);
 /* Blocking assembly of one job (replaces compute_H/G/B_trans/B_angular_matrices + compute_B_euler_matrices). */
 int cqbem_assemble(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params *params, cqbem_outputs *out);
 
+/* cqbem_assemble without any device allocation.  No counterpart in the reference: for callers that cannot
+ * allocate device memory while they run (serving loops that forbid it, code under HIP graph capture in the
+ * caller's own design, harnesses that ban hipMalloc).  The outputs must be caller-provided device buffers
+ * (out->on_device != 0); every other device buffer of the job (mesh copies, rule tables, Laplace-domain spectra
+ * in row slabs, B spectra) is carved from a fixed scratch array of CQBEM_NOALLOC_SCRATCH_MIB MiB that the
+ * library holds on each GPU it is loaded on until the process ends.  No hipMalloc, hipFree or HIP event is
+ * created.  A job whose buffers exceed the array (even with one-row slabs) returns CQBEM_ERR_NOMEM with the size
+ * it needed.  Results are bit-identical to cqbem_assemble's.  Calls on one GPU share its array and run one at a
+ * time within a process (the library serializes them); the array keeps the previous call's contents, and every
+ * kernel writes what it later reads.  Collocation and Galerkin are both accepted. */
+#define CQBEM_NOALLOC_SCRATCH_MIB 256
+int cqbem_assemble_noalloc(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params *params,
+                           const cqbem_outputs *out);
+
 /* Device bytes cqbem_assemble needs for a job (outputs on device included when outputs_on_device != 0). */
 int cqbem_job_footprint(const cqbem_mesh *mesh, const cqbem_params *params, int outputs_on_device, uint64_t *bytes);
 

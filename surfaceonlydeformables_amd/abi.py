@@ -23,7 +23,7 @@ EXPORTED = [
     "cqbem_create", "cqbem_destroy", "cqbem_last_error", "cqbem_assemble", "cqbem_job_footprint",
     "cqbem_laplace", "cqbem_frequencies", "cqbem_last_timings", "cqbem_submit", "cqbem_poll",
     "cqbem_wait", "cqbem_reserved_bytes", "cqbem_history", "cqbem_matvec", "cqbem_split_traction_discontinuity",
-    "cqbem_wavelet_compress", "cqbem_csr_matvec",
+    "cqbem_wavelet_compress", "cqbem_csr_matvec", "cqbem_assemble_noalloc",
 ]
 
 
@@ -72,6 +72,7 @@ def lib():
         L.cqbem_last_error.argtypes = [C.c_void_p]
         L.cqbem_last_error.restype = C.c_char_p
         L.cqbem_assemble.argtypes = [C.c_void_p, P(cqbem_mesh), P(cqbem_params), P(cqbem_outputs)]
+        L.cqbem_assemble_noalloc.argtypes = [C.c_void_p, P(cqbem_mesh), P(cqbem_params), P(cqbem_outputs)]
         L.cqbem_job_footprint.argtypes = [P(cqbem_mesh), P(cqbem_params), C.c_int, P(C.c_uint64)]
         L.cqbem_laplace.argtypes = [C.c_void_p, P(cqbem_mesh), P(cqbem_params), C.c_int32, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p]
@@ -200,6 +201,15 @@ class Context:
             setattr(out, k, p)
         out.on_device = 1
         self._check(lib().cqbem_assemble(self._h, C.byref(mesh), C.byref(params), C.byref(out)))
+
+    def assemble_device_noalloc(self, mesh: cqbem_mesh, params: cqbem_params, ptrs: dict):
+        """assemble_device through cqbem_assemble_noalloc: no device allocation, scratch from the library's fixed
+        per-device array (CQBEM_NOALLOC_SCRATCH_MIB)."""
+        out = cqbem_outputs()
+        for k, p in ptrs.items():
+            setattr(out, k, p)
+        out.on_device = 1
+        self._check(lib().cqbem_assemble_noalloc(self._h, C.byref(mesh), C.byref(params), C.byref(out)))
 
     def laplace(self, mesh: cqbem_mesh, params: cqbem_params, l: int):
         n3 = 3 * mesh.nv

@@ -806,13 +806,15 @@ void zero(cd *p, long long n, int num_cus, hipStream_t st) {
 }
 
 struct EvTimer {
-    hipEvent_t a, b;
+    hipEvent_t a = nullptr, b = nullptr;
     hipStream_t st;
     double *acc;
-    EvTimer(hipStream_t s, double *d) : st(s), acc(d) {
-        (void)hipEventCreate(&a), (void)hipEventCreate(&b), (void)hipEventRecord(a, st);
+    bool on;
+    EvTimer(hipStream_t s, double *d) : st(s), acc(d), on(phase_timers_on()) {
+        if (on) (void)hipEventCreate(&a), (void)hipEventCreate(&b), (void)hipEventRecord(a, st);
     }
     ~EvTimer() {
+        if (!on) return;
         float ms = 0;
         if (hipEventRecord(b, st) == hipSuccess && hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess)
             *acc += ms;

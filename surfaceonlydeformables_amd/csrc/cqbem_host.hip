@@ -32,6 +32,16 @@
 
 using namespace cqbem;
 
+// Device scratch of cqbem_assemble_noalloc: one array per GPU the library is loaded on, held until the process
+// ends.  Host code never uses its name as a device address: the address of each device's copy comes from
+// hipGetSymbolAddress (cqbem_create, with that device current) and is kept in noalloc_scratch_ptr.
+static constexpr size_t kNoAllocScratchBytes = (size_t)CQBEM_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static constexpr int kMaxDevices = 64;
+static std::mutex noalloc_setup_mu;
+static void *noalloc_scratch_ptr[kMaxDevices];  // per device, set once
+static std::mutex noalloc_call_mu[kMaxDevices];  // calls on one device share its array: serialized in-process
+
 namespace {
 
 struct Err {
@@ -405,15 +415,18 @@ int slab_rows(int nv, int Lf, int nmat, size_t bytes) {
 }
 
 struct Timer {
-    hipEvent_t a, b;
+    hipEvent_t a = nullptr, b = nullptr;
     hipStream_t st;
     double *acc;
-    Timer(hipStream_t s, double *dst) : st(s), acc(dst) {
+    bool on;
+    Timer(hipStream_t s, double *dst) : st(s), acc(dst), on(phase_timers_on()) {
+        if (!on) return;
         HIPCHK(hipEventCreate(&a));
         HIPCHK(hipEventCreate(&b));
         HIPCHK(hipEventRecord(a, st));
     }
     ~Timer() {
+        if (!on) return;
         if (hipEventRecord(b, st) == hipSuccess && hipEventSynchronize(b) == hipSuccess) {
             float ms = 0;
             if (hipEventElapsedTime(&ms, a, b) == hipSuccess) *acc += ms;
@@ -459,15 +472,15 @@ uint64_t footprint(const cqbem_mesh *mesh, const cqbem_params *p, int outputs_on
 
 // Runs one job on stream `st`.  `slab_budget` bounds the Laplace-domain scratch.
 void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const cqbem_params *p, const cqbem_outputs *out,
-             size_t slab_budget, double *tm, bool sync_outputs) {
+             size_t slab_budget, double *tm, bool sync_outputs, JobArena *fixed_arena = nullptr) {
     // every device buffer of the job comes from the context's pool and goes back to it when the stream is done
-    // (CQBEM_POOL=0: plain hipMalloc / hipFree per buffer)
+    // (CQBEM_POOL=0: plain hipMalloc / hipFree per buffer), or from the caller's fixed region
     static const bool use_pool = [] {
         const char *e = std::getenv("CQBEM_POOL");
         return !(e && !std::strcmp(e, "0"));
     }();
     JobArena arena(&ctx->pool, st);
-    ArenaScope arena_scope(use_pool ? &arena : nullptr);
+    ArenaScope arena_scope(fixed_arena ? fixed_arena : use_pool ? &arena : nullptr);
     const auto t_host0 = std::chrono::steady_clock::now();
     Prepared P = prepare(mesh, p);
     const int nv = P.nv, L = P.L, Lf = P.Lf;
@@ -640,6 +653,15 @@ int cqbem_create(cqbem_context **out, int device, uint64_t hbm_budget_bytes) {
         if (device < 0 || device >= n) throw Err{CQBEM_ERR_ARG, "no such HIP device"};
         HIPCHK(hipSetDevice(device));
         ctx->device = device;
+        if (device < kMaxDevices) {
+            // the no-allocation entry point's scratch: this device's copy of the array, looked up once
+            std::lock_guard<std::mutex> lk(noalloc_setup_mu);
+            if (!noalloc_scratch_ptr[device]) {
+                void *ptr = nullptr;
+                HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+                noalloc_scratch_ptr[device] = ptr;
+            }
+        }
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, device));
         ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
@@ -775,6 +797,43 @@ int cqbem_assemble(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_param
         if (fixed > avail) throw Err{CQBEM_ERR_NOMEM, "job does not fit the HBM budget"};
         const size_t slab = (size_t)std::min<uint64_t>(avail - fixed, (uint64_t)48 << 30);
         run_job(ctx, ctx->stream, mesh, params, out, slab, tm, true);
+    } catch (const Err &e) {
+        return fail(ctx, e);
+    } catch (const std::exception &e) {
+        return fail(ctx, Err{CQBEM_ERR_HIP, e.what()});
+    }
+    tm[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::copy(tm, tm + 8, ctx->timings);
+    return CQBEM_OK;
+}
+
+int cqbem_assemble_noalloc(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params *params,
+                           const cqbem_outputs *out) {
+    if (!ctx) return CQBEM_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const auto t0 = std::chrono::steady_clock::now();
+    double tm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    try {
+        check_inputs(mesh, params);
+        if (!out || !out->on_device) throw Err{CQBEM_ERR_ARG, "assemble_noalloc: outputs must be device buffers"};
+        void *base = ctx->device < kMaxDevices ? noalloc_scratch_ptr[ctx->device] : nullptr;
+        if (!base) throw Err{CQBEM_ERR_UNSUPPORTED, "assemble_noalloc: no scratch on this device"};
+        // every device buffer of the job but the caller's outputs, with one row slab of spectra
+        const uint64_t fixed = footprint(mesh, params, 1, 0);
+        if (fixed > kNoAllocScratchBytes)
+            throw Err{CQBEM_ERR_NOMEM, "assemble_noalloc: the job needs " + std::to_string(fixed >> 20) +
+                                           " MiB of scratch, more than the " + std::to_string(CQBEM_NOALLOC_SCRATCH_MIB) +
+                                           " MiB maximum"};
+        const size_t slab = (size_t)(kNoAllocScratchBytes - fixed);
+        HIPCHK(hipSetDevice(ctx->device));
+        std::lock_guard<std::mutex> dev_lk(noalloc_call_mu[ctx->device]);
+        struct TimersOff {
+            bool prev = phase_timers_on();
+            TimersOff() { phase_timers_on() = false; }
+            ~TimersOff() { phase_timers_on() = prev; }
+        } timers_off;
+        JobArena region(base, kNoAllocScratchBytes, ctx->stream);
+        run_job(ctx, ctx->stream, mesh, params, out, slab, tm, true, &region);
     } catch (const Err &e) {
         return fail(ctx, e);
     } catch (const std::exception &e) {

@@ -74,6 +74,9 @@ class DevicePool {
 class JobArena {
   public:
     JobArena(DevicePool *pool, hipStream_t st) : pool_(pool), st_(st) {}
+    // Fixed-region mode: every buffer is carved, 256-byte aligned, out of [base, base + cap) in request order
+    // and nothing is allocated or freed; a request beyond the region fails (nullptr).
+    JobArena(void *base, size_t cap, hipStream_t st) : pool_(nullptr), st_(st), base_((char *)base), cap_(cap) {}
     JobArena(const JobArena &) = delete;
     JobArena &operator=(const JobArena &) = delete;
     ~JobArena() {
@@ -81,17 +84,33 @@ class JobArena {
         for (auto &b : held_) pool_->put(b.first, b.second);
     }
     void *get(size_t n) {
+        if (!pool_) {
+            const size_t off = (used_ + 255) & ~(size_t)255;
+            if (off > cap_ || n > cap_ - off) return nullptr;
+            used_ = off + n;
+            return base_ + off;
+        }
         size_t got = 0;
         void *p = pool_->get(n, &got);
         if (p) held_.emplace_back(p, got);
         return p;
     }
+    size_t used() const { return used_; }
 
   private:
     DevicePool *pool_;
     hipStream_t st_;
     std::vector<std::pair<void *, size_t>> held_;
+    char *base_ = nullptr;
+    size_t cap_ = 0, used_ = 0;
 };
+
+// Whether the job running on this thread times its phases with HIP events (off where the caller allows no
+// resource creation beyond the launches themselves).
+inline bool &phase_timers_on() {
+    static thread_local bool on = true;
+    return on;
+}
 
 // The arena of the job running on this thread (set for the duration of a job), or null.
 inline JobArena *&current_arena() {
