@@ -183,13 +183,39 @@ def run_jobs(ctx, jobs, arrivals, torch):
     return lat, time.perf_counter() - t0
 
 
+def scene_graph(ctx, jobs, torch):
+    """All objects through cqbem_assemble_scene (shared tile launches, one HIP graph per wave), every object's
+    outputs allocated up front and resident at the end; returns the wall time of the call."""
+    from surfaceonlydeformables_amd import abi
+    dev = torch.device("cuda:0")
+    outs = []
+    for V, F, p in jobs:
+        n3 = 3 * len(V)
+        outs.append({"G": torch.empty((p.L, n3, n3), dtype=torch.float64, device=dev),
+                     "H": torch.empty((p.L, n3, n3), dtype=torch.float64, device=dev),
+                     "B": torch.empty((p.L, n3, 6), dtype=torch.float64, device=dev)})
+    meshes = [abi.make_mesh(V, F) for V, F, _ in jobs]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.assemble_scene(meshes, [p for *_, p in jobs], [{n: t.data_ptr() for n, t in o.items()} for o in outs])
+    wall = time.perf_counter() - t0
+    last = ctx.last_timings()
+    del outs
+    torch.cuda.empty_cache()
+    return wall, last
+
+
 def scene_main(args):
-    """Multi-object scene: all objects submitted at t = 0, overlapping on the context's stream slots."""
+    """Multi-object scene (16 objects, seed 0): cqbem_assemble_scene (all objects' tiles in shared launches,
+    graph-captured waves) as the line's value, beside the same jobs submitted at once to the serving slots and one
+    after another."""
     import torch
     from surfaceonlydeformables_amd import abi
     jobs = scene_jobs()
     ctx = abi.Context(0)
     entries = sum(2.0 * (3 * len(V)) ** 2 * p.L for V, F, p in jobs)
+    scene_graph(ctx, jobs[:3], torch)  # warm-up
+    graph_wall, graph_t = scene_graph(ctx, jobs, torch)
     run_jobs(ctx, jobs[:2], [0.0, 0.0], torch)  # warm-up
     lat, makespan = run_jobs(ctx, jobs, [0.0] * len(jobs), torch)
     # the same jobs one after another, each alone on the device
@@ -206,31 +232,176 @@ def scene_main(args):
         serial += time.perf_counter() - t0
         del outs
     ctx.close()
-    print(json.dumps({"metric": "multi-object scene: G+H entries/s (all objects submitted at once)",
-                      "value": entries / makespan, "unit": "entries/s", "objects": len(jobs),
-                      "makespan_ms": makespan * 1e3, "serial_sum_ms": serial * 1e3,
-                      "latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3, "p99": float(np.percentile(lat, 99)) * 1e3},
+    print(json.dumps({"metric": "multi-object scene: G+H entries/s (cqbem_assemble_scene)",
+                      "value": entries / graph_wall, "unit": "entries/s", "objects": len(jobs),
+                      "scene_graph_ms": graph_wall * 1e3, "scene_waves": graph_t["gh_launches"],
+                      "scene_waves_ms": graph_t["regular_gh"], "scene_b_graph_ms": graph_t["regular_b"],
+                      "slots_makespan_ms": makespan * 1e3, "serial_sum_ms": serial * 1e3,
+                      "slots_latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3,
+                                           "p99": float(np.percentile(lat, 99)) * 1e3},
                       "scene": [{"nv": len(V), "L": p.L} for V, F, p in jobs], "dtype": "f64", "data": "synthetic"}),
           flush=True)
 
 
-def serving_main(args):
-    """Poisson stream of scene-distribution jobs at `rate` jobs/s for `duration` seconds."""
-    import torch
+SERVING_BUDGET = 200e9  # bytes of HBM the serving context may hold (scratch of running jobs + resident outputs)
+SERVING_STEPS = 500      # time steps an object runs on its resident matrices before it is released
+
+
+def serving_jobs(n, rng, precision):
+    """n jobs of the serving mix: 90% the scene distribution (icosphere L1-4 or cube k 4-16, L in {16, 32, 64}),
+    10% icosphere level 5 (10242 vertices) at L = 16, materials and sizes as the scene's."""
     from surfaceonlydeformables_amd import abi
-    rng = np.random.default_rng(1)
-    n = max(1, int(args.serving * args.duration))
+    from surfaceonlydeformables_amd import mesh as M
+    prec = abi.MIXED if precision == "mixed" else abi.FP64
+    ico5 = M.icosphere(5, 1.0)
+    jobs = []
+    for k in range(n):
+        size = float(np.exp(rng.uniform(np.log(0.05), np.log(0.3))))
+        if rng.uniform() < 0.1:
+            V, F = ico5[0] * size, ico5[1]
+            L = 16
+        else:
+            if rng.uniform() < 0.5:
+                V, F = M.icosphere(int(rng.integers(1, 5)), size)
+            else:
+                V, F = M.cube(int(rng.integers(4, 17)), size)
+            L = int(rng.choice([16, 32, 64]))
+        V = M.center(M.jitter(V, F, 0.01, 1000 + k), F)
+        E = float(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
+        nu = float(rng.uniform(0.2, 0.45))
+        _, c1, c2 = M.material_speeds(E, nu, 1000.0)
+        params = abi.make_params(c1, c2, 1000.0, nu, 1e-3, 1e-10 ** (1.0 / (2 * L)), L, bdf=abi.BDF2, order=5,
+                                 subdivision=2, precision=prec)
+        jobs.append((V, F, params))
+    return jobs
+
+
+def serve_stream(jobs, arrivals, torch, drain_s):
+    """The request loop of the serving mix on one 200 GB context.  Arriving jobs queue; any queued job is submitted
+    (cqbem_submit_resident) as soon as a stream slot and the budget admit it, by its footprint -- scratch plus its
+    resident outputs -- next to the jobs running and the objects resident.  Policy for jobs whose double G + H
+    outputs alone exceed the budget (icosphere L5 at L = 16: 242 GB): their G and H are stored in float (121 GB).
+    A finished job's object runs SERVING_STEPS history steps on its resident matrices (a stepping thread with its own
+    context and stream), then its matrices are released.  Returns per-job records."""
+    import threading
+    from surfaceonlydeformables_amd import abi
+    dev = torch.device("cuda:0")
+    ctx = abi.Context(0, hbm_budget_bytes=int(SERVING_BUDGET))
+    stepper = abi.Context(0)
+    meshes = [abi.make_mesh(V, F) for V, F, _ in jobs]
+    rec = [{"nv": len(V), "L": p.L, "arrival": arrivals[k], "done": None, "f32": False} for k, (V, F, p) in
+           enumerate(jobs)]
+    resident, lock, stop = [], threading.Lock(), threading.Event()
+    released = []
+
+    def step_loop():
+        hist = {}
+        while not stop.is_set() or resident:
+            with lock:
+                objs = list(resident)
+            if not objs:
+                time.sleep(0.001)
+                continue
+            for obj in objs:
+                k, jid, outs, steps = obj["k"], obj["jid"], obj["outs"], obj["steps"]
+                n3, L = 3 * rec[k]["nv"], rec[k]["L"]
+                if k not in hist:
+                    hist[k] = tuple(torch.randn(s, dtype=torch.float64, device=dev) for s in
+                                    ((L - 1, n3), (L - 1, n3), (L - 1, 6), (n3,)))
+                    torch.cuda.synchronize()
+                hp, hu, ha, hb = hist[k]
+                fn = stepper.history_f32 if rec[k]["f32"] else stepper.history
+                fn(n3, L, outs["G"].data_ptr(), outs["H"].data_ptr(), outs["B"].data_ptr(), hp.data_ptr(),
+                   hu.data_ptr(), ha.data_ptr(), L - 1, hb.data_ptr())
+                obj["steps"] = steps + 1
+                if obj["steps"] >= SERVING_STEPS:
+                    ctx.release(jid)
+                    with lock:
+                        resident.remove(obj)
+                    del hist[k]
+                    released.append(k)
+
+    th = threading.Thread(target=step_loop, daemon=True)
+    th.start()
+    queue, running, nxt = [], {}, 0
+    t0 = time.perf_counter()
+    try:
+        while True:
+            now = time.perf_counter() - t0
+            while nxt < len(jobs) and arrivals[nxt] <= now:
+                queue.append(nxt)
+                nxt += 1
+            for k in list(queue):
+                V, F, p = jobs[k]
+                n3 = 3 * len(V)
+                if 2 * p.L * n3 * n3 * 8 > SERVING_BUDGET:  # the policy for over-budget double outputs
+                    p.storage = abi.STORE_F32
+                    rec[k]["f32"] = True
+                mt = torch.float32 if rec[k]["f32"] else torch.float64
+                out_bytes = 2 * p.L * n3 * n3 * (4 if rec[k]["f32"] else 8) + p.L * n3 * 6 * 8
+                if ctx.reserved_bytes() + out_bytes > SERVING_BUDGET:
+                    continue  # cannot be admitted before resident objects are released
+                try:
+                    outs = {"G": torch.empty((p.L, n3, n3), dtype=mt, device=dev),
+                            "H": torch.empty((p.L, n3, n3), dtype=mt, device=dev),
+                            "B": torch.empty((p.L, n3, 6), dtype=torch.float64, device=dev)}
+                except torch.cuda.OutOfMemoryError:
+                    break
+                torch.cuda.synchronize()
+                jid = ctx.submit(meshes[k], p, {n: t.data_ptr() for n, t in outs.items()}, resident=True)
+                if jid < 0:
+                    del outs
+                    continue  # not admitted now; a smaller queued job may be
+                running[jid] = (k, outs)
+                queue.remove(k)
+            for jid in [j for j in running if ctx.poll(j)]:
+                ctx.wait(jid)
+                k, outs = running.pop(jid)
+                rec[k]["done"] = time.perf_counter() - t0
+                with lock:
+                    resident.append({"k": k, "jid": jid, "outs": outs, "steps": 0})
+            now = time.perf_counter() - t0
+            if nxt >= len(jobs) and not queue and not running:
+                break
+            if now > arrivals[-1] + drain_s:
+                break
+            time.sleep(0.0005)
+        end = time.perf_counter() - t0
+    finally:
+        stop.set()
+        th.join()
+        torch.cuda.synchronize()
+        stepper.close()
+        ctx.close()
+    return rec, end
+
+
+def serving_main(args):
+    """The serving mix at `--serving` jobs/s (one rate) for `--duration` seconds of arrivals: p50 / p99 latency
+    from arrival to resident matrices (jobs still unfinished when the drain ends count with their waiting time so
+    far, a lower bound), aggregate G + H entries/s of the finished jobs over the run, admission counts."""
+    import torch
+    rng = np.random.default_rng(int(args.serving * 1000) + 1)
+    n = max(1, int(round(args.serving * args.duration)))
     arrivals = np.cumsum(rng.exponential(1.0 / args.serving, n)).tolist()
-    jobs = scene_jobs(n, seed=2)
-    ctx = abi.Context(0)
-    run_jobs(ctx, jobs[:2], [0.0, 0.0], torch)  # warm-up
-    lat, makespan = run_jobs(ctx, jobs, arrivals, torch)
-    ctx.close()
-    entries = sum(2.0 * (3 * len(V)) ** 2 * p.L for V, F, p in jobs)
-    print(json.dumps({"metric": "serving: Poisson job stream, G+H entries/s and job latency",
-                      "rate_jobs_per_s": args.serving, "jobs": n, "value": entries / makespan, "unit": "entries/s",
+    jobs = serving_jobs(n, rng, args.precision)
+    from surfaceonlydeformables_amd import abi
+    warm = scene_jobs(2, seed=5)  # small objects: loads the kernels and fills the context's pool
+    for _, _, p in warm:
+        p.precision = abi.MIXED if args.precision == "mixed" else abi.FP64
+    serve_stream(warm, [0.0, 0.0], torch, 60.0)
+    rec, end = serve_stream(jobs, arrivals, torch, drain_s=30.0)
+    lat = [(r["done"] if r["done"] is not None else end) - r["arrival"] for r in rec]
+    done = [r for r in rec if r["done"] is not None]
+    entries = sum(2.0 * (3 * r["nv"]) ** 2 * r["L"] for r in done)
+    print(json.dumps({"metric": "serving mix: G+H entries/s and p50/p99 job latency (arrival to resident matrices)",
+                      "rate_jobs_per_s": args.serving, "jobs": n, "finished": len(done),
+                      "icosphere_l5_jobs": sum(r["nv"] == 10242 for r in rec),
+                      "stored_f32": sum(r["f32"] for r in rec), "value": entries / end, "unit": "entries/s",
                       "latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3, "p99": float(np.percentile(lat, 99)) * 1e3},
-                      "makespan_s": makespan, "dtype": "f64", "data": "synthetic"}), flush=True)
+                      "run_s": end, "budget_GB": SERVING_BUDGET / 1e9, "steps_per_object": SERVING_STEPS,
+                      "precision": args.precision, "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+                      "data": "synthetic"}), flush=True)
 
 
 def time_stepping_main(args):
@@ -456,16 +627,50 @@ def main():
         phases = {k: last[k] for k in phase_keys}
     line = {"metric": METRIC, "value": value, "unit": "entries/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f64+f32",
+            "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32+f64",
             "data": "synthetic",
             "config": {"workload": desc, "nv": nv, "nf": len(F), "L": L, "precision": args.precision,
                        "entries_per_step": entries},
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
             "phases_ms_last_step": phases, "roofline": roof, "time_step_history": history}
+    if args.precision == "mixed":
+        line["mixed_vs_fp64"] = mixed_against_fp64(ctx, mesh, params, outs, ms_per_step, torch, dev)
+        if roof:
+            roof["note"] = "mixed: the timed G/H phase is the double H pair kernel plus the float G kernel"
+            roof["kernel"] = "k_colloc_gh_pair<false,true,false> + k_colloc_g32"
     if not args.no_cpu_baseline and not params.galerkin:
         line["cpu_baseline"] = cpu_baseline(V, F, params)
     ctx.close()
     print(json.dumps(line), flush=True)
+
+
+def mixed_against_fp64(ctx, mesh, params, outs, mixed_ms, torch, dev, steps=3):
+    """The same job in FP64 (a few untimed-for-value steps, timed for the ratio) and the relative Frobenius error
+    of every mixed output against it, per matrix over the whole stack and per weight n (norms on the device)."""
+    from surfaceonlydeformables_amd import abi
+    p64 = abi.make_params(params.c1, params.c2, params.rho, params.nu, params.dt, params.R, params.L, bdf=params.bdf,
+                          order=params.order, subdivision=params.subdivision, galerkin=params.galerkin,
+                          which=params.which)
+    ref = {k: torch.empty_like(t) for k, t in outs.items()}
+    ptrs = {k: t.data_ptr() for k, t in ref.items()}
+    torch.cuda.synchronize()
+    ctx.assemble_device(mesh, p64, ptrs)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.assemble_device(mesh, p64, ptrs)
+    fp64_ms = (time.perf_counter() - t0) * 1e3 / steps
+    torch.cuda.synchronize()
+    per_weight, stack = {}, {}
+    for k in outs:
+        a, b = outs[k].reshape(params.L, -1), ref[k].reshape(params.L, -1)
+        per_weight[k] = [float(torch.linalg.norm(a[n] - b[n]) / torch.linalg.norm(b[n])) for n in range(params.L)]
+        stack[k] = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+    del ref
+    torch.cuda.empty_cache()
+    return {"fp64_ms_per_step": fp64_ms, "speedup_vs_fp64": fp64_ms / mixed_ms, "rel_err_stack": stack,
+            "rel_err_per_weight": per_weight, "target": "<= 1e-6 relative error, >= 3x FP64",
+            "note": "far-field G, B_trans, B_angular in float (Kahan-compensated face sums); H, singular faces and "
+                    "the inverse transform in double"}
 
 
 def roofline(gh_ms, gh_evals, gh_launches, with_traffic):

@@ -42,7 +42,16 @@ enum {
 };
 
 enum { CQBEM_BDF1 = 1, CQBEM_BDF2 = 2 };
+/* CQBEM_MIXED (collocation): the reference's own precision policy -- far-field G, B_trans and B_angular
+ * integrated in float (G_collocation.cu:14-15, B_trans_collocation.cu:14, B_angular_collocation.cu:14) with
+ * Kahan-compensated face sums; H, the singular faces and the inverse transform in double.  Galerkin with
+ * CQBEM_MIXED returns CQBEM_ERR_UNSUPPORTED. */
 enum { CQBEM_FP64 = 0, CQBEM_MIXED = 1 };
+/* Element type of the G and H outputs: double, or float (weights rounded once from the double transform; device
+ * outputs only).  B_trans, B_angular and B are always double.  No counterpart in the reference: a storage policy for
+ * jobs whose double weights exceed the memory they may hold resident (an icosphere-L5 job at L = 16 needs 242 GB
+ * in double, 121 GB in float). */
+enum { CQBEM_STORE_F64 = 0, CQBEM_STORE_F32 = 1 };
 enum { CQBEM_G = 1, CQBEM_H = 2, CQBEM_B_TRANS = 4, CQBEM_B_ANGULAR = 8, CQBEM_ALL = 15 };
 
 typedef struct cqbem_context cqbem_context;
@@ -69,6 +78,7 @@ typedef struct {
     int32_t galerkin;     /* 0 collocation, 1 Galerkin */
     int32_t precision;    /* CQBEM_FP64 | CQBEM_MIXED */
     int32_t which;        /* bitmask of CQBEM_G | CQBEM_H | CQBEM_B_TRANS | CQBEM_B_ANGULAR */
+    int32_t storage;      /* CQBEM_STORE_F64 | CQBEM_STORE_F32: element type of the G and H outputs */
 } cqbem_params;
 
 /* Output buffers; any may be NULL.  on_device != 0: device pointers on the context's GPU;
@@ -102,6 +112,18 @@ int cqbem_assemble(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_param
 #define CQBEM_NOALLOC_SCRATCH_MIB 256
 int cqbem_assemble_noalloc(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params *params,
                            const cqbem_outputs *out);
+
+/* All objects of a scene in one call (device outputs only).  No counterpart in the reference, whose
+ * Simulator::init assembles one object group after another (simulator.cpp:101-125).  The G/H regular-face tiles
+ * of every collocation object -- (column patch, frequency, row) tiles of all objects, largest objects first --
+ * run in shared launches of the pair kernel, in waves whose Laplace-domain spectra fit the HBM budget; each
+ * wave (the shared launch, the float G kernel of CQBEM_MIXED objects, the singular faces and the transforms of
+ * every object slab) is captured in one HIP graph and launched once, and B, C and the B post-processing of all
+ * objects follow in one more graph.  Galerkin objects run after them, one by one, as cqbem_assemble runs them.
+ * Every object's outputs are bit-identical to its own cqbem_assemble.  cqbem_last_timings: [1] the waves' wall
+ * time, [2] the B graph, [6] the waves, [7] the far-field evaluations. */
+int cqbem_assemble_scene(cqbem_context *ctx, int32_t n, const cqbem_mesh *meshes, const cqbem_params *params,
+                         const cqbem_outputs *outs);
 
 /* Device bytes cqbem_assemble needs for a job (outputs on device included when outputs_on_device != 0). */
 int cqbem_job_footprint(const cqbem_mesh *mesh, const cqbem_params *params, int outputs_on_device, uint64_t *bytes);
@@ -144,8 +166,19 @@ int cqbem_submit(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params 
 /* *done = 1 once the job finished (its scratch is then released); status of the job in the return value. */
 int cqbem_poll(cqbem_context *ctx, uint64_t job_id, int32_t *done);
 int cqbem_wait(cqbem_context *ctx, uint64_t job_id);
-/* Bytes currently reserved by in-flight jobs. */
+/* Bytes currently reserved: the scratch of in-flight jobs and the outputs of resident jobs. */
 int cqbem_reserved_bytes(const cqbem_context *ctx, uint64_t *bytes);
+/* cqbem_submit for a job whose outputs stay resident after it finishes (the matrices a simulated object steps on):
+ * admission counts the job's output bytes (G and H at their storage size, B_trans, B_angular and B) next to its
+ * scratch, and those bytes stay reserved -- so later jobs are admitted against them -- until cqbem_release.
+ * cqbem_poll and cqbem_wait work as for cqbem_submit; the job's record is kept until cqbem_release.  No
+ * counterpart in the reference (its precompute runs once per object group at start-up). */
+int cqbem_submit_resident(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params *params,
+                          const cqbem_outputs *out, uint64_t *job_id);
+/* Ends the residency of a finished job submitted with cqbem_submit_resident (the caller may then free its outputs):
+ * its output bytes leave the reservation.  CQBEM_ERR_BUSY while the job still runs, CQBEM_ERR_NOTFOUND for an
+ * unknown id or a job that was not resident. */
+int cqbem_release(cqbem_context *ctx, uint64_t job_id);
 
 /* ---- time stepping on the resident matrices ----
  * History term of one CQ time step (reference: ElastodynamicsCQBEMObject::estimate_next_state,
@@ -157,6 +190,10 @@ int cqbem_reserved_bytes(const cqbem_context *ctx, uint64_t *bytes);
  * (its term is skipped).  b (n3) is complete when the call returns. */
 int cqbem_history(cqbem_context *ctx, int64_t n3, int32_t L, const double *G, const double *H, const double *B,
                   const double *p_hist, const double *u_hist, const double *acc_hist, int32_t nhist, double *b);
+/* The same history term on G and H stored in float (CQBEM_STORE_F32), products and sums in double.  n3 must be
+ * even, G and H 8-byte and the histories 16-byte aligned.  Same reference as cqbem_history. */
+int cqbem_history_f32(cqbem_context *ctx, int64_t n3, int32_t L, const float *G, const float *H, const double *B,
+                      const double *p_hist, const double *u_hist, const double *acc_hist, int32_t nhist, double *b);
 /* y = A x for a row-major n x n device matrix A and device vectors x, y (the u_p * p product of a
  * constrained step, elastodynamics_cqbem_object.cpp:136).  y is complete when the call returns. */
 int cqbem_matvec(cqbem_context *ctx, int64_t n, const double *A, const double *x, double *y);

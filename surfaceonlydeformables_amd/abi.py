@@ -17,13 +17,15 @@ CQBEM_OK = 0
 ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_BUSY, ERR_NOTFOUND = -1, -2, -3, -4, -5, -6
 BDF1, BDF2 = 1, 2
 FP64, MIXED = 0, 1
+STORE_F64, STORE_F32 = 0, 1
 G, H, B_TRANS, B_ANGULAR, ALL = 1, 2, 4, 8, 15
 
 EXPORTED = [
     "cqbem_create", "cqbem_destroy", "cqbem_last_error", "cqbem_assemble", "cqbem_job_footprint",
     "cqbem_laplace", "cqbem_frequencies", "cqbem_last_timings", "cqbem_submit", "cqbem_poll",
     "cqbem_wait", "cqbem_reserved_bytes", "cqbem_history", "cqbem_matvec", "cqbem_split_traction_discontinuity",
-    "cqbem_wavelet_compress", "cqbem_csr_matvec", "cqbem_assemble_noalloc",
+    "cqbem_wavelet_compress", "cqbem_csr_matvec", "cqbem_assemble_noalloc", "cqbem_submit_resident", "cqbem_release",
+    "cqbem_history_f32", "cqbem_assemble_scene",
 ]
 
 
@@ -36,7 +38,7 @@ class cqbem_params(C.Structure):
     _fields_ = [("c1", C.c_double), ("c2", C.c_double), ("rho", C.c_double), ("nu", C.c_double),
                 ("dt", C.c_double), ("R", C.c_double), ("L", C.c_int32), ("bdf", C.c_int32),
                 ("order", C.c_int32), ("subdivision", C.c_int32), ("galerkin", C.c_int32),
-                ("precision", C.c_int32), ("which", C.c_int32)]
+                ("precision", C.c_int32), ("which", C.c_int32), ("storage", C.c_int32)]
 
 
 class cqbem_outputs(C.Structure):
@@ -73,12 +75,16 @@ def lib():
         L.cqbem_last_error.restype = C.c_char_p
         L.cqbem_assemble.argtypes = [C.c_void_p, P(cqbem_mesh), P(cqbem_params), P(cqbem_outputs)]
         L.cqbem_assemble_noalloc.argtypes = [C.c_void_p, P(cqbem_mesh), P(cqbem_params), P(cqbem_outputs)]
+        L.cqbem_assemble_scene.argtypes = [C.c_void_p, C.c_int32, P(cqbem_mesh), P(cqbem_params), P(cqbem_outputs)]
         L.cqbem_job_footprint.argtypes = [P(cqbem_mesh), P(cqbem_params), C.c_int, P(C.c_uint64)]
         L.cqbem_laplace.argtypes = [C.c_void_p, P(cqbem_mesh), P(cqbem_params), C.c_int32, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p]
         L.cqbem_frequencies.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_int32, P(C.c_double)]
         L.cqbem_last_timings.argtypes = [C.c_void_p, P(C.c_double), C.c_int32]
         L.cqbem_submit.argtypes = [C.c_void_p, P(cqbem_mesh), P(cqbem_params), P(cqbem_outputs), P(C.c_uint64)]
+        L.cqbem_submit_resident.argtypes = [C.c_void_p, P(cqbem_mesh), P(cqbem_params), P(cqbem_outputs), P(C.c_uint64)]
+        L.cqbem_release.argtypes = [C.c_void_p, C.c_uint64]
+        L.cqbem_history_f32.argtypes = [C.c_void_p, C.c_int64, C.c_int32] + [C.c_void_p] * 6 + [C.c_int32, C.c_void_p]
         L.cqbem_poll.argtypes = [C.c_void_p, C.c_uint64, P(C.c_int32)]
         L.cqbem_wait.argtypes = [C.c_void_p, C.c_uint64]
         L.cqbem_reserved_bytes.argtypes = [C.c_void_p, P(C.c_uint64)]
@@ -108,8 +114,8 @@ def make_mesh(V, F, vmap=None):
 
 
 def make_params(c1, c2, rho, nu, dt, R, L, bdf=BDF2, order=5, subdivision=2, galerkin=False, precision=FP64,
-                which=ALL):
-    return cqbem_params(c1, c2, rho, nu, dt, R, L, bdf, order, subdivision, int(galerkin), precision, which)
+                which=ALL, storage=STORE_F64):
+    return cqbem_params(c1, c2, rho, nu, dt, R, L, bdf, order, subdivision, int(galerkin), precision, which, storage)
 
 
 def split_traction_discontinuity(V, F, threshold_deg=20.0):
@@ -202,6 +208,19 @@ class Context:
         out.on_device = 1
         self._check(lib().cqbem_assemble(self._h, C.byref(mesh), C.byref(params), C.byref(out)))
 
+    def assemble_scene(self, meshes, params, ptrs):
+        """All objects at once (cqbem_assemble_scene): meshes and params are lists, ptrs a list of {name: device
+        pointer} dicts."""
+        n = len(meshes)
+        M_ = (cqbem_mesh * n)(*meshes)
+        P_ = (cqbem_params * n)(*params)
+        O_ = (cqbem_outputs * n)()
+        for k, d in enumerate(ptrs):
+            for name, p in d.items():
+                setattr(O_[k], name, p)
+            O_[k].on_device = 1
+        self._check(lib().cqbem_assemble_scene(self._h, n, M_, P_, O_))
+
     def assemble_device_noalloc(self, mesh: cqbem_mesh, params: cqbem_params, ptrs: dict):
         """assemble_device through cqbem_assemble_noalloc: no device allocation, scratch from the library's fixed
         per-device array (CQBEM_NOALLOC_SCRATCH_MIB)."""
@@ -222,17 +241,23 @@ class Context:
         return {"U": U, "P": P, "Bt": Bt, "Ba": Ba}
 
     # serving
-    def submit(self, mesh, params, ptrs: dict) -> int:
+    def submit(self, mesh, params, ptrs: dict, resident: bool = False) -> int:
+        """Enqueues a job (-1 when it is not admitted now: retry after a running job finishes).  resident: its
+        outputs stay reserved after it finishes, until release()."""
         out = cqbem_outputs()
         for k, p in ptrs.items():
             setattr(out, k, p)
         out.on_device = 1
         jid = C.c_uint64()
-        rc = lib().cqbem_submit(self._h, C.byref(mesh), C.byref(params), C.byref(out), C.byref(jid))
+        fn = lib().cqbem_submit_resident if resident else lib().cqbem_submit
+        rc = fn(self._h, C.byref(mesh), C.byref(params), C.byref(out), C.byref(jid))
         if rc == ERR_BUSY:
             return -1
         self._check(rc)
         return jid.value
+
+    def release(self, job_id: int):
+        self._check(lib().cqbem_release(self._h, job_id))
 
     def poll(self, job_id: int) -> bool:
         done = C.c_int32()
@@ -251,6 +276,11 @@ class Context:
         """History term of one CQ step on device buffers (device pointers as ints; G, H, B may be 0)."""
         self._check(lib().cqbem_history(self._h, n3, L, G or None, H or None, B or None, p_hist or None,
                                         u_hist or None, acc_hist or None, nhist, b))
+
+    def history_f32(self, n3: int, L: int, G, H, B, p_hist, u_hist, acc_hist, nhist: int, b):
+        """History term on G and H stored in float (CQBEM_STORE_F32)."""
+        self._check(lib().cqbem_history_f32(self._h, n3, L, G or None, H or None, B or None, p_hist or None,
+                                            u_hist or None, acc_hist or None, nhist, b))
 
     def matvec(self, n: int, A, x, y):
         """y = A x on device buffers (row-major n x n A; device pointers as ints)."""

@@ -28,6 +28,7 @@
 #include "cqbem_wavelet.hpp"
 #include "cqbem_pool.hpp"
 #include "cqbem_kernels.hpp"
+#include "cqbem_mixed.hpp"
 #include "cqbem_rules.hpp"
 
 using namespace cqbem;
@@ -103,7 +104,9 @@ void check_inputs(const cqbem_mesh *mesh, const cqbem_params *p) {
     if (p->order < 1 || p->order > 5) throw Err{CQBEM_ERR_ARG, "order must be in [1, 5]"};
     if (p->subdivision < 1 || p->subdivision > 16) throw Err{CQBEM_ERR_ARG, "subdivision must be in [1, 16]"};
     if (p->precision != CQBEM_FP64 && p->precision != CQBEM_MIXED) throw Err{CQBEM_ERR_ARG, "unknown precision"};
-    if (p->precision == CQBEM_MIXED) throw Err{CQBEM_ERR_UNSUPPORTED, "CQBEM_MIXED is not implemented"};
+    if (p->precision == CQBEM_MIXED && p->galerkin)
+        throw Err{CQBEM_ERR_UNSUPPORTED, "CQBEM_MIXED is implemented for collocation only"};
+    if (p->storage != CQBEM_STORE_F64 && p->storage != CQBEM_STORE_F32) throw Err{CQBEM_ERR_ARG, "unknown storage"};
     if ((p->which & ~CQBEM_ALL) != 0) throw Err{CQBEM_ERR_ARG, "unknown matrix bits in `which`"};
     // the device sincos reduces |Im(s) r / c| <= max|s_l| diam / min(c1, c2) exactly below kTrigArgMax
     double lo[3] = {mesh->V[0], mesh->V[1], mesh->V[2]}, hi[3] = {lo[0], lo[1], lo[2]};
@@ -132,7 +135,9 @@ struct JobRecord {
     cqbem_mesh mesh{};
     cqbem_params params{};
     cqbem_outputs out{};
-    uint64_t bytes = 0;
+    uint64_t bytes = 0;           // scratch reservation, released when the job finishes
+    uint64_t resident_bytes = 0;  // outputs reservation of a resident job, released by cqbem_release
+    bool resident = false;
     int slot = -1;
     std::atomic<int> done{0};
     int status = 0;
@@ -495,9 +500,12 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
     const bool wantB = wantBt || wantBa || out->B;
     tm[5] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
 
-    // outputs: caller device buffers, or staged device buffers copied back at the end
+    // outputs: caller device buffers, or staged device buffers copied back at the end; with CQBEM_STORE_F32 the
+    // caller's G and H buffers hold float (the transform rounds each weight once)
     DBuf sG, sH, sBt, sBa, sB6;
     double *G = nullptr, *H = nullptr;
+    const bool f32 = p->storage == CQBEM_STORE_F32;
+    if (f32 && !out->on_device) throw Err{CQBEM_ERR_ARG, "CQBEM_STORE_F32 needs device outputs"};
     if (wantG) {
         if (out->on_device) G = out->G;
         else sG.alloc(matrix_bytes(nv, L)), G = sG.as<double>();
@@ -506,6 +514,13 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
         if (out->on_device) H = out->H;
         else sH.alloc(matrix_bytes(nv, L)), H = sH.as<double>();
     }
+    // weights n of rows from `row` on: out[n][row..] = transform of the spectra
+    auto transform_into = [&](double *dst, const cd *spec, long long E, long long spec_ls, long long row_off) {
+        if (f32)
+            return launch_transform(spec, E, spec_ls, Lf, d.tab.as<double2>(), L, reinterpret_cast<float *>(dst) + row_off,
+                                    n3 * n3, ctx->num_cus, st);
+        return launch_transform(spec, E, spec_ls, Lf, d.tab.as<double2>(), L, dst + row_off, n3 * n3, ctx->num_cus, st);
+    };
     DBuf specB, specBa;
     double *Wbt = nullptr, *Wba = nullptr, *W6 = nullptr;
     if (wantB) {
@@ -526,6 +541,12 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
     const Freq *freqs = d.freqs.as<Freq>();
     const double2 *tab = d.tab.as<double2>();
     const int npts = P.npts, nline = (int)P.line.size();
+    // mixed precision: far-field G and B in float (the reference's typedefs), H and the rest in double
+    const bool mixed = p->precision == CQBEM_MIXED;
+    DBuf dfreq32;
+    if (mixed) upload(dfreq32, freqs_f32(P.freqs), st);
+    const FreqF *freqs32 = dfreq32.as<FreqF>();
+    const MatF mat32 = mat_f32(P.mat);
 
     if (!p->galerkin) {
         const int nmat = (wantG ? 1 : 0) + (wantH ? 1 : 0);
@@ -539,7 +560,14 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
                 const int nr = std::min(rows, nv - r0);
                 {
                     Timer t(st, &tm[1]);
-                    HIPCHK(launch_gh(d, P, freqs, 0, Lf, r0, nr, specG.as<cd>(), specH.as<cd>(), n3, lstride, st));
+                    if (!mixed) {
+                        HIPCHK(launch_gh(d, P, freqs, 0, Lf, r0, nr, specG.as<cd>(), specH.as<cd>(), n3, lstride, st));
+                    } else {
+                        if (wantH) HIPCHK(launch_gh(d, P, freqs, 0, Lf, r0, nr, nullptr, specH.as<cd>(), n3, lstride, st));
+                        if (wantG)
+                            HIPCHK(launch_colloc_g32(g, quad, npts, freqs32, 0, Lf, mat32, r0, nr, specG.as<cd>(), n3, lstride,
+                                                     st));
+                    }
                 }
                 tm[6] += 1;
                 for (int i = r0; i < r0 + nr; i++) tm[7] += (double)(P.nf - (P.sg_ptr[i + 1] - P.sg_ptr[i])) * npts * Lf;
@@ -555,17 +583,18 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
                 {
                     Timer t(st, &tm[4]);
                     const long long E = 3LL * nr * n3;
-                    if (wantG)
-                        HIPCHK(launch_transform(specG.as<cd>(), E, lstride, Lf, tab, L, G + 3LL * r0 * n3, n3 * n3, ctx->num_cus, st));
-                    if (wantH)
-                        HIPCHK(launch_transform(specH.as<cd>(), E, lstride, Lf, tab, L, H + 3LL * r0 * n3, n3 * n3, ctx->num_cus, st));
+                    if (wantG) HIPCHK(transform_into(G, specG.as<cd>(), E, lstride, 3LL * r0 * n3));
+                    if (wantH) HIPCHK(transform_into(H, specH.as<cd>(), E, lstride, 3LL * r0 * n3));
                 }
             }
         }
         if (wantB) {
             {
                 Timer t(st, &tm[2]);
-                HIPCHK(launch_colloc_b(g, quad, npts, freqs, 0, Lf, P.mat, 0, nv, specB.as<cd>(), specBa.as<cd>(), st));
+                if (mixed)
+                    HIPCHK(launch_colloc_b32(g, quad, npts, freqs32, 0, Lf, mat32, 0, nv, specB.as<cd>(), specBa.as<cd>(), st));
+                else
+                    HIPCHK(launch_colloc_b(g, quad, npts, freqs, 0, Lf, P.mat, 0, nv, specB.as<cd>(), specBa.as<cd>(), st));
             }
             {
                 Timer t(st, &tm[3]);
@@ -578,7 +607,10 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
                         ? c_matrix_collocation_split(mesh->V, nv, mesh->F, P.nf, P.N.data(), mesh->vertex_map_inverse, p->nu)
                         : c_matrix_collocation(mesh->V, nv, mesh->F, P.nf, P.N.data(), p->nu);
             upload(dC, Chost, st);
-            HIPCHK(launch_add_c_colloc(H, dC.as<double>(), nv, st));
+            if (f32)
+                HIPCHK(launch_add_c_colloc(reinterpret_cast<float *>(H), dC.as<double>(), nv, st));
+            else
+                HIPCHK(launch_add_c_colloc(H, dC.as<double>(), nv, st));
         }
     } else {
         GalerkinJob gj{g, quad, npts, line, nline, freqs, Lf, P.mat, ctx->num_cus, d.pair_pts.as<double4>(), (int)P.quad.size(),
@@ -594,8 +626,8 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
             run_galerkin(gj, mesh->F, mesh->vertex_map_inverse, specG.as<cd>(), specH.as<cd>(),
                          wantB ? specB.as<cd>() : nullptr, wantB ? specBa.as<cd>() : nullptr, tm, st);
             Timer t(st, &tm[4]);
-            if (wantG) HIPCHK(launch_transform(specG.as<cd>(), n3 * n3, n3 * n3, Lf, tab, L, G, n3 * n3, ctx->num_cus, st));
-            if (wantH) HIPCHK(launch_transform(specH.as<cd>(), n3 * n3, n3 * n3, Lf, tab, L, H, n3 * n3, ctx->num_cus, st));
+            if (wantG) HIPCHK(transform_into(G, specG.as<cd>(), n3 * n3, n3 * n3, 0));
+            if (wantH) HIPCHK(transform_into(H, specH.as<cd>(), n3 * n3, n3 * n3, 0));
         }
         if (wantH) {
             // 1/2 P1 mass matrix per component (elastics_bem_matrix_collection.cpp:447-467), summed per entry on the host
@@ -611,7 +643,11 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
             std::vector<double> val;
             for (auto &kv : acc) idx.push_back(kv.first), val.push_back(kv.second);
             upload(dCidx, idx, st), upload(dCval, val, st);
-            HIPCHK(launch_add_triplets(H, dCidx.as<long long>(), dCval.as<double>(), (long long)idx.size(), st));
+            if (f32)
+                HIPCHK(launch_add_triplets(reinterpret_cast<float *>(H), dCidx.as<long long>(), dCval.as<double>(),
+                                           (long long)idx.size(), st));
+            else
+                HIPCHK(launch_add_triplets(H, dCidx.as<long long>(), dCval.as<double>(), (long long)idx.size(), st));
         }
     }
     if (wantB) {
@@ -631,6 +667,255 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
         tm[5] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     HIPCHK(hipStreamSynchronize(st));
+}
+
+// Stream capture of a launch sequence into a HIP graph that is launched once and waited for.
+struct GraphRun {
+    hipStream_t st;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    explicit GraphRun(hipStream_t s) : st(s) { HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed)); }
+    void launch() {
+        HIPCHK(hipStreamEndCapture(st, &graph));
+        HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        HIPCHK(hipGraphLaunch(exec, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    ~GraphRun() {
+        if (!graph) {  // capture abandoned by an exception: end it so the stream is usable again
+            hipGraph_t g = nullptr;
+            if (hipStreamEndCapture(st, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+        }
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+// A scene: every object's G/H regular-face tiles go through shared launches of the pair kernel (k_scene_gh_tiles:
+// all objects' tiles in one grid, largest pieces first), in waves whose spectra fit the scratch budget; each wave's
+// launches -- the shared pair kernel, the float G kernel of mixed objects, the singular faces and the transforms of
+// every piece -- are captured in one HIP graph and launched once.  B, C and the B post-processing of all objects
+// follow in a last graph.  Galerkin objects run one after another through run_job.
+void run_scene(cqbem_context *ctx, hipStream_t st, int n, const cqbem_mesh *meshes, const cqbem_params *params,
+               const cqbem_outputs *outs, double *tm) {
+    JobArena arena(&ctx->pool, st);
+    ArenaScope arena_scope(&arena);
+    struct Obj {
+        int k;
+        Prepared P;
+        DevMesh d;
+        DBuf specB, specBa, dC, dfreq32, sBt, sBa;
+        double *Wbt = nullptr, *Wba = nullptr;  // B_trans / B_angular weights: the caller's, or staged for B
+        bool wantG, wantH, wantB, mixed, f32;
+        MatF mat32;
+    };
+    std::vector<std::unique_ptr<Obj>> objs;
+    std::vector<int> galerkin;
+    size_t fixed = 0;
+    const auto t_host0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < n; k++) {
+        check_inputs(&meshes[k], &params[k]);
+        if (!outs[k].on_device) throw Err{CQBEM_ERR_ARG, "assemble_scene: outputs must be device buffers"};
+        if (params[k].galerkin) {
+            galerkin.push_back(k);
+            continue;
+        }
+        auto o = std::make_unique<Obj>();
+        o->k = k;
+        const cqbem_params *p = &params[k];
+        const cqbem_outputs *out = &outs[k];
+        o->P = prepare(&meshes[k], p);
+        upload_mesh(o->d, &meshes[k], o->P, st);
+        o->wantG = (p->which & CQBEM_G) && out->G, o->wantH = (p->which & CQBEM_H) && out->H;
+        o->wantB = ((p->which & CQBEM_B_TRANS) && out->B_trans) || ((p->which & CQBEM_B_ANGULAR) && out->B_angular) || out->B;
+        o->mixed = p->precision == CQBEM_MIXED, o->f32 = p->storage == CQBEM_STORE_F32;
+        o->mat32 = mat_f32(o->P.mat);
+        if (o->mixed) upload(o->dfreq32, freqs_f32(o->P.freqs), st);
+        const long long n3 = 3LL * o->P.nv;
+        if (o->wantB) {
+            o->specB.alloc((size_t)o->P.Lf * n3 * 3 * sizeof(cd));
+            o->specBa.alloc((size_t)o->P.Lf * n3 * 9 * sizeof(cd));
+            if (out->B_trans) o->Wbt = out->B_trans;
+            else o->sBt.alloc((size_t)o->P.L * n3 * 3 * sizeof(double)), o->Wbt = o->sBt.as<double>();
+            if (out->B_angular) o->Wba = out->B_angular;
+            else o->sBa.alloc((size_t)o->P.L * n3 * 9 * sizeof(double)), o->Wba = o->sBa.as<double>();
+        }
+        if (o->wantH) {
+            const auto C = meshes[k].vertex_map_inverse
+                               ? c_matrix_collocation_split(meshes[k].V, o->P.nv, meshes[k].F, o->P.nf, o->P.N.data(),
+                                                            meshes[k].vertex_map_inverse, p->nu)
+                               : c_matrix_collocation(meshes[k].V, o->P.nv, meshes[k].F, o->P.nf, o->P.N.data(), p->nu);
+            upload(o->dC, C, st);
+        }
+        fixed += footprint(&meshes[k], p, 1, 0);
+        objs.push_back(std::move(o));
+    }
+    tm[5] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
+    // the spectra budget: what the HBM budget and the free memory leave, at most 48 GiB
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    const uint64_t avail = std::min<uint64_t>(ctx->budget, (uint64_t)fr + ctx->pool.cached_bytes());
+    if (fixed >= avail) throw Err{CQBEM_ERR_NOMEM, "assemble_scene: the objects do not fit the HBM budget"};
+    const uint64_t W = std::min<uint64_t>(avail - fixed, (uint64_t)48 << 30);
+    // pieces: row slabs of each object, sized to the budget
+    struct Piece {
+        Obj *o;
+        int row0, nrows;
+        uint64_t bytes;
+        long long tiles;
+    };
+    std::vector<Piece> pieces;
+    for (auto &op : objs) {
+        Obj *o = op.get();
+        const int nmat = (o->wantG ? 1 : 0) + (o->wantH ? 1 : 0);
+        if (!nmat) continue;
+        const uint64_t per_row = (uint64_t)nmat * o->P.Lf * 9 * (uint64_t)o->P.nv * sizeof(cd);
+        const int rows = (int)std::min<uint64_t>(std::min<uint64_t>(std::max<uint64_t>(W / per_row, 1), o->P.nv), 65535);
+        for (int r0 = 0; r0 < o->P.nv; r0 += rows) {
+            const int nr = std::min(rows, o->P.nv - r0);
+            pieces.push_back(Piece{o, r0, nr, per_row * nr, (long long)o->P.pair_npatches * o->P.Lf * nr});
+        }
+    }
+    // waves: first fit of the pieces, largest first, into the budget
+    std::sort(pieces.begin(), pieces.end(), [](const Piece &a, const Piece &b) { return a.bytes > b.bytes; });
+    std::vector<std::vector<Piece>> waves;
+    std::vector<uint64_t> wave_bytes;
+    for (const Piece &pc : pieces) {
+        size_t w = 0;
+        while (w < waves.size() && wave_bytes[w] + pc.bytes > W) ++w;
+        if (w == waves.size()) waves.emplace_back(), wave_bytes.push_back(0);
+        waves[w].push_back(pc), wave_bytes[w] += pc.bytes;
+    }
+    DBuf spec, dpieces, counter;
+    if (!waves.empty()) {
+        spec.alloc(*std::max_element(wave_bytes.begin(), wave_bytes.end()));
+        counter.alloc(sizeof(unsigned));
+    }
+    for (auto &wave : waves) {
+        // per piece: its spectra in the wave's region, its tiles in one of the two shared launches
+        std::vector<ScenePiece> gh, honly, gonly;
+        struct Slab {
+            Piece pc;
+            cd *sG, *sH;
+            long long lstride;
+        };
+        std::vector<Slab> slabs;
+        size_t off = 0;
+        for (const Piece &pc : wave) {
+            Obj *o = pc.o;
+            const long long n3 = 3LL * o->P.nv, lstride = 3LL * pc.nrows * n3;
+            cd *sG = nullptr, *sH = nullptr;
+            if (o->wantG) sG = reinterpret_cast<cd *>(spec.as<char>() + off), off += (size_t)o->P.Lf * lstride * sizeof(cd);
+            if (o->wantH) sH = reinterpret_cast<cd *>(spec.as<char>() + off), off += (size_t)o->P.Lf * lstride * sizeof(cd);
+            slabs.push_back(Slab{pc, sG, sH, lstride});
+            // mixed objects: H through the shared launch, G through the float kernel
+            const bool g_in_pair = o->wantG && !o->mixed;
+            if (!g_in_pair && !o->wantH) continue;
+            ScenePiece sp{o->d.g, o->d.pair_pts.as<double4>(), (int)o->P.quad.size() / 2, o->d.freqs.as<Freq>(), o->P.mat,
+                          pc.row0, pc.nrows, o->P.Lf, o->P.pair_npatches, o->d.pair_patch_pass.as<int>(),
+                          o->d.pair_pass_info.as<int2>(), o->d.pair_pass_faces.as<int4>(), o->d.pair_patch_verts.as<int>(),
+                          g_in_pair ? sG : nullptr, o->wantH ? sH : nullptr, n3, lstride, 0u};
+            (g_in_pair ? (o->wantH ? gh : gonly) : honly).push_back(sp);
+        }
+        auto number = [](std::vector<ScenePiece> &v) {
+            long long base = 0;
+            for (auto &sp : v) {
+                sp.base = (unsigned)base;
+                base += (long long)sp.npatches * sp.nl * sp.nrows;
+            }
+            if (base >= (1LL << 31)) throw Err{CQBEM_ERR_UNSUPPORTED, "assemble_scene: too many tiles in one wave"};
+            return base;
+        };
+        const long long ngh = number(gh), nho = number(honly), ngo = number(gonly);
+        std::vector<ScenePiece> all(gh);
+        all.insert(all.end(), honly.begin(), honly.end());
+        all.insert(all.end(), gonly.begin(), gonly.end());
+        upload(dpieces, all, st);
+        HIPCHK(hipStreamSynchronize(st));  // the piece table is on the device before the capture starts
+        const ScenePiece *dgh = dpieces.as<ScenePiece>(), *dho = dgh + gh.size(), *dgo = dho + honly.size();
+        const auto t0 = std::chrono::steady_clock::now();
+        {
+            // the shared launches run one after another on the stream (they share the tile counter)
+            GraphRun gr(st);
+            if (ngh) HIPCHK(launch_scene_gh_pair(dgh, (int)gh.size(), ngh, counter.as<unsigned>(), true, true, ctx->num_cus, st));
+            if (nho) HIPCHK(launch_scene_gh_pair(dho, (int)honly.size(), nho, counter.as<unsigned>(), false, true, ctx->num_cus, st));
+            if (ngo) HIPCHK(launch_scene_gh_pair(dgo, (int)gonly.size(), ngo, counter.as<unsigned>(), true, false, ctx->num_cus, st));
+            for (const Slab &sb : slabs) {
+                Obj *o = sb.pc.o;
+                const Geom &g = o->d.g;
+                const long long n3 = 3LL * o->P.nv;
+                const Freq *freqs = o->d.freqs.as<Freq>();
+                const double2 *line = o->d.line.as<double2>();
+                const int nline = (int)o->P.line.size(), Lf = o->P.Lf, L = o->P.L;
+                if (o->wantG && o->mixed)
+                    HIPCHK(launch_colloc_g32(g, o->d.quad.as<double4>(), o->P.npts, o->dfreq32.as<FreqF>(), 0, Lf, o->mat32,
+                                             sb.pc.row0, sb.pc.nrows, sb.sG, n3, sb.lstride, st));
+                if (o->wantG)
+                    HIPCHK(launch_colloc_singular(0, g, line, nline, freqs, 0, Lf, o->P.mat, sb.pc.row0, sb.pc.nrows, sb.sG, n3,
+                                                  sb.lstride, nullptr, nullptr, st));
+                if (o->wantH)
+                    HIPCHK(launch_colloc_singular(2, g, line, nline, freqs, 0, Lf, o->P.mat, sb.pc.row0, sb.pc.nrows, sb.sH, n3,
+                                                  sb.lstride, nullptr, nullptr, st));
+                const long long E = 3LL * sb.pc.nrows * n3, row_off = 3LL * sb.pc.row0 * n3;
+                const cqbem_outputs &out = outs[o->k];
+                const double2 *tab = o->d.tab.as<double2>();
+                for (int m = 0; m < 2; m++) {
+                    const cd *src = m == 0 ? sb.sG : sb.sH;
+                    double *dst = m == 0 ? out.G : out.H;
+                    if (!src || !(m == 0 ? o->wantG : o->wantH)) continue;
+                    if (o->f32)
+                        HIPCHK(launch_transform(src, E, sb.lstride, Lf, tab, L, reinterpret_cast<float *>(dst) + row_off, n3 * n3,
+                                                ctx->num_cus, st));
+                    else
+                        HIPCHK(launch_transform(src, E, sb.lstride, Lf, tab, L, dst + row_off, n3 * n3, ctx->num_cus, st));
+                }
+            }
+            gr.launch();
+        }
+        tm[1] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        tm[6] += 1;
+        for (const Piece &pc : wave)
+            for (int i = pc.row0; i < pc.row0 + pc.nrows; i++)
+                tm[7] += (double)(pc.o->P.nf - (pc.o->P.sg_ptr[i + 1] - pc.o->P.sg_ptr[i])) * pc.o->P.npts * pc.o->P.Lf;
+    }
+    // B, the jump term and the B post-processing of every object, in one graph
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        GraphRun gr(st);
+        for (auto &op : objs) {
+            Obj *o = op.get();
+            const cqbem_outputs &out = outs[o->k];
+            const Geom &g = o->d.g;
+            const int nv = o->P.nv, Lf = o->P.Lf, L = o->P.L;
+            const long long n3 = 3LL * nv;
+            const Freq *freqs = o->d.freqs.as<Freq>();
+            if (o->wantH) {
+                if (o->f32)
+                    HIPCHK(launch_add_c_colloc(reinterpret_cast<float *>(out.H), o->dC.as<double>(), nv, st));
+                else
+                    HIPCHK(launch_add_c_colloc(out.H, o->dC.as<double>(), nv, st));
+            }
+            if (!o->wantB) continue;
+            if (o->mixed)
+                HIPCHK(launch_colloc_b32(g, o->d.quad.as<double4>(), o->P.npts, o->dfreq32.as<FreqF>(), 0, Lf, o->mat32, 0, nv,
+                                         o->specB.as<cd>(), o->specBa.as<cd>(), st));
+            else
+                HIPCHK(launch_colloc_b(g, o->d.quad.as<double4>(), o->P.npts, freqs, 0, Lf, o->P.mat, 0, nv, o->specB.as<cd>(),
+                                       o->specBa.as<cd>(), st));
+            HIPCHK(launch_colloc_singular(1, g, o->d.line.as<double2>(), (int)o->P.line.size(), freqs, 0, Lf, o->P.mat, 0, nv,
+                                          nullptr, n3, 0, o->specB.as<cd>(), o->specBa.as<cd>(), st));
+            const double2 *tab = o->d.tab.as<double2>();
+            HIPCHK(launch_transform(o->specB.as<cd>(), n3 * 3, n3 * 3, Lf, tab, L, o->Wbt, n3 * 3, ctx->num_cus, st));
+            HIPCHK(launch_transform(o->specBa.as<cd>(), n3 * 9, n3 * 9, Lf, tab, L, o->Wba, n3 * 9, ctx->num_cus, st));
+            HIPCHK(launch_b_post(o->Wbt, o->Wba, out.B, g.V, nv, L, st));
+        }
+        gr.launch();
+        tm[2] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    for (int k : galerkin) {
+        const size_t slab = (size_t)std::min<uint64_t>(W, (uint64_t)48 << 30);
+        run_job(ctx, st, &meshes[k], &params[k], &outs[k], slab, tm, true);
+    }
 }
 
 int fail(cqbem_context *ctx, const Err &e) {
@@ -710,6 +995,31 @@ int cqbem_history(cqbem_context *ctx, int64_t n3, int32_t L, const double *G, co
             HIPCHK(hipMemsetAsync(b, 0, sizeof(double) * n3, ctx->stream));
         } else {
             HIPCHK(launch_history(G, H, B, p_hist, u_hist, acc_hist, n3, m, nhist, b, ctx->stream));
+        }
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    } catch (const Err &e) {
+        ctx->last_error = e.msg;
+        return e.code;
+    }
+    return CQBEM_OK;
+}
+
+int cqbem_history_f32(cqbem_context *ctx, int64_t n3, int32_t L, const float *G, const float *H, const double *B,
+                      const double *p_hist, const double *u_hist, const double *acc_hist, int32_t nhist, double *b) {
+    if (!ctx) return CQBEM_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    try {
+        if (n3 <= 0 || (n3 & 1) || L < 1 || nhist < 0 || !b) throw Err{CQBEM_ERR_ARG, "history_f32: bad sizes or null output"};
+        const int m = std::min<int>(nhist, L - 1);
+        if (m > 0 && (((G && !p_hist) || (H && !u_hist) || (B && !acc_hist))))
+            throw Err{CQBEM_ERR_ARG, "history_f32: a matrix without its history"};
+        HIPCHK(hipSetDevice(ctx->device));
+        if (m == 0 || (!G && !H && !B)) {
+            HIPCHK(hipMemsetAsync(b, 0, sizeof(double) * n3, ctx->stream));
+        } else {
+            const hipError_t e = launch_history_f32(G, H, B, p_hist, u_hist, acc_hist, n3, m, nhist, b, ctx->stream);
+            if (e == hipErrorInvalidValue) throw Err{CQBEM_ERR_ARG, "history_f32: misaligned matrices or histories"};
+            HIPCHK(e);
         }
         HIPCHK(hipStreamSynchronize(ctx->stream));
     } catch (const Err &e) {
@@ -807,6 +1117,30 @@ int cqbem_assemble(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_param
     return CQBEM_OK;
 }
 
+int cqbem_assemble_scene(cqbem_context *ctx, int32_t n, const cqbem_mesh *meshes, const cqbem_params *params,
+                         const cqbem_outputs *outs) {
+    if (!ctx || n < 0 || (n > 0 && (!meshes || !params || !outs))) return CQBEM_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const auto t0 = std::chrono::steady_clock::now();
+    double tm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    try {
+        HIPCHK(hipSetDevice(ctx->device));
+        struct TimersOff {  // no event waits inside the captured graphs
+            bool prev = phase_timers_on();
+            TimersOff() { phase_timers_on() = false; }
+            ~TimersOff() { phase_timers_on() = prev; }
+        } timers_off;
+        run_scene(ctx, ctx->stream, n, meshes, params, outs, tm);
+    } catch (const Err &e) {
+        return fail(ctx, e);
+    } catch (const std::exception &e) {
+        return fail(ctx, Err{CQBEM_ERR_HIP, e.what()});
+    }
+    tm[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::copy(tm, tm + 8, ctx->timings);
+    return CQBEM_OK;
+}
+
 int cqbem_assemble_noalloc(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params *params,
                            const cqbem_outputs *out) {
     if (!ctx) return CQBEM_ERR_ARG;
@@ -875,7 +1209,21 @@ int cqbem_laplace(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params
         const double2 *line = d.line.as<double2>();
         const Freq *freqs = d.freqs.as<Freq>();
         const int npts = P.npts, nline = (int)P.line.size();
-        if (!params->galerkin) {
+        const bool mixed = params->precision == CQBEM_MIXED;
+        DBuf dfreq32;
+        if (mixed) upload(dfreq32, freqs_f32(P.freqs), st);
+        const MatF mat32 = mat_f32(P.mat);
+        if (!params->galerkin && mixed) {  // far-field G and B in float, H in double (as cqbem_assemble)
+            if (wH) HIPCHK(launch_gh(d, P, freqs, l, 1, 0, nv, nullptr, dP.as<cd>(), n3, n3 * n3, st));
+            if (wG) HIPCHK(launch_colloc_g32(d.g, quad, npts, dfreq32.as<FreqF>(), l, 1, mat32, 0, nv, dU.as<cd>(), n3, n3 * n3, st));
+            if (wG) HIPCHK(launch_colloc_singular(0, d.g, line, nline, freqs, l, 1, P.mat, 0, nv, dU.as<cd>(), n3, n3 * n3, nullptr, nullptr, st));
+            if (wH) HIPCHK(launch_colloc_singular(2, d.g, line, nline, freqs, l, 1, P.mat, 0, nv, dP.as<cd>(), n3, n3 * n3, nullptr, nullptr, st));
+            if (wB) {
+                HIPCHK(launch_colloc_b32(d.g, quad, npts, dfreq32.as<FreqF>(), l, 1, mat32, 0, nv, dBt.as<cd>(), dBa.as<cd>(), st));
+                HIPCHK(launch_colloc_singular(1, d.g, line, nline, freqs, l, 1, P.mat, 0, nv, nullptr, n3, 0, dBt.as<cd>(),
+                                              dBa.as<cd>(), st));
+            }
+        } else if (!params->galerkin) {
             if (wG || wH) {
                 HIPCHK(launch_gh(d, P, freqs, l, 1, 0, nv, dU.as<cd>(), dP.as<cd>(), n3, n3 * n3, st));
                 if (wG) HIPCHK(launch_colloc_singular(0, d.g, line, nline, freqs, l, 1, P.mat, 0, nv, dU.as<cd>(), n3, n3 * n3, nullptr, nullptr, st));
@@ -908,10 +1256,28 @@ int cqbem_laplace(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params
 }
 
 // ---- serving ----
-static constexpr int kSlots = 4;
+// Stream slots per context (concurrent jobs): CQBEM_SLOTS, 1..16, default 4.
+static int slot_count() {
+    const char *e = std::getenv("CQBEM_SLOTS");
+    const int v = e ? std::atoi(e) : 4;
+    return std::min(16, std::max(1, v));
+}
 
-int cqbem_submit(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params *params, const cqbem_outputs *out,
-                 uint64_t *job_id) {
+// Device bytes of the outputs a job writes (G and H at their storage size).
+static uint64_t output_bytes(const cqbem_mesh *mesh, const cqbem_params *p, const cqbem_outputs *out) {
+    const uint64_t n3 = 3 * (uint64_t)mesh->nv, L = (uint64_t)p->L;
+    const uint64_t esz = p->storage == CQBEM_STORE_F32 ? 4 : 8;
+    uint64_t b = 0;
+    if ((p->which & CQBEM_G) && out->G) b += L * n3 * n3 * esz;
+    if ((p->which & CQBEM_H) && out->H) b += L * n3 * n3 * esz;
+    if (out->B_trans) b += L * n3 * 3 * 8;
+    if (out->B_angular) b += L * n3 * 9 * 8;
+    if (out->B) b += L * n3 * 6 * 8;
+    return b;
+}
+
+static int submit_impl(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params *params, const cqbem_outputs *out,
+                       uint64_t *job_id, bool resident) {
     if (!ctx || !job_id) return CQBEM_ERR_ARG;
     std::lock_guard<std::mutex> lk(ctx->mu);
     try {
@@ -919,24 +1285,44 @@ int cqbem_submit(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params 
         if (!out || !out->on_device) throw Err{CQBEM_ERR_ARG, "serving needs device-resident outputs"};
         HIPCHK(hipSetDevice(ctx->device));
         if (ctx->slots.empty()) {
-            ctx->slots.resize(kSlots), ctx->slot_job.assign(kSlots, 0);
+            const int ns = slot_count();
+            ctx->slots.resize(ns), ctx->slot_job.assign(ns, 0);
             for (auto &s : ctx->slots) HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         }
+        const int nslots = (int)ctx->slots.size();
         // reap finished jobs' slots
-        for (int k = 0; k < kSlots; k++) {
+        for (int k = 0; k < nslots; k++) {
             const uint64_t jid = ctx->slot_job[k];
-            if (jid && ctx->jobs.count(jid) && ctx->jobs[jid]->done.load()) ctx->slot_job[k] = 0;
+            if (jid && (!ctx->jobs.count(jid) || ctx->jobs[jid]->done.load())) ctx->slot_job[k] = 0;
         }
-        const size_t slab = (size_t)2 << 30;
-        const uint64_t bytes = footprint(mesh, params, 1, slab);
-        if (bytes > ctx->budget) throw Err{CQBEM_ERR_NOMEM, "job footprint exceeds the HBM budget"};
         int slot = -1;
-        for (int k = 0; k < kSlots && slot < 0; k++)
+        for (int k = 0; k < nslots && slot < 0; k++)
             if (!ctx->slot_job[k]) slot = k;
-        if (slot < 0 || ctx->reserved + bytes > ctx->budget) {
-            ctx->last_error = "no free slot or HBM budget";
+        if (slot < 0) {
+            ctx->last_error = "no free stream slot";
             return CQBEM_ERR_BUSY;
         }
+        // admission by footprint: the job's scratch (and, when resident, its outputs) next to what is reserved
+        const uint64_t res = resident ? output_bytes(mesh, params, out) : 0;
+        const uint64_t fixed = footprint(mesh, params, 1, 0);  // scratch with one-row slabs
+        if (fixed + res > ctx->budget)
+            throw Err{CQBEM_ERR_NOMEM, "job footprint (" + std::to_string((fixed + res) >> 20) +
+                                           " MiB) exceeds the HBM budget"};
+        // the spectra slab: half of what the budget leaves (all of it up to 256 MiB), at most 8 GiB; one-row slabs
+        // when even that does not fit
+        const uint64_t used = ctx->reserved + fixed + res;
+        const uint64_t left = ctx->budget > used ? ctx->budget - used : 0;
+        size_t slab = (size_t)std::min<uint64_t>(std::max<uint64_t>(left / 2, std::min<uint64_t>(left, (uint64_t)256 << 20)),
+                                                 (uint64_t)8 << 30);
+        uint64_t scratch = footprint(mesh, params, 1, slab);
+        if (ctx->reserved + scratch + res > ctx->budget) slab = 0, scratch = fixed;
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        if (ctx->reserved + scratch + res > ctx->budget || (uint64_t)fr + ctx->pool.cached_bytes() < scratch) {
+            ctx->last_error = "HBM budget or device memory held by running and resident jobs";
+            return CQBEM_ERR_BUSY;
+        }
+        const uint64_t bytes = scratch;
         auto job = std::make_shared<JobRecord>();
         job->id = ctx->next_job++;
         job->V.assign(mesh->V, mesh->V + 3 * mesh->nv);
@@ -947,8 +1333,10 @@ int cqbem_submit(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params 
         job->params = *params;
         job->out = *out;
         job->bytes = bytes;
+        job->resident_bytes = res;
+        job->resident = resident;
         job->slot = slot;
-        ctx->reserved += bytes;
+        ctx->reserved += bytes + res;
         ctx->slot_job[slot] = job->id;
         ctx->jobs[job->id] = job;
         hipStream_t st = ctx->slots[slot];
@@ -974,6 +1362,37 @@ int cqbem_submit(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params 
     } catch (const Err &e) {
         return fail(ctx, e);
     }
+    return CQBEM_OK;
+}
+
+int cqbem_submit(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params *params, const cqbem_outputs *out,
+                 uint64_t *job_id) {
+    return submit_impl(ctx, mesh, params, out, job_id, false);
+}
+
+int cqbem_submit_resident(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params *params,
+                          const cqbem_outputs *out, uint64_t *job_id) {
+    return submit_impl(ctx, mesh, params, out, job_id, true);
+}
+
+int cqbem_release(cqbem_context *ctx, uint64_t job_id) {
+    if (!ctx) return CQBEM_ERR_ARG;
+    std::thread worker;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        auto it = ctx->jobs.find(job_id);
+        if (it == ctx->jobs.end() || !it->second->resident) return CQBEM_ERR_NOTFOUND;
+        if (!it->second->done.load()) {
+            ctx->last_error = "release: the job is still running";
+            return CQBEM_ERR_BUSY;
+        }
+        worker = std::move(it->second->worker);
+        ctx->reserved -= it->second->resident_bytes;
+        for (int k = 0; k < (int)ctx->slot_job.size(); k++)
+            if (ctx->slot_job[k] == job_id) ctx->slot_job[k] = 0;
+        ctx->jobs.erase(it);
+    }
+    if (worker.joinable()) worker.join();  // finished: returns at once
     return CQBEM_OK;
 }
 
@@ -1015,7 +1434,7 @@ int cqbem_wait(cqbem_context *ctx, uint64_t job_id) {
     std::lock_guard<std::mutex> lk(ctx->mu);
     for (int k = 0; k < (int)ctx->slot_job.size(); k++)
         if (ctx->slot_job[k] == job_id) ctx->slot_job[k] = 0;
-    ctx->jobs.erase(job_id);
+    if (!job->resident) ctx->jobs.erase(job_id);  // a resident job's record lives until cqbem_release
     if (job->status) {
         ctx->last_error = job->error;
         return job->status;

@@ -14,8 +14,10 @@
 // of src/elastodynamics_cqbem_{G,H,B_trans,B_angular}.cu.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
+#include <cstring>
 
 #include "cqbem_device.hpp"
 #include "cqbem_kernels.hpp"
@@ -158,18 +160,16 @@ __device__ inline double pair_swap(double v) {
     return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
+// One (patch pc, frequency l, row il) tile of the pair form, for a block of kPairThreads threads.  acc, fgeo and
+// mtab are the block's LDS (mtab loaded and visible); the tile zeroes acc itself and ends with its write-out
+// (callers that run another tile on the same LDS synchronise first).
 template <bool DG, bool DH, bool GAL>
-__global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, const double4 *__restrict__ pts, int nrounds,
-                                                                  const Freq *__restrict__ freqs, int l0, Mat m, int row0,
-                                                                  const double4 *__restrict__ tpts,
-                                                                  const int *__restrict__ patch_pass,
-                                                                  const int2 *__restrict__ pass_info,
-                                                                  const int4 *__restrict__ pass_faces,
-                                                                  const int *__restrict__ patch_verts, cd *__restrict__ UG,
-                                                                  cd *__restrict__ UH, long long ld, long long lstride) {
-    __shared__ double acc[kPairWords * kPairPV];  // [word][slot]
-    __shared__ double fgeo[kFaceWords * kPairFaces];  // [word][face of pass]: re-read every round, not held in VGPRs
-    const int pc = blockIdx.x, l = blockIdx.y, il = blockIdx.z;
+__device__ inline void pair_tile(const Geom &g, const double4 *__restrict__ pts, int nrounds, const Freq *__restrict__ freqs,
+                                 int l0, const Mat &m, int row0, const double4 *__restrict__ tpts,
+                                 const int *__restrict__ patch_pass, const int2 *__restrict__ pass_info,
+                                 const int4 *__restrict__ pass_faces, const int *__restrict__ patch_verts,
+                                 cd *__restrict__ UG, cd *__restrict__ UH, long long ld, long long lstride, int pc, int l,
+                                 int il, double *acc, double *fgeo, const MathTables &mtab) {
     const int i = row0 + il;
     const int tid = threadIdx.x, pr = tid >> 1, side = tid & 1;
     const Freq f = freqs[l0 + l];
@@ -185,8 +185,6 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
         x0 = g.V[3 * i], x1 = g.V[3 * i + 1], x2 = g.V[3 * i + 2];
     }
     for (int k = tid; k < kPairWords * kPairPV; k += kPairThreads) acc[k] = 0.0;
-    __shared__ MathTables mtab;  // tables of the exponentials
-    load_math_tables(&mtab);
     __syncthreads();
 
     for (int pass = patch_pass[pc]; pass < patch_pass[pc + 1]; ++pass) {
@@ -339,6 +337,79 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
                 if (DH) UH[obase + p * ld + b] = cd{a(14 + 2 * (p * 3 + b)), a(15 + 2 * (p * 3 + b))} * kInv4Pi;
             }
     }
+}
+
+template <bool DG, bool DH, bool GAL>
+__global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, const double4 *__restrict__ pts, int nrounds,
+                                                                  const Freq *__restrict__ freqs, int l0, Mat m, int row0,
+                                                                  const double4 *__restrict__ tpts,
+                                                                  const int *__restrict__ patch_pass,
+                                                                  const int2 *__restrict__ pass_info,
+                                                                  const int4 *__restrict__ pass_faces,
+                                                                  const int *__restrict__ patch_verts, cd *__restrict__ UG,
+                                                                  cd *__restrict__ UH, long long ld, long long lstride) {
+    __shared__ double acc[kPairWords * kPairPV];      // [word][slot]
+    __shared__ double fgeo[kFaceWords * kPairFaces];  // [word][face of pass]: re-read every round, not held in VGPRs
+    __shared__ MathTables mtab;                       // tables of the exponentials
+    load_math_tables(&mtab);
+    pair_tile<DG, DH, GAL>(g, pts, nrounds, freqs, l0, m, row0, tpts, patch_pass, pass_info, pass_faces, patch_verts, UG, UH,
+                           ld, lstride, blockIdx.x, blockIdx.y, blockIdx.z, acc, fgeo, mtab);
+}
+
+// ------------------------------------------------------------------------
+// Several objects' G/H regular-face work in one launch: each piece (an object's row slab) contributes
+// npatches x nl x nrows tiles, numbered from its `base`, and each tile runs exactly the computation of
+// k_colloc_gh_pair's block for it, so every object's spectra are bit-identical to its own launch.  Two forms:
+//   k_scene_gh_tiles: one block per tile; the hardware's workgroup dispatcher hands tiles to CUs in tile order as
+//     they free up (pieces are ordered largest first by the host).  Same registers as the single-object kernel
+//     plus the piece lookup: 2 waves per SIMD.
+//   k_scene_gh_pair: persistent blocks (2 per CU) that take tile numbers from a device counter with one atomic add
+//     per tile.  The loop-carried state costs 8 registers beyond 256, i.e. 1 wave per SIMD (CQBEM_SCENE_KERNEL=
+//     persistent selects it, for comparison).
+// ------------------------------------------------------------------------
+template <bool DG, bool DH>
+__device__ inline void scene_tile(const ScenePiece *__restrict__ pieces, int npieces, unsigned t, double *acc, double *fgeo,
+                                  const MathTables &mtab) {
+    int pi = 0;
+    while (pi + 1 < npieces && pieces[pi + 1].base <= t) ++pi;
+    const ScenePiece *P = pieces + pi;
+    const unsigned local = t - P->base, np = (unsigned)P->npatches, nl = (unsigned)P->nl;
+    const unsigned rest = local / np;
+    pair_tile<DG, DH, false>(P->g, P->pts, P->nrounds, P->freqs, 0, P->m, P->row0, nullptr, P->patch_pass, P->pass_info,
+                             P->pass_faces, P->patch_verts, P->UG, P->UH, P->ld, P->lstride, (int)(local - rest * np),
+                             (int)(rest % nl), (int)(rest / nl), acc, fgeo, mtab);
+}
+
+template <bool DG, bool DH>
+__global__ void __launch_bounds__(kPairThreads, 2) k_scene_gh_tiles(const ScenePiece *__restrict__ pieces, int npieces) {
+    __shared__ double acc[kPairWords * kPairPV];
+    __shared__ double fgeo[kFaceWords * kPairFaces];
+    __shared__ MathTables mtab;
+    load_math_tables(&mtab);
+    scene_tile<DG, DH>(pieces, npieces, blockIdx.x, acc, fgeo, mtab);
+}
+
+template <bool DG, bool DH>
+__global__ void __launch_bounds__(kPairThreads, 2) k_scene_gh_pair(const ScenePiece *__restrict__ pieces, int npieces,
+                                                                 unsigned ntiles, unsigned *__restrict__ counter) {
+    __shared__ double acc[kPairWords * kPairPV];
+    __shared__ double fgeo[kFaceWords * kPairFaces];
+    __shared__ MathTables mtab;
+    __shared__ unsigned next_tile;
+    load_math_tables(&mtab);
+    for (;;) {
+        __syncthreads();  // the previous tile's write-out has read acc; next_tile is free
+        if (threadIdx.x == 0) next_tile = atomicAdd(counter, 1u);
+        __syncthreads();
+        const unsigned t = __builtin_amdgcn_readfirstlane(next_tile);
+        if (t >= ntiles) break;  // uniform across the block: every wave leaves
+        scene_tile<DG, DH>(pieces, npieces, t, acc, fgeo, mtab);
+    }
+}
+
+// Zeroes the tile counter of a scene launch (a kernel, not a memset: it is captured in the scene's graph).
+__global__ void k_zero_counter(unsigned *counter) {
+    if (threadIdx.x == 0) *counter = 0u;
 }
 
 // ------------------------------------------------------------------------
@@ -745,10 +816,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
 // that LC loads per lane are in flight.
 constexpr int kTransformNC = 32, kTransformLC = 8, kTransformMaxLf = 64;
 
-template <int NC, int LC>
+// OutT: double, or float for resident weights stored in single precision (the sum is formed in double and
+// rounded once).
+template <int NC, int LC, class OutT>
 __global__ void __launch_bounds__(256) k_transform(const cd *__restrict__ spec, long long E, long long spec_ls, int Lf,
                                                     const double2 *__restrict__ tab, int L, int n0,
-                                                    double *__restrict__ out, long long out_nstride) {
+                                                    OutT *__restrict__ out, long long out_nstride) {
     __shared__ double2 T[NC * kTransformMaxLf];  // [l][k]
     for (int t = threadIdx.x; t < NC * Lf; t += 256) {
         const int l = t / NC, k = t % NC;
@@ -779,16 +852,16 @@ __global__ void __launch_bounds__(256) k_transform(const cd *__restrict__ spec, 
 #pragma unroll
         for (int k = 0; k < NC; k++) {
             const int n = n0 + k;
-            if (n < L) __builtin_nontemporal_store(acc[k], out + (long long)n * out_nstride + e);  // written once
+            if (n < L) __builtin_nontemporal_store((OutT)acc[k], out + (long long)n * out_nstride + e);  // written once
         }
     }
 }
 
 // Same for Lf > kTransformMaxLf, with the table read from global memory.
-template <int NC>
+template <int NC, class OutT>
 __global__ void __launch_bounds__(256) k_transform_wide(const cd *__restrict__ spec, long long E, long long spec_ls, int Lf,
                                                          const double2 *__restrict__ tab, int L, int n0,
-                                                         double *__restrict__ out, long long out_nstride) {
+                                                         OutT *__restrict__ out, long long out_nstride) {
     const long long stride = (long long)gridDim.x * 256;
     for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < E; e += stride) {
         double acc[NC];
@@ -808,7 +881,7 @@ __global__ void __launch_bounds__(256) k_transform_wide(const cd *__restrict__ s
 #pragma unroll
         for (int k = 0; k < NC; k++) {
             const int n = n0 + k;
-            if (n < L) out[(long long)n * out_nstride + e] = acc[k];
+            if (n < L) out[(long long)n * out_nstride + e] = (OutT)acc[k];
         }
     }
 }
@@ -879,9 +952,15 @@ __global__ void __launch_bounds__(256) k_history(const double *__restrict__ G, c
 // B terms are spread over the lanes before the wave reduction.  Matrix i starts at G + i * mat_stride (n3^2 for the
 // history; 0 with m = nh = 1 makes it y = A x).
 typedef double hist_d2 __attribute__((ext_vector_type(2)));
+typedef float hist_f2 __attribute__((ext_vector_type(2)));
+template <class S> struct HistPair;
+template <> struct HistPair<double> { typedef hist_d2 type; };
+template <> struct HistPair<float> { typedef hist_f2 type; };
 
-template <bool HG, bool HH, int R>
-__global__ void __launch_bounds__(256) k_history_v2(const double *__restrict__ G, const double *__restrict__ H,
+// S: the matrices' element type (double, or float for weights stored in single precision; the products and
+// sums are formed in double either way).
+template <class S, bool HG, bool HH, int R>
+__global__ void __launch_bounds__(256) k_history_v2(const S *__restrict__ G, const S *__restrict__ H,
                                                      const double *__restrict__ B, const double *__restrict__ p,
                                                      const double *__restrict__ u, const double *__restrict__ acc,
                                                      long long n3, long long mat_stride, int m, int nh,
@@ -900,15 +979,17 @@ __global__ void __launch_bounds__(256) k_history_v2(const double *__restrict__ G
         const long long mo = (long long)i * mat_stride;
         const hist_d2 *xp = HG ? reinterpret_cast<const hist_d2 *>(p + (long long)(nh - i) * n3) : nullptr;
         const hist_d2 *xu = HH ? reinterpret_cast<const hist_d2 *>(u + (long long)(nh - i) * n3) : nullptr;
-        const hist_d2 *gr[R], *hr[R];
+        typedef typename HistPair<S>::type s2;
+        const s2 *gr[R], *hr[R];
 #pragma unroll
         for (int k = 0; k < R; k++) {
-            gr[k] = HG ? reinterpret_cast<const hist_d2 *>(G + mo + rr[k] * n3) : nullptr;
-            hr[k] = HH ? reinterpret_cast<const hist_d2 *>(H + mo + rr[k] * n3) : nullptr;
+            gr[k] = HG ? reinterpret_cast<const s2 *>(G + mo + rr[k] * n3) : nullptr;
+            hr[k] = HH ? reinterpret_cast<const s2 *>(H + mo + rr[k] * n3) : nullptr;
         }
         long long c = lane;
         for (; c + 64 * (kHistUnroll - 1) < nc; c += 64 * kHistUnroll) {
-            hist_d2 x[kHistUnroll], y[kHistUnroll], g[R][kHistUnroll], h[R][kHistUnroll];
+            hist_d2 x[kHistUnroll], y[kHistUnroll];
+            s2 g[R][kHistUnroll], h[R][kHistUnroll];
 #pragma unroll
             for (int j = 0; j < kHistUnroll; j++) {
                 if (HG) x[j] = xp[c + 64 * j];
@@ -923,8 +1004,8 @@ __global__ void __launch_bounds__(256) k_history_v2(const double *__restrict__ G
             for (int j = 0; j < kHistUnroll; j++)
 #pragma unroll
                 for (int k = 0; k < R; k++) {
-                    if (HG) s[k] = fma(g[k][j].y, x[j].y, fma(g[k][j].x, x[j].x, s[k]));
-                    if (HH) s[k] = fma(-h[k][j].y, y[j].y, fma(-h[k][j].x, y[j].x, s[k]));
+                    if (HG) s[k] = fma((double)g[k][j].y, x[j].y, fma((double)g[k][j].x, x[j].x, s[k]));
+                    if (HH) s[k] = fma(-(double)h[k][j].y, y[j].y, fma(-(double)h[k][j].x, y[j].x, s[k]));
                 }
         }
         for (; c < nc; c += 64) {
@@ -932,12 +1013,12 @@ __global__ void __launch_bounds__(256) k_history_v2(const double *__restrict__ G
 #pragma unroll
             for (int k = 0; k < R; k++) {
                 if (HG) {
-                    const hist_d2 g = __builtin_nontemporal_load(gr[k] + c);
-                    s[k] = fma(g.y, x.y, fma(g.x, x.x, s[k]));
+                    const s2 g = __builtin_nontemporal_load(gr[k] + c);
+                    s[k] = fma((double)g.y, x.y, fma((double)g.x, x.x, s[k]));
                 }
                 if (HH) {
-                    const hist_d2 h = __builtin_nontemporal_load(hr[k] + c);
-                    s[k] = fma(-h.y, y.y, fma(-h.x, y.x, s[k]));
+                    const s2 h = __builtin_nontemporal_load(hr[k] + c);
+                    s[k] = fma(-(double)h.y, y.y, fma(-(double)h.x, y.x, s[k]));
                 }
             }
         }
@@ -972,19 +1053,22 @@ __global__ void __launch_bounds__(256) k_matvec(const double *__restrict__ A, co
 }
 
 // H_0 += C on the 3x3 diagonal blocks (collocation, H.cu:186-191).
-__global__ void k_add_c_colloc(double *H0, const double *C, int nv) {
+template <class T>
+__global__ void k_add_c_colloc(T *H0, const double *C, int nv) {
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= 9 * nv) return;
     const int v = t / 9, a = (t % 9) / 3, b = t % 3;
     const long long n3 = 3LL * nv;
-    H0[(3LL * v + a) * n3 + 3 * v + b] += C[t];
+    T &h = H0[(3LL * v + a) * n3 + 3 * v + b];
+    h = (T)((double)h + C[t]);
 }
 
 // H_0 += C for Galerkin: sparse (row, col, value) triplets, one lane each, rows unique per launch.
-__global__ void k_add_triplets(double *H0, const long long *idx, const double *val, long long n) {
+template <class T>
+__global__ void k_add_triplets(T *H0, const long long *idx, const double *val, long long n) {
     const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
     if (t >= n) return;
-    H0[idx[t]] += val[t];
+    H0[idx[t]] = (T)((double)H0[idx[t]] + val[t]);
 }
 
 // B_angular += B_trans (x) x_v and B = [B_trans | B_euler] (B_angular.cu:167-175, matrix_collection.cpp:280-305).
@@ -1043,6 +1127,35 @@ hipError_t launch_colloc_gh_pair(const Geom &g, const double4 *pts, int npts_eve
 #undef CQBEM_PAIR
     return hipGetLastError();
 }
+hipError_t launch_scene_gh_pair(const ScenePiece *pieces, int npieces, long long ntiles, unsigned *counter, bool G, bool H,
+                                int num_cus, hipStream_t st) {
+    if ((!G && !H) || npieces <= 0 || ntiles <= 0 || ntiles >= (1LL << 31)) return hipErrorInvalidValue;
+    const char *env = std::getenv("CQBEM_SCENE_KERNEL");  // read per launch (tests switch it)
+    const bool persistent = env && !std::strcmp(env, "persistent");
+    if (!persistent) {
+        const dim3 grid((unsigned)ntiles);
+        if (G && H)
+            hipLaunchKernelGGL((k_scene_gh_tiles<true, true>), grid, dim3(kPairThreads), 0, st, pieces, npieces);
+        else if (G)
+            hipLaunchKernelGGL((k_scene_gh_tiles<true, false>), grid, dim3(kPairThreads), 0, st, pieces, npieces);
+        else
+            hipLaunchKernelGGL((k_scene_gh_tiles<false, true>), grid, dim3(kPairThreads), 0, st, pieces, npieces);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, counter);
+    const unsigned blocks = (unsigned)std::min<long long>(ntiles, 2LL * num_cus);
+    if (G && H)
+        hipLaunchKernelGGL((k_scene_gh_pair<true, true>), dim3(blocks), dim3(kPairThreads), 0, st, pieces, npieces,
+                           (unsigned)ntiles, counter);
+    else if (G)
+        hipLaunchKernelGGL((k_scene_gh_pair<true, false>), dim3(blocks), dim3(kPairThreads), 0, st, pieces, npieces,
+                           (unsigned)ntiles, counter);
+    else
+        hipLaunchKernelGGL((k_scene_gh_pair<false, true>), dim3(blocks), dim3(kPairThreads), 0, st, pieces, npieces,
+                           (unsigned)ntiles, counter);
+    return hipGetLastError();
+}
+
 int pair_patch_size() { return kPairPV; }
 int pair_pass_faces() { return kPairFaces; }
 
@@ -1080,43 +1193,51 @@ hipError_t launch_colloc_singular(int kind, const Geom &g, const double2 *line, 
     return hipGetLastError();
 }
 
-hipError_t launch_transform(const cd *spec, long long E, long long spec_ls, int Lf, const double2 *tab, int L, double *out,
-                            long long out_nstride,
-                            int num_cus, hipStream_t st) {
+template <class OutT>
+static hipError_t launch_transform_t(const cd *spec, long long E, long long spec_ls, int Lf, const double2 *tab, int L,
+                                     OutT *out, long long out_nstride, int num_cus, hipStream_t st) {
     long long blocks = (E + 255) / 256;
     const long long cap = (long long)num_cus * 64;
     if (blocks > cap) blocks = cap;
     if (blocks < 1) blocks = 1;
     for (int n0 = 0; n0 < L; n0 += kTransformNC) {
         if (Lf <= kTransformMaxLf)
-            hipLaunchKernelGGL((k_transform<kTransformNC, kTransformLC>), dim3((unsigned)blocks), dim3(256), 0, st, spec, E,
-                               spec_ls, Lf, tab, L, n0, out, out_nstride);
+            hipLaunchKernelGGL((k_transform<kTransformNC, kTransformLC, OutT>), dim3((unsigned)blocks), dim3(256), 0, st, spec,
+                               E, spec_ls, Lf, tab, L, n0, out, out_nstride);
         else
-            hipLaunchKernelGGL((k_transform_wide<kTransformNC>), dim3((unsigned)blocks), dim3(256), 0, st, spec, E, spec_ls, Lf,
-                               tab, L, n0, out, out_nstride);
+            hipLaunchKernelGGL((k_transform_wide<kTransformNC, OutT>), dim3((unsigned)blocks), dim3(256), 0, st, spec, E,
+                               spec_ls, Lf, tab, L, n0, out, out_nstride);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
+hipError_t launch_transform(const cd *spec, long long E, long long spec_ls, int Lf, const double2 *tab, int L, double *out,
+                            long long out_nstride, int num_cus, hipStream_t st) {
+    return launch_transform_t(spec, E, spec_ls, Lf, tab, L, out, out_nstride, num_cus, st);
+}
+hipError_t launch_transform(const cd *spec, long long E, long long spec_ls, int Lf, const double2 *tab, int L, float *out,
+                            long long out_nstride, int num_cus, hipStream_t st) {
+    return launch_transform_t(spec, E, spec_ls, Lf, tab, L, out, out_nstride, num_cus, st);
+}
 
-template <int R>
-static void launch_history_v2(const double *G, const double *H, const double *B, const double *p, const double *u,
+template <int R, class S>
+static void launch_history_v2(const S *G, const S *H, const double *B, const double *p, const double *u,
                               const double *acc, long long n3, long long mat_stride, int m, int nh, double *b,
                               hipStream_t st) {
     const long long waves = (n3 + R - 1) / R;
     const dim3 grid((unsigned)((waves + 3) / 4));
     if (G && H)
-        hipLaunchKernelGGL((k_history_v2<true, true, R>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, mat_stride, m,
+        hipLaunchKernelGGL((k_history_v2<S, true, true, R>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, mat_stride, m,
                            nh, b);
     else if (G)
-        hipLaunchKernelGGL((k_history_v2<true, false, R>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, mat_stride, m,
+        hipLaunchKernelGGL((k_history_v2<S, true, false, R>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, mat_stride, m,
                            nh, b);
     else if (H)
-        hipLaunchKernelGGL((k_history_v2<false, true, R>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, mat_stride, m,
+        hipLaunchKernelGGL((k_history_v2<S, false, true, R>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, mat_stride, m,
                            nh, b);
     else
-        hipLaunchKernelGGL((k_history_v2<false, false, R>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, mat_stride,
+        hipLaunchKernelGGL((k_history_v2<S, false, false, R>), grid, dim3(256), 0, st, G, H, B, p, u, acc, n3, mat_stride,
                            m, nh, b);
 }
 
@@ -1129,6 +1250,16 @@ static int history_rows() {
 }
 
 static bool aligned16(const double *q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; }
+static bool aligned8(const void *q) { return (reinterpret_cast<uintptr_t>(q) & 7) == 0; }
+
+hipError_t launch_history_f32(const float *G, const float *H, const double *B, const double *p, const double *u,
+                              const double *acc, long long n3, int m, int nh, double *b, hipStream_t st) {
+    // single-precision matrices: 8-byte pairs of weights against 16-byte pairs of history entries
+    if (n3 % 2 != 0 || (G && (!aligned8(G) || !aligned16(p))) || (H && (!aligned8(H) || !aligned16(u))))
+        return hipErrorInvalidValue;
+    launch_history_v2<4>(G, H, B, p, u, acc, n3, n3 * n3, m, nh, b, st);
+    return hipGetLastError();
+}
 
 hipError_t launch_history(const double *G, const double *H, const double *B, const double *p, const double *u,
                           const double *acc, long long n3, int m, int nh, double *b, hipStream_t st) {
@@ -1156,9 +1287,9 @@ hipError_t launch_matvec(const double *A, const double *x, long long n, double *
     const int rows = history_rows();
     if (rows > 1 && n % 2 == 0 && aligned16(A) && aligned16(x)) {
         if (rows == 4)
-            launch_history_v2<4>(A, nullptr, nullptr, x, nullptr, nullptr, n, 0, 1, 1, y, st);
+            launch_history_v2<4, double>(A, nullptr, nullptr, x, nullptr, nullptr, n, 0, 1, 1, y, st);
         else
-            launch_history_v2<2>(A, nullptr, nullptr, x, nullptr, nullptr, n, 0, 1, 1, y, st);
+            launch_history_v2<2, double>(A, nullptr, nullptr, x, nullptr, nullptr, n, 0, 1, 1, y, st);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_matvec, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, A, x, n, y);
@@ -1166,13 +1297,22 @@ hipError_t launch_matvec(const double *A, const double *x, long long n, double *
 }
 
 hipError_t launch_add_c_colloc(double *H0, const double *C, int nv, hipStream_t st) {
-    hipLaunchKernelGGL(k_add_c_colloc, dim3((9 * nv + 255) / 256), dim3(256), 0, st, H0, C, nv);
+    hipLaunchKernelGGL(k_add_c_colloc<double>, dim3((9 * nv + 255) / 256), dim3(256), 0, st, H0, C, nv);
+    return hipGetLastError();
+}
+hipError_t launch_add_c_colloc(float *H0, const double *C, int nv, hipStream_t st) {
+    hipLaunchKernelGGL(k_add_c_colloc<float>, dim3((9 * nv + 255) / 256), dim3(256), 0, st, H0, C, nv);
     return hipGetLastError();
 }
 
 hipError_t launch_add_triplets(double *H0, const long long *idx, const double *val, long long n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_add_triplets, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, H0, idx, val, n);
+    hipLaunchKernelGGL(k_add_triplets<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, H0, idx, val, n);
+    return hipGetLastError();
+}
+hipError_t launch_add_triplets(float *H0, const long long *idx, const double *val, long long n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_add_triplets<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, H0, idx, val, n);
     return hipGetLastError();
 }
 

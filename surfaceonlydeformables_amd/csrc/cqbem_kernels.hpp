@@ -36,6 +36,28 @@ hipError_t launch_colloc_gh_pair(const Geom &g, const double4 *pts, int npts_eve
                                  long long lstride, hipStream_t st);
 int pair_patch_size();
 int pair_pass_faces();
+
+// One object's row slab in a multi-object launch of the pair kernel (collocation rows row0..row0+nrows-1 at all
+// nl frequencies of `freqs`): its npatches x nl x nrows tiles are numbered base .. base + npatches nl nrows - 1.
+struct ScenePiece {
+    Geom g;
+    const double4 *pts;
+    int nrounds;  // padded rule points / 2
+    const Freq *freqs;
+    Mat m;
+    int row0, nrows, nl, npatches;
+    const int *patch_pass;
+    const int2 *pass_info;
+    const int4 *pass_faces;
+    const int *patch_verts;
+    cd *UG, *UH;
+    long long ld, lstride;
+    unsigned base;
+};
+// The pieces (device array, ascending base, all with the same G / H selection) in one persistent launch of
+// min(ntiles, 2 blocks per CU) blocks that take tiles from *counter (zeroed by the launcher with a kernel).
+hipError_t launch_scene_gh_pair(const ScenePiece *pieces, int npieces, long long ntiles, unsigned *counter, bool G, bool H,
+                                int num_cus, hipStream_t st);
 hipError_t launch_colloc_b(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
                            int row0, int nrows, cd *Bt, cd *Ba, hipStream_t st);
 // B_trans / B_angular rows of the test points tpts[row0..row0+nrows-1] (Galerkin non-singular set, as above), into
@@ -50,12 +72,20 @@ hipError_t launch_colloc_singular(int kind, const Geom &g, const double2 *line, 
 hipError_t launch_transform(const cd *spec, long long E, long long spec_ls, int Lf, const double2 *tab, int L, double *out,
                             long long out_nstride,
                             int num_cus, hipStream_t st);
+// The same with the weights rounded to float (resident storage in single precision).
+hipError_t launch_transform(const cd *spec, long long E, long long spec_ls, int Lf, const double2 *tab, int L, float *out,
+                            long long out_nstride, int num_cus, hipStream_t st);
 hipError_t launch_add_c_colloc(double *H0, const double *C, int nv, hipStream_t st);
+hipError_t launch_add_c_colloc(float *H0, const double *C, int nv, hipStream_t st);
 // b = sum_{i=1..m} (G_i p[nh-i] - H_i u[nh-i] + B_i acc[nh-i]); any of G, H, B may be null.
 hipError_t launch_history(const double *G, const double *H, const double *B, const double *p, const double *u,
                           const double *acc, long long n3, int m, int nh, double *b, hipStream_t st);
 hipError_t launch_matvec(const double *A, const double *x, long long n, double *y, hipStream_t st);
 hipError_t launch_add_triplets(double *H0, const long long *idx, const double *val, long long n, hipStream_t st);
+hipError_t launch_add_triplets(float *H0, const long long *idx, const double *val, long long n, hipStream_t st);
+// The history sum with G_i, H_i stored in float (n3 even, G/H 8-byte and p/u 16-byte aligned).
+hipError_t launch_history_f32(const float *G, const float *H, const double *B, const double *p, const double *u,
+                              const double *acc, long long n3, int m, int nh, double *b, hipStream_t st);
 hipError_t launch_b_post(double *Bt, double *Ba, double *B6, const double *V, int nv, int L, hipStream_t st);
 
 }  // namespace cqbem

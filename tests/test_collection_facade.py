@@ -10,6 +10,7 @@ import subprocess
 import numpy as np
 import pytest
 
+from post_numpy import cross_matrix as _cross
 from post_numpy import tetra_inertia as _tetra_inertia
 from surfaceonlydeformables_amd import mesh as M
 

@@ -93,13 +93,113 @@ def test_job_larger_than_budget_is_refused():
     _, c1, c2 = M.material_speeds(5e5, 0.3, RHO)
     p = abi.make_params(c1, c2, RHO, 0.3, 1e-3, 1e-10 ** (1 / 32), 16)
     mesh = abi.make_mesh(V, F)
-    need = abi.job_footprint(mesh, p)
-    ctx = abi.Context(0, hbm_budget_bytes=need // 2)
+    # below even the one-row-slab scratch (the B spectra and the quadrature tables alone are larger)
+    ctx = abi.Context(0, hbm_budget_bytes=64 << 10)
     try:
         outs = device_outputs(torch, len(V), p.L)
         with pytest.raises(abi.CQBEMError) as err:
             ctx.submit(mesh, p, {n: t.data_ptr() for n, t in outs.items()})
         assert err.value.code == abi.ERR_NOMEM
+        assert ctx.reserved_bytes() == 0
+    finally:
+        ctx.close()
+
+
+def test_resident_jobs_keep_their_reservation_until_released():
+    """cqbem_submit_resident: the job's outputs stay reserved after it finishes, so admission counts them, until
+    cqbem_release; releasing a running, unknown or non-resident job is refused."""
+    import torch
+    V, F = M.icosphere(1, 0.2)
+    _, c1, c2 = M.material_speeds(5e5, 0.3, RHO)
+    p = abi.make_params(c1, c2, RHO, 0.3, 1e-3, 1e-10 ** (1 / 32), 16)
+    mesh = abi.make_mesh(V, F)
+    n3, L = 3 * len(V), 16
+    ctx = abi.Context(0)
+    try:
+        outs = device_outputs(torch, len(V), L)
+        jid = ctx.submit(mesh, p, {n: t.data_ptr() for n, t in outs.items()}, resident=True)
+        assert jid > 0
+        ctx.wait(jid)
+        want = L * n3 * n3 * 8 * 2 + L * n3 * (3 + 9 + 6) * 8
+        assert ctx.reserved_bytes() == want
+        assert ctx.poll(jid)  # the record stays until release
+        ctx.release(jid)
+        assert ctx.reserved_bytes() == 0
+        with pytest.raises(abi.CQBEMError) as err:
+            ctx.release(jid)
+        assert err.value.code == abi.ERR_NOTFOUND
+        plain = ctx.submit(mesh, p, {n: t.data_ptr() for n, t in outs.items()})
+        ctx.wait(plain)
+        with pytest.raises(abi.CQBEMError):
+            ctx.release(plain)
+    finally:
+        ctx.close()
+
+
+def test_single_precision_storage_policy():
+    """A job whose resident double outputs exceed the budget is refused (CQBEM_ERR_NOMEM); stored in float it is
+    admitted.  Its G weights are the double weights rounded once, bit for bit, and so are H's for n >= 1 and H_0's
+    off the vertices' own blocks (on those, H_0 gets the jump term C after the rounding).  The history sum on the float matrices equals, bit for bit,
+    the double history kernel on the same matrices widened to double."""
+    import torch
+    V, F = M.icosphere(2, 0.2)
+    V = M.center(M.jitter(V, F, 0.01, 2), F)
+    _, c1, c2 = M.material_speeds(5e5, 0.3, RHO)
+    L = 16
+    nv, n3 = len(V), 3 * len(V)
+    mesh = abi.make_mesh(V, F)
+    p64 = abi.make_params(c1, c2, RHO, 0.3, 1e-3, 1e-10 ** (1 / (2 * L)), L)
+    p32 = abi.make_params(c1, c2, RHO, 0.3, 1e-3, 1e-10 ** (1 / (2 * L)), L, storage=abi.STORE_F32)
+    out64 = device_outputs(torch, nv, L)
+    dev = torch.device("cuda:0")
+    out32 = {"G": torch.full((L, n3, n3), float("nan"), dtype=torch.float32, device=dev),
+             "H": torch.full((L, n3, n3), float("nan"), dtype=torch.float32, device=dev),
+             "B": torch.full((L, n3, 6), float("nan"), dtype=torch.float64, device=dev)}
+    torch.cuda.synchronize()
+    full = abi.Context(0)
+    try:
+        full.assemble_device(mesh, p64, {n: t.data_ptr() for n, t in out64.items()})
+    finally:
+        full.close()
+    gh64 = 2 * L * n3 * n3 * 8
+    budget = gh64 * 3 // 4  # the float outputs (half) plus scratch fit, the double ones do not
+    ctx = abi.Context(0, hbm_budget_bytes=budget)
+    try:
+        with pytest.raises(abi.CQBEMError) as err:
+            ctx.submit(mesh, p64, {n: t.data_ptr() for n, t in out64.items()}, resident=True)
+        assert err.value.code == abi.ERR_NOMEM
+        jid = ctx.submit(mesh, p32, {n: t.data_ptr() for n, t in out32.items()}, resident=True)
+        assert jid > 0
+        ctx.wait(jid)
+        torch.cuda.synchronize()
+        g32, h32 = out32["G"].cpu().numpy(), out32["H"].cpu().numpy()
+        g64, h64 = out64["G"].cpu().numpy(), out64["H"].cpu().numpy()
+        assert np.array_equal(g32, g64.astype(np.float32))
+        assert np.array_equal(h32[1:], h64[1:].astype(np.float32))
+        # H_0 off the vertices' own 3x3 blocks gets no jump term: rounded once, bit for bit.  On those blocks the
+        # weight was rounded before C (|C_ij| <= 1) was added and rounded again: within half a float ulp of
+        # |H_0| + 1 plus half one of |H_0|, at most 2^-23 (|H_0| + 1).
+        own = (np.arange(n3)[:, None] // 3) == (np.arange(n3)[None, :] // 3)
+        assert np.array_equal(h32[0][~own], h64[0][~own].astype(np.float32))
+        d = np.abs(h32[0].astype(np.float64) - h64[0])[own]
+        assert np.all(d <= 2.0 ** -23 * (np.abs(h64[0][own]) + 1.0))
+        assert np.array_equal(out32["B"].cpu().numpy(), out64["B"].cpu().numpy())
+        # history on the float matrices against the double kernel on the widened ones
+        nh = L - 1
+        gen = torch.Generator(device="cpu").manual_seed(3)
+        hp = torch.randn((nh, n3), generator=gen, dtype=torch.float64).to(dev)
+        hu = torch.randn((nh, n3), generator=gen, dtype=torch.float64).to(dev)
+        ha = torch.randn((nh, 6), generator=gen, dtype=torch.float64).to(dev)
+        b32 = torch.empty(n3, dtype=torch.float64, device=dev)
+        b64 = torch.empty(n3, dtype=torch.float64, device=dev)
+        gw, hw = out32["G"].double(), out32["H"].double()
+        torch.cuda.synchronize()
+        ctx.history_f32(n3, L, out32["G"].data_ptr(), out32["H"].data_ptr(), out32["B"].data_ptr(), hp.data_ptr(),
+                        hu.data_ptr(), ha.data_ptr(), nh, b32.data_ptr())
+        ctx.history(n3, L, gw.data_ptr(), hw.data_ptr(), out32["B"].data_ptr(), hp.data_ptr(), hu.data_ptr(),
+                    ha.data_ptr(), nh, b64.data_ptr())
+        assert torch.equal(b32, b64)
+        ctx.release(jid)
         assert ctx.reserved_bytes() == 0
     finally:
         ctx.close()
