@@ -143,14 +143,16 @@ def scene_jobs(n_objects=16, seed=0):
     return jobs
 
 
-def run_jobs(ctx, jobs, arrivals, torch):
+def run_jobs(ctx, jobs, arrivals, torch, outs=None):
     """Feeds jobs to the serving API at the given arrival times (seconds from start; BUSY answers wait
-    for a running job), keeps each job's outputs until it completes, and returns per-job latencies
-    (completion - arrival) and the makespan."""
+    for a running job), keeps each job's outputs until it completes (allocated per job, or outs[k] when given),
+    and returns per-job latencies (completion - arrival) and the makespan."""
     from surfaceonlydeformables_amd import abi
     dev = torch.device("cuda:0")
     queue, running, lat = [], {}, [None] * len(jobs)
     meshes = [abi.make_mesh(V, F) for V, F, _ in jobs]
+    stats = {"max_running": 0, "refused": {}}
+    head_outs = None  # outputs of the queue's first job, kept across refused submits
     t0 = time.perf_counter()
     nxt = 0
     while nxt < len(jobs) or queue or running:
@@ -162,15 +164,29 @@ def run_jobs(ctx, jobs, arrivals, torch):
             k = queue[0]
             V, F, p = jobs[k]
             n3 = 3 * len(V)
-            outs = {"G": torch.empty((p.L, n3, n3), dtype=torch.float64, device=dev),
-                    "H": torch.empty((p.L, n3, n3), dtype=torch.float64, device=dev),
-                    "B": torch.empty((p.L, n3, 6), dtype=torch.float64, device=dev)}
-            jid = ctx.submit(meshes[k], p, {n: t.data_ptr() for n, t in outs.items()})
+            if head_outs is None and outs is not None:
+                head_outs = outs[k]
+            if head_outs is None:
+                try:
+                    head_outs = {"G": torch.empty((p.L, n3, n3), dtype=torch.float64, device=dev),
+                                 "H": torch.empty((p.L, n3, n3), dtype=torch.float64, device=dev),
+                                 "B": torch.empty((p.L, n3, 6), dtype=torch.float64, device=dev)}
+                except torch.OutOfMemoryError:
+                    if not running:
+                        raise
+                    stats["refused"]["torch out of memory"] = stats["refused"].get("torch out of memory", 0) + 1
+                    break  # the outputs of running jobs hold the memory: wait for one to finish
+            jid = ctx.submit(meshes[k], p, {n: t.data_ptr() for n, t in head_outs.items()})
             if jid < 0:
-                del outs
+                why = abi.lib().cqbem_last_error(ctx._h).decode()
+                stats["refused"][why] = stats["refused"].get(why, 0) + 1
+                if not running:  # device memory held in torch's cache: give it back before the retry
+                    torch.cuda.empty_cache()
                 break
-            running[jid] = (k, outs)
+            running[jid] = (k, head_outs)
+            head_outs = None
             queue.pop(0)
+            stats["max_running"] = max(stats["max_running"], len(running))
         for jid in [j for j in running if ctx.poll(j)]:
             ctx.wait(jid)
             k, _ = running.pop(jid)
@@ -180,13 +196,12 @@ def run_jobs(ctx, jobs, arrivals, torch):
         else:
             time.sleep(0.0005)
     torch.cuda.synchronize()
+    run_jobs.stats = stats
     return lat, time.perf_counter() - t0
 
 
-def scene_graph(ctx, jobs, torch):
-    """All objects through cqbem_assemble_scene (shared tile launches, one HIP graph per wave), every object's
-    outputs allocated up front and resident at the end; returns the wall time of the call."""
-    from surfaceonlydeformables_amd import abi
+def scene_outputs(jobs, torch):
+    """Every object's G, H and B outputs on the device (allocated once, outside every timed region)."""
     dev = torch.device("cuda:0")
     outs = []
     for V, F, p in jobs:
@@ -194,48 +209,68 @@ def scene_graph(ctx, jobs, torch):
         outs.append({"G": torch.empty((p.L, n3, n3), dtype=torch.float64, device=dev),
                      "H": torch.empty((p.L, n3, n3), dtype=torch.float64, device=dev),
                      "B": torch.empty((p.L, n3, 6), dtype=torch.float64, device=dev)})
+    torch.cuda.synchronize()
+    return outs
+
+
+def scene_graph(ctx, jobs, outs, torch):
+    """All objects through cqbem_assemble_scene (shared tile launches, one HIP graph per wave) into `outs`; returns
+    the wall time of the call and its phase timings."""
+    from surfaceonlydeformables_amd import abi
     meshes = [abi.make_mesh(V, F) for V, F, _ in jobs]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ctx.assemble_scene(meshes, [p for *_, p in jobs], [{n: t.data_ptr() for n, t in o.items()} for o in outs])
     wall = time.perf_counter() - t0
-    last = ctx.last_timings()
-    del outs
-    torch.cuda.empty_cache()
-    return wall, last
+    return wall, ctx.last_timings()
 
 
 def scene_main(args):
     """Multi-object scene (16 objects, seed 0): cqbem_assemble_scene (all objects' tiles in shared launches,
     graph-captured waves) as the line's value, beside the same jobs submitted at once to the serving slots and one
-    after another."""
+    after another.  Every path writes the same preallocated outputs and is timed after a warm-up pass (the first
+    pass fills the library's device memory cache)."""
     import torch
     from surfaceonlydeformables_amd import abi
     jobs = scene_jobs()
-    ctx = abi.Context(0)
     entries = sum(2.0 * (3 * len(V)) ** 2 * p.L for V, F, p in jobs)
-    scene_graph(ctx, jobs[:3], torch)  # warm-up
-    graph_wall, graph_t = scene_graph(ctx, jobs, torch)
-    run_jobs(ctx, jobs[:2], [0.0, 0.0], torch)  # warm-up
-    lat, makespan = run_jobs(ctx, jobs, [0.0] * len(jobs), torch)
-    # the same jobs one after another, each alone on the device
+    outs = scene_outputs(jobs, torch)  # 257 GB: the same buffers for every path
+    ctx = abi.Context(0)
+    first_wall, _ = scene_graph(ctx, jobs, outs, torch)  # warm-up: the first call allocates the spectra region
+    graph_wall, graph_t = scene_graph(ctx, jobs, outs, torch)
+    os.environ["CQBEM_SCENE_GRAPH"] = "0"  # the same launches, on streams with concurrent branches, not captured
+    try:
+        direct_wall, direct_t = scene_graph(ctx, jobs, outs, torch)
+    finally:
+        del os.environ["CQBEM_SCENE_GRAPH"]
+    ctx.close()  # its device memory cache holds the scene's spectra region
+    if args.scene_only:
+        print(json.dumps({"scene_first_ms": first_wall * 1e3, "scene_graph_ms": graph_wall * 1e3, "graph": graph_t,
+                          "scene_uncaptured_ms": direct_wall * 1e3, "uncaptured": direct_t}), flush=True)
+        return
+    ctx = abi.Context(0)
+    run_jobs(ctx, jobs, [0.0] * len(jobs), torch, outs)  # warm-up
+    lat, makespan = run_jobs(ctx, jobs, [0.0] * len(jobs), torch, outs)
+    # the same jobs one after another, each alone on the device (a warm-up pass, then the timed pass)
+    meshes = [abi.make_mesh(V, F) for V, F, _ in jobs]
     serial = 0.0
-    dev = torch.device("cuda:0")
-    for V, F, p in jobs:
-        n3 = 3 * len(V)
-        outs = {"G": torch.empty((p.L, n3, n3), dtype=torch.float64, device=dev),
-                "H": torch.empty((p.L, n3, n3), dtype=torch.float64, device=dev),
-                "B": torch.empty((p.L, n3, 6), dtype=torch.float64, device=dev)}
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        ctx.assemble_device(abi.make_mesh(V, F), p, {n: t.data_ptr() for n, t in outs.items()})
-        serial += time.perf_counter() - t0
-        del outs
+    for rep in range(2):
+        serial = 0.0
+        for k, (V, F, p) in enumerate(jobs):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.assemble_device(meshes[k], p, {n: t.data_ptr() for n, t in outs[k].items()})
+            serial += time.perf_counter() - t0
     ctx.close()
+    del outs
+    torch.cuda.empty_cache()
     print(json.dumps({"metric": "multi-object scene: G+H entries/s (cqbem_assemble_scene)",
                       "value": entries / graph_wall, "unit": "entries/s", "objects": len(jobs),
                       "scene_graph_ms": graph_wall * 1e3, "scene_waves": graph_t["gh_launches"],
                       "scene_waves_ms": graph_t["regular_gh"], "scene_b_graph_ms": graph_t["regular_b"],
+                      "scene_host_ms": graph_t["host"], "scene_first_call_ms": first_wall * 1e3,
+                      "scene_uncaptured_ms": direct_wall * 1e3, "scene_uncaptured_waves_ms": direct_t["regular_gh"],
+                      "scene_uncaptured_b_ms": direct_t["regular_b"], "slots_stats": run_jobs.stats,
                       "slots_makespan_ms": makespan * 1e3, "serial_sum_ms": serial * 1e3,
                       "slots_latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3,
                                            "p99": float(np.percentile(lat, 99)) * 1e3},
@@ -543,6 +578,7 @@ def main():
                     help="one untimed assembly followed by a 1 GiB device copy, for HBM counter runs "
                          "(the copy calibrates FETCH_SIZE / WRITE_SIZE); prints no JSON line")
     ap.add_argument("--scene", action="store_true", help="multi-object scene throughput instead of the single-object line")
+    ap.add_argument("--scene-only", action="store_true", help="with --scene: only the cqbem_assemble_scene runs")
     ap.add_argument("--serving", type=float, default=0.0, help="Poisson serving stream at this many jobs/s")
     ap.add_argument("--duration", type=float, default=10.0, help="serving stream length in seconds")
     ap.add_argument("--time-stepping", action="store_true",

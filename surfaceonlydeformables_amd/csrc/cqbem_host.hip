@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <stdexcept>
 #include <memory>
 #include <mutex>
@@ -160,6 +161,9 @@ struct cqbem_context {
     std::vector<hipStream_t> slots;
     std::vector<uint64_t> slot_job;
     DevicePool pool;  // job buffers, reused across jobs
+    // scenes: the branches of a captured graph are recorded on these streams (fork and join through the events)
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> side_ev;
 };
 
 namespace {
@@ -669,20 +673,31 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
     HIPCHK(hipStreamSynchronize(st));
 }
 
-// Stream capture of a launch sequence into a HIP graph that is launched once and waited for.
+// Stream capture of a launch sequence into a HIP graph that is launched once and waited for.  With capture off
+// (CQBEM_SCENE_GRAPH=0) the same launches run directly on their streams and launch() only waits.
+bool scene_graphs_on() {
+    const char *e = std::getenv("CQBEM_SCENE_GRAPH");
+    return !(e && !std::strcmp(e, "0"));
+}
 struct GraphRun {
     hipStream_t st;
+    bool capture;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    explicit GraphRun(hipStream_t s) : st(s) { HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed)); }
+    explicit GraphRun(hipStream_t s) : st(s), capture(scene_graphs_on()) {
+        if (capture) HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    }
     void launch() {
-        HIPCHK(hipStreamEndCapture(st, &graph));
-        HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        HIPCHK(hipGraphLaunch(exec, st));
+        if (capture) {
+            HIPCHK(hipStreamEndCapture(st, &graph));
+            HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            HIPCHK(hipGraphLaunch(exec, st));
+        }
         HIPCHK(hipStreamSynchronize(st));
+        capture = false;
     }
     ~GraphRun() {
-        if (!graph) {  // capture abandoned by an exception: end it so the stream is usable again
+        if (capture && !graph) {  // capture abandoned by an exception: end it so the stream is usable again
             hipGraph_t g = nullptr;
             if (hipStreamEndCapture(st, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
         }
@@ -691,6 +706,54 @@ struct GraphRun {
     }
 };
 
+// Concurrent branches of an uncaptured launch sequence: fork() makes the side streams wait for what `st` has
+// queued so far, on(i) is the stream of branch i (round robin), join() makes `st` wait for every branch.  Inactive
+// (every branch on `st`) while `st` is being captured: the captured scene graphs are single-stream.
+struct Branches {
+    hipStream_t st;
+    bool active;
+    const std::vector<hipStream_t> &side;
+    const std::vector<hipEvent_t> &ev;  // side.size() + 1 events
+    Branches(cqbem_context *ctx, hipStream_t s, bool captured)
+        : st(s), active(!captured && !ctx->side.empty()), side(ctx->side), ev(ctx->side_ev) {}
+    void fork() {
+        if (!active) return;
+        HIPCHK(hipEventRecord(ev.back(), st));
+        for (hipStream_t s : side) HIPCHK(hipStreamWaitEvent(s, ev.back(), 0));
+    }
+    hipStream_t on(size_t i) const { return active ? side[i % side.size()] : st; }
+    void join() {
+        if (!active) return;
+        for (size_t i = 0; i < side.size(); i++) {
+            HIPCHK(hipEventRecord(ev[i], side[i]));
+            HIPCHK(hipStreamWaitEvent(st, ev[i], 0));
+        }
+    }
+};
+
+// Runs fn(0..n-1) on up to 16 host threads; the first exception is rethrown once all have finished.
+template <class Fn> void parallel_for(size_t n, Fn fn) {
+    const size_t nt = std::min<size_t>(n, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    std::atomic<size_t> next{0};
+    std::exception_ptr err;
+    std::mutex em;
+    auto work = [&]() {
+        for (size_t i; (i = next++) < n;) {
+            try {
+                fn(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> l(em);
+                if (!err) err = std::current_exception();
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; t++) th.emplace_back(work);
+    work();
+    for (auto &t : th) t.join();
+    if (err) std::rethrow_exception(err);
+}
+
 // A scene: every object's G/H regular-face tiles go through shared launches of the pair kernel (k_scene_gh_tiles:
 // all objects' tiles in one grid, largest pieces first), in waves whose spectra fit the scratch budget; each wave's
 // launches -- the shared pair kernel, the float G kernel of mixed objects, the singular faces and the transforms of
@@ -698,6 +761,12 @@ struct GraphRun {
 // follow in a last graph.  Galerkin objects run one after another through run_job.
 void run_scene(cqbem_context *ctx, hipStream_t st, int n, const cqbem_mesh *meshes, const cqbem_params *params,
                const cqbem_outputs *outs, double *tm) {
+    if (ctx->side.empty() && !scene_graphs_on()) {  // the branches' streams (default priority), made once per context
+        constexpr int kBranches = 8;
+        ctx->side.resize(kBranches), ctx->side_ev.resize(kBranches + 1);
+        for (auto &s : ctx->side) HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        for (auto &e : ctx->side_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     JobArena arena(&ctx->pool, st);
     ArenaScope arena_scope(&arena);
     struct Obj {
@@ -708,6 +777,7 @@ void run_scene(cqbem_context *ctx, hipStream_t st, int n, const cqbem_mesh *mesh
         double *Wbt = nullptr, *Wba = nullptr;  // B_trans / B_angular weights: the caller's, or staged for B
         bool wantG, wantH, wantB, mixed, f32;
         MatF mat32;
+        std::vector<double> C;  // jump term, host copy
     };
     std::vector<std::unique_ptr<Obj>> objs;
     std::vector<int> galerkin;
@@ -720,16 +790,33 @@ void run_scene(cqbem_context *ctx, hipStream_t st, int n, const cqbem_mesh *mesh
             galerkin.push_back(k);
             continue;
         }
-        auto o = std::make_unique<Obj>();
-        o->k = k;
+        objs.push_back(std::make_unique<Obj>());
+        objs.back()->k = k;
+    }
+    // host tables of every object (mesh tables, rules, frequencies, jump term) on host threads
+    parallel_for(objs.size(), [&](size_t j) {
+        Obj *o = objs[j].get();
+        const int k = o->k;
         const cqbem_params *p = &params[k];
         const cqbem_outputs *out = &outs[k];
         o->P = prepare(&meshes[k], p);
-        upload_mesh(o->d, &meshes[k], o->P, st);
         o->wantG = (p->which & CQBEM_G) && out->G, o->wantH = (p->which & CQBEM_H) && out->H;
         o->wantB = ((p->which & CQBEM_B_TRANS) && out->B_trans) || ((p->which & CQBEM_B_ANGULAR) && out->B_angular) || out->B;
         o->mixed = p->precision == CQBEM_MIXED, o->f32 = p->storage == CQBEM_STORE_F32;
         o->mat32 = mat_f32(o->P.mat);
+        if (o->wantH)
+            o->C = meshes[k].vertex_map_inverse
+                       ? c_matrix_collocation_split(meshes[k].V, o->P.nv, meshes[k].F, o->P.nf, o->P.N.data(),
+                                                    meshes[k].vertex_map_inverse, p->nu)
+                       : c_matrix_collocation(meshes[k].V, o->P.nv, meshes[k].F, o->P.nf, o->P.N.data(), p->nu);
+    });
+    // their device copies, from this thread (the buffers come from the scene's arena)
+    for (auto &op : objs) {
+        Obj *o = op.get();
+        const int k = o->k;
+        const cqbem_params *p = &params[k];
+        const cqbem_outputs *out = &outs[k];
+        upload_mesh(o->d, &meshes[k], o->P, st);
         if (o->mixed) upload(o->dfreq32, freqs_f32(o->P.freqs), st);
         const long long n3 = 3LL * o->P.nv;
         if (o->wantB) {
@@ -740,15 +827,8 @@ void run_scene(cqbem_context *ctx, hipStream_t st, int n, const cqbem_mesh *mesh
             if (out->B_angular) o->Wba = out->B_angular;
             else o->sBa.alloc((size_t)o->P.L * n3 * 9 * sizeof(double)), o->Wba = o->sBa.as<double>();
         }
-        if (o->wantH) {
-            const auto C = meshes[k].vertex_map_inverse
-                               ? c_matrix_collocation_split(meshes[k].V, o->P.nv, meshes[k].F, o->P.nf, o->P.N.data(),
-                                                            meshes[k].vertex_map_inverse, p->nu)
-                               : c_matrix_collocation(meshes[k].V, o->P.nv, meshes[k].F, o->P.nf, o->P.N.data(), p->nu);
-            upload(o->dC, C, st);
-        }
+        if (o->wantH) upload(o->dC, o->C, st);
         fixed += footprint(&meshes[k], p, 1, 0);
-        objs.push_back(std::move(o));
     }
     tm[5] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
     // the spectra budget: what the HBM budget and the free memory leave, at most 48 GiB
@@ -840,7 +920,13 @@ void run_scene(cqbem_context *ctx, hipStream_t st, int n, const cqbem_mesh *mesh
             if (ngh) HIPCHK(launch_scene_gh_pair(dgh, (int)gh.size(), ngh, counter.as<unsigned>(), true, true, ctx->num_cus, st));
             if (nho) HIPCHK(launch_scene_gh_pair(dho, (int)honly.size(), nho, counter.as<unsigned>(), false, true, ctx->num_cus, st));
             if (ngo) HIPCHK(launch_scene_gh_pair(dgo, (int)gonly.size(), ngo, counter.as<unsigned>(), true, false, ctx->num_cus, st));
+            // each slab's float G, singular faces and transforms: one branch per slab (uncaptured), after the shared
+            // launches
+            Branches br(ctx, st, gr.capture);
+            br.fork();
+            size_t bi = 0;
             for (const Slab &sb : slabs) {
+                const hipStream_t st = br.on(bi++);
                 Obj *o = sb.pc.o;
                 const Geom &g = o->d.g;
                 const long long n3 = 3LL * o->P.nv;
@@ -870,6 +956,7 @@ void run_scene(cqbem_context *ctx, hipStream_t st, int n, const cqbem_mesh *mesh
                         HIPCHK(launch_transform(src, E, sb.lstride, Lf, tab, L, dst + row_off, n3 * n3, ctx->num_cus, st));
                 }
             }
+            br.join();
             gr.launch();
         }
         tm[1] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -882,7 +969,11 @@ void run_scene(cqbem_context *ctx, hipStream_t st, int n, const cqbem_mesh *mesh
     {
         const auto t0 = std::chrono::steady_clock::now();
         GraphRun gr(st);
+        Branches br(ctx, st, gr.capture);  // one branch per object
+        br.fork();
+        size_t bi = 0;
         for (auto &op : objs) {
+            const hipStream_t st = br.on(bi++);
             Obj *o = op.get();
             const cqbem_outputs &out = outs[o->k];
             const Geom &g = o->d.g;
@@ -909,11 +1000,14 @@ void run_scene(cqbem_context *ctx, hipStream_t st, int n, const cqbem_mesh *mesh
             HIPCHK(launch_transform(o->specBa.as<cd>(), n3 * 9, n3 * 9, Lf, tab, L, o->Wba, n3 * 9, ctx->num_cus, st));
             HIPCHK(launch_b_post(o->Wbt, o->Wba, out.B, g.V, nv, L, st));
         }
+        br.join();
         gr.launch();
         tm[2] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
+    // the Galerkin objects' scratch: what the spectra region (held until the scene ends) leaves of the budget
+    const uint64_t held = spec.p ? spec.bytes : 0;
     for (int k : galerkin) {
-        const size_t slab = (size_t)std::min<uint64_t>(W, (uint64_t)48 << 30);
+        const size_t slab = (size_t)std::max<uint64_t>(W > held ? W - held : 0, (uint64_t)256 << 20);
         run_job(ctx, st, &meshes[k], &params[k], &outs[k], slab, tm, true);
     }
 }
@@ -968,6 +1062,8 @@ int cqbem_destroy(cqbem_context *ctx) {
         if (kv.second->worker.joinable()) kv.second->worker.join();
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream), (void)hipStreamDestroy(ctx->stream);
     for (hipStream_t s : ctx->slots) (void)hipStreamSynchronize(s), (void)hipStreamDestroy(s);
+    for (hipStream_t s : ctx->side) (void)hipStreamSynchronize(s), (void)hipStreamDestroy(s);
+    for (hipEvent_t e : ctx->side_ev) (void)hipEventDestroy(e);
     delete ctx;
     return CQBEM_OK;
 }
@@ -1308,12 +1404,13 @@ static int submit_impl(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_p
         if (fixed + res > ctx->budget)
             throw Err{CQBEM_ERR_NOMEM, "job footprint (" + std::to_string((fixed + res) >> 20) +
                                            " MiB) exceeds the HBM budget"};
-        // the spectra slab: half of what the budget leaves (all of it up to 256 MiB), at most 8 GiB; one-row slabs
-        // when even that does not fit
+        // the spectra slab: half of what the budget leaves (all of it up to 256 MiB), at most 2 GiB (a slab of that
+        // size already holds ~30k tiles of the pair kernel, and smaller slabs leave room for the other slots' jobs);
+        // one-row slabs when even that does not fit
         const uint64_t used = ctx->reserved + fixed + res;
         const uint64_t left = ctx->budget > used ? ctx->budget - used : 0;
         size_t slab = (size_t)std::min<uint64_t>(std::max<uint64_t>(left / 2, std::min<uint64_t>(left, (uint64_t)256 << 20)),
-                                                 (uint64_t)8 << 30);
+                                                 (uint64_t)2 << 30);
         uint64_t scratch = footprint(mesh, params, 1, slab);
         if (ctx->reserved + scratch + res > ctx->budget) slab = 0, scratch = fixed;
         size_t fr = 0, tot = 0;

@@ -164,7 +164,7 @@ __device__ inline double pair_swap(double v) {
 // mtab are the block's LDS (mtab loaded and visible); the tile zeroes acc itself and ends with its write-out
 // (callers that run another tile on the same LDS synchronise first).
 template <bool DG, bool DH, bool GAL>
-__device__ inline void pair_tile(const Geom &g, const double4 *__restrict__ pts, int nrounds, const Freq *__restrict__ freqs,
+__device__ __attribute__((always_inline)) inline void pair_tile(const Geom &g, const double4 *__restrict__ pts, int nrounds, const Freq *__restrict__ freqs,
                                  int l0, const Mat &m, int row0, const double4 *__restrict__ tpts,
                                  const int *__restrict__ patch_pass, const int2 *__restrict__ pass_info,
                                  const int4 *__restrict__ pass_faces, const int *__restrict__ patch_verts,
@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
 //     persistent selects it, for comparison).
 // ------------------------------------------------------------------------
 template <bool DG, bool DH>
-__device__ inline void scene_tile(const ScenePiece *__restrict__ pieces, int npieces, unsigned t, double *acc, double *fgeo,
+__device__ __attribute__((always_inline)) inline void scene_tile(const ScenePiece *__restrict__ pieces, int npieces, unsigned t, double *acc, double *fgeo,
                                   const MathTables &mtab) {
     int pi = 0;
     while (pi + 1 < npieces && pieces[pi + 1].base <= t) ++pi;
@@ -381,12 +381,13 @@ __device__ inline void scene_tile(const ScenePiece *__restrict__ pieces, int npi
 }
 
 template <bool DG, bool DH>
-__global__ void __launch_bounds__(kPairThreads, 2) k_scene_gh_tiles(const ScenePiece *__restrict__ pieces, int npieces) {
+__global__ void __launch_bounds__(kPairThreads, 2) k_scene_gh_tiles(const ScenePiece *__restrict__ pieces, int npieces,
+                                                                  unsigned t0) {
     __shared__ double acc[kPairWords * kPairPV];
     __shared__ double fgeo[kFaceWords * kPairFaces];
     __shared__ MathTables mtab;
     load_math_tables(&mtab);
-    scene_tile<DG, DH>(pieces, npieces, blockIdx.x, acc, fgeo, mtab);
+    scene_tile<DG, DH>(pieces, npieces, t0 + blockIdx.x, acc, fgeo, mtab);
 }
 
 template <bool DG, bool DH>
@@ -1133,13 +1134,17 @@ hipError_t launch_scene_gh_pair(const ScenePiece *pieces, int npieces, long long
     const char *env = std::getenv("CQBEM_SCENE_KERNEL");  // read per launch (tests switch it)
     const bool persistent = env && !std::strcmp(env, "persistent");
     if (!persistent) {
-        const dim3 grid((unsigned)ntiles);
-        if (G && H)
-            hipLaunchKernelGGL((k_scene_gh_tiles<true, true>), grid, dim3(kPairThreads), 0, st, pieces, npieces);
-        else if (G)
-            hipLaunchKernelGGL((k_scene_gh_tiles<true, false>), grid, dim3(kPairThreads), 0, st, pieces, npieces);
-        else
-            hipLaunchKernelGGL((k_scene_gh_tiles<false, true>), grid, dim3(kPairThreads), 0, st, pieces, npieces);
+        // a dispatch holds fewer than 2^32 work-items: grids of at most 2^22 blocks of 256, in tile order
+        constexpr long long kChunk = 1LL << 22;
+        for (long long t0 = 0; t0 < ntiles; t0 += kChunk) {
+            const dim3 grid((unsigned)std::min(kChunk, ntiles - t0));
+            if (G && H)
+                hipLaunchKernelGGL((k_scene_gh_tiles<true, true>), grid, dim3(kPairThreads), 0, st, pieces, npieces, (unsigned)t0);
+            else if (G)
+                hipLaunchKernelGGL((k_scene_gh_tiles<true, false>), grid, dim3(kPairThreads), 0, st, pieces, npieces, (unsigned)t0);
+            else
+                hipLaunchKernelGGL((k_scene_gh_tiles<false, true>), grid, dim3(kPairThreads), 0, st, pieces, npieces, (unsigned)t0);
+        }
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, counter);

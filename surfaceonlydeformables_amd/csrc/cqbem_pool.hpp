@@ -2,7 +2,8 @@
 // waits for the whole device, so concurrent jobs on other streams would hold each other up at every release.
 // A job takes its buffers from the context's pool through a JobArena and gives them back when its stream has
 // finished; blocks are reused by later jobs of similar sizes and released to HIP only when an allocation
-// fails (then the pool is emptied and the allocation retried) or the context is destroyed.
+// fails (then a larger idle block is taken if there is one, else the pool is emptied and the allocation retried) or
+// the context is destroyed.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -37,6 +38,19 @@ class DevicePool {
         void *p = nullptr;
         if (hipMalloc(&p, n) != hipSuccess) {
             (void)hipGetLastError();
+            {
+                // the device is full: a larger idle block serves before the cache is emptied (trimming and
+                // allocating again costs a device-wide free and the clearing of the new pages)
+                std::lock_guard<std::mutex> lk(mu_);
+                auto it = free_.lower_bound(n);
+                if (it != free_.end()) {
+                    p = it->second;
+                    *got = it->first;
+                    cached_ -= it->first;
+                    free_.erase(it);
+                    return p;
+                }
+            }
             trim();
             if (hipMalloc(&p, n) != hipSuccess) {
                 (void)hipGetLastError();

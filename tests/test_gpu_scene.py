@@ -4,8 +4,9 @@ every collocation object in one grid per wave, then the singular faces and trans
 Every object's outputs must be bit-identical to its own cqbem_assemble: the tiles run the single-object kernel's
 computation unchanged, and row slabs and waves do not change any sum.  Checked for a mixed scene (sizes, L, BDF1 and
 BDF2, a vertex-split cube, a CQBEM_MIXED object, a float-stored object, a Galerkin object that takes the per-object
-path), with both tile dispatch forms (one block per tile, and persistent blocks on a device tile counter), and under
-a small budget that splits the objects into row slabs over several waves."""
+path), with both tile dispatch forms (one block per tile, and persistent blocks on a device tile counter), under a small
+budget that splits the objects into row slabs over several waves, and with the launches run on their streams
+instead of captured (CQBEM_SCENE_GRAPH=0)."""
 import numpy as np
 import pytest
 
@@ -55,10 +56,11 @@ def outputs(torch, nv, p):
     return o
 
 
-@pytest.mark.parametrize("variant", ["tiles", "persistent", "tiles_small_budget"])
+@pytest.mark.parametrize("variant", ["tiles", "persistent", "tiles_small_budget", "uncaptured"])
 def test_scene_matches_solo_assembly(variant, monkeypatch):
     import torch
     monkeypatch.setenv("CQBEM_SCENE_KERNEL", "persistent" if variant == "persistent" else "tiles")
+    monkeypatch.setenv("CQBEM_SCENE_GRAPH", "0" if variant == "uncaptured" else "1")
     objs = scene()
     meshes = [abi.make_mesh(V, F, vmap) for V, F, vmap, p in objs]
     solo_ctx = abi.Context(0)
