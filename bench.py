@@ -571,6 +571,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--quad", choices=["reference", "banded"], default="reference",
+                    help="regular-face rule: the reference's uniform rule, or the distance-banded approximation")
     ap.add_argument("--workload", choices=WORKLOADS, default=WORKLOADS[0],
                     help="single-object workload (default: the icosphere-L4 collocation line)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -606,6 +608,10 @@ def main():
     from surfaceonlydeformables_amd import abi
 
     V, F, params, desc = workload(args.workload, args.precision)
+    if args.quad == "banded":
+        if params.galerkin or args.precision != "fp64":
+            raise SystemExit("--quad banded applies to FP64 collocation workloads")
+        params.quad_policy = abi.QUAD_BANDED
     nv, L = len(V), params.L
     n3 = 3 * nv
     mesh = abi.make_mesh(V, F)
@@ -666,9 +672,21 @@ def main():
             "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32+f64",
             "data": "synthetic",
             "config": {"workload": desc, "nv": nv, "nf": len(F), "L": L, "precision": args.precision,
+                       "quadrature": "reference rule" if args.quad == "reference" else
+                       "distance-banded (far tiles: order 5 x subdivision 1, default kappa)",
                        "entries_per_step": entries},
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
             "phases_ms_last_step": phases, "roofline": roof, "time_step_history": history}
+    if args.quad == "banded":
+        cmp = mixed_against_fp64(ctx, mesh, params, outs, ms_per_step, torch, dev)
+        cmp["speedup_vs_reference_rule"] = cmp.pop("speedup_vs_fp64")
+        cmp["reference_rule_ms_per_step"] = cmp.pop("fp64_ms_per_step")
+        cmp["note"] = ("banded rule against the reference rule, both FP64; its accuracy against a refined rule is in "
+                       "surfaceonlydeformables_amd/band_tables/")
+        cmp.pop("target")
+        line["banded_vs_reference_rule"] = cmp
+        if roof:
+            roof["note"] = "banded: evaluations counted per tile's rule"
     if args.precision == "mixed":
         line["mixed_vs_fp64"] = mixed_against_fp64(ctx, mesh, params, outs, ms_per_step, torch, dev)
         if roof:
@@ -681,8 +699,9 @@ def main():
 
 
 def mixed_against_fp64(ctx, mesh, params, outs, mixed_ms, torch, dev, steps=3):
-    """The same job in FP64 (a few untimed-for-value steps, timed for the ratio) and the relative Frobenius error
-    of every mixed output against it, per matrix over the whole stack and per weight n (norms on the device)."""
+    """The same job in FP64 with the reference rule (a few steps, timed for the ratio) and the relative Frobenius
+    error of every output of the mixed (or banded) job against it, per matrix over the whole stack and per weight n
+    (norms on the device)."""
     from surfaceonlydeformables_amd import abi
     p64 = abi.make_params(params.c1, params.c2, params.rho, params.nu, params.dt, params.R, params.L, bdf=params.bdf,
                           order=params.order, subdivision=params.subdivision, galerkin=params.galerkin,

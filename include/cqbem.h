@@ -52,6 +52,15 @@ enum { CQBEM_FP64 = 0, CQBEM_MIXED = 1 };
  * jobs whose double weights exceed the memory they may hold resident (an icosphere-L5 job at L = 16 needs 242 GB
  * in double, 121 GB in float). */
 enum { CQBEM_STORE_F64 = 0, CQBEM_STORE_F32 = 1 };
+/* Regular-face quadrature of the collocation G and H (no counterpart in the reference, which applies one uniform
+ * rule to every non-touching pair: integrator_collocation.hpp:6-17).  CQBEM_QUAD_REFERENCE (default): that rule
+ * everywhere.  CQBEM_QUAD_BANDED: per tile (column patch, row point), the rule (far_order, far_subdivision)
+ * replaces it when the row point lies at least band_kappa x h beyond the patch's bounding sphere (h: the patch's
+ * largest face diameter), at every frequency alike (a rule that changed along the spectrum would make each entry's
+ * quadrature error jump between frequencies, which the inverse transform's R^-n factor amplifies).  An
+ * approximation with a measured error, not bit-compatible with the reference.  Zeros select the tuned defaults
+ * (far rule: order, subdivision 1). */
+enum { CQBEM_QUAD_REFERENCE = 0, CQBEM_QUAD_BANDED = 1 };
 enum { CQBEM_G = 1, CQBEM_H = 2, CQBEM_B_TRANS = 4, CQBEM_B_ANGULAR = 8, CQBEM_ALL = 15 };
 
 typedef struct cqbem_context cqbem_context;
@@ -79,6 +88,11 @@ typedef struct {
     int32_t precision;    /* CQBEM_FP64 | CQBEM_MIXED */
     int32_t which;        /* bitmask of CQBEM_G | CQBEM_H | CQBEM_B_TRANS | CQBEM_B_ANGULAR */
     int32_t storage;      /* CQBEM_STORE_F64 | CQBEM_STORE_F32: element type of the G and H outputs */
+    int32_t quad_policy;  /* CQBEM_QUAD_REFERENCE | CQBEM_QUAD_BANDED (collocation, FP64) */
+    int32_t far_order;    /* banded: far rule order (0: order) */
+    int32_t far_subdivision; /* banded: far rule subdivision (0: 1) */
+    int32_t reserved;     /* 0 */
+    double band_kappa;    /* banded: distance threshold in face diameters (0: tuned default) */
 } cqbem_params;
 
 /* Output buffers; any may be NULL.  on_device != 0: device pointers on the context's GPU;
@@ -127,6 +141,13 @@ int cqbem_assemble_scene(cqbem_context *ctx, int32_t n, const cqbem_mesh *meshes
 
 /* Device bytes cqbem_assemble needs for a job (outputs on device included when outputs_on_device != 0). */
 int cqbem_job_footprint(const cqbem_mesh *mesh, const cqbem_params *params, int outputs_on_device, uint64_t *bytes);
+
+/* The tile plan of CQBEM_QUAD_BANDED (host only, no context): *npatches receives the column patch count;
+ * patch_of_vertex (nv, may be NULL) the patch holding each vertex's column; far (nv x npatches, row-major, may be
+ * NULL) whether tile (row vertex, patch) takes the far rule, at every frequency.  Entry (row i, column j) of G and
+ * H is integrated entirely in tile (i, patch_of_vertex[j]).  No counterpart in the reference: for tests and tuning. */
+int cqbem_band_plan(const cqbem_mesh *mesh, const cqbem_params *params, int32_t *patch_of_vertex, uint8_t *far,
+                    int32_t *npatches);
 
 /* Laplace-domain matrices of one CQ frequency index l (0..L/2), complex interleaved, row-major:
  * U, P (3nv x 3nv), Bt (3nv x 3), Ba (3nv x 9).  Host pointers; any may be NULL.  Used for parity checks

@@ -1116,20 +1116,17 @@ void laplace_galerkin(const Mesh &m, const orc_params &p, cx<double> s, const Wh
     GalKernels<T> KT{w, KParam<T>{(T)p.c1, (T)p.c2, (T)p.rho, cx<T>((T)s.real(), (T)s.imag())}};
     GalKernels<S> KS{w, KParam<S>{(S)p.c1, (S)p.c2, (S)p.rho, cx<S>((S)s.real(), (S)s.imag())}};
     out.zero();
-    // One test face f1 at a time: its source faces are split over the threads, each summing into a row block
-    // of its own (the 9 rows of f1's corners); the blocks are then added into the output rows in thread order.
-    // The reference serializes the same scatter with omp critical (G_galerkin_non_singular.cu:84).
-    const int nth =
-#ifdef _OPENMP
-        omp_get_max_threads();
-#else
-        1;
-#endif
+    // One test face f1 at a time: its source faces are split into kChunks cyclic chunks (f2 = ch, ch + kChunks,
+    // ...), each summed in face order into a row block of its own (the 9 rows of f1's corners) by whichever thread
+    // takes the chunk; the blocks are then added into the output rows in chunk order.  The sums therefore do not
+    // depend on the number of threads.  The reference serializes the same scatter with omp critical
+    // (G_galerkin_non_singular.cu:84).
+    constexpr int kChunks = 16;
     const int n3 = out.n3;
     struct RowBlock {
         std::vector<cx<A>> U, P, Bt, Ba;  // 9 x n3, 9 x n3, 9 x 3, 9 x 9 (row 3c + a for corner c of f1)
     };
-    std::vector<RowBlock> blk(nth);
+    std::vector<RowBlock> blk(kChunks);
     for (auto &b : blk) {
         if (w.G) b.U.resize(9 * (size_t)n3);
         if (w.H) b.P.resize(9 * (size_t)n3);
@@ -1140,15 +1137,9 @@ void laplace_galerkin(const Mesh &m, const orc_params &p, cx<double> s, const Wh
         bool in_range = false;
         for (int c = 0; c < 3; c++) in_range |= m.F[3 * f1 + c] >= row0 && m.F[3 * f1 + c] < row1;
         if (!in_range) continue;
-#pragma omp parallel
-        {
-            int tid =
-#ifdef _OPENMP
-                omp_get_thread_num();
-#else
-                0;
-#endif
-            RowBlock &mine = blk[tid];
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int ch = 0; ch < kChunks; ch++) {
+            RowBlock &mine = blk[ch];
             for (auto *v : {&mine.U, &mine.P, &mine.Bt, &mine.Ba}) std::fill(v->begin(), v->end(), cx<A>(0));
             PairAcc<T> accT;
             PairAcc<S> accS;
@@ -1184,8 +1175,7 @@ void laplace_galerkin(const Mesh &m, const orc_params &p, cx<double> s, const Wh
                     }
                 }
             };
-#pragma omp for schedule(static, 1)  // cyclic: the expensive touching pairs spread over the threads
-            for (int f2 = 0; f2 < m.nf; f2++) {
+            for (int f2 = ch; f2 < m.nf; f2 += kChunks) {  // cyclic: the expensive touching pairs spread out
                 if (galerkin_nonsingular_pair<T>(m, f1, f2, KT, q, ref, accT)) {
                     add(accT, m.F + 3 * f1, m.F + 3 * f2);
                 } else {

@@ -18,6 +18,7 @@ ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_BUSY, ERR_NOTFOUND = -1, -2, -
 BDF1, BDF2 = 1, 2
 FP64, MIXED = 0, 1
 STORE_F64, STORE_F32 = 0, 1
+QUAD_REFERENCE, QUAD_BANDED = 0, 1
 G, H, B_TRANS, B_ANGULAR, ALL = 1, 2, 4, 8, 15
 
 EXPORTED = [
@@ -25,7 +26,7 @@ EXPORTED = [
     "cqbem_laplace", "cqbem_frequencies", "cqbem_last_timings", "cqbem_submit", "cqbem_poll",
     "cqbem_wait", "cqbem_reserved_bytes", "cqbem_history", "cqbem_matvec", "cqbem_split_traction_discontinuity",
     "cqbem_wavelet_compress", "cqbem_csr_matvec", "cqbem_assemble_noalloc", "cqbem_submit_resident", "cqbem_release",
-    "cqbem_history_f32", "cqbem_assemble_scene",
+    "cqbem_history_f32", "cqbem_assemble_scene", "cqbem_band_plan",
 ]
 
 
@@ -38,7 +39,9 @@ class cqbem_params(C.Structure):
     _fields_ = [("c1", C.c_double), ("c2", C.c_double), ("rho", C.c_double), ("nu", C.c_double),
                 ("dt", C.c_double), ("R", C.c_double), ("L", C.c_int32), ("bdf", C.c_int32),
                 ("order", C.c_int32), ("subdivision", C.c_int32), ("galerkin", C.c_int32),
-                ("precision", C.c_int32), ("which", C.c_int32), ("storage", C.c_int32)]
+                ("precision", C.c_int32), ("which", C.c_int32), ("storage", C.c_int32),
+                ("quad_policy", C.c_int32), ("far_order", C.c_int32), ("far_subdivision", C.c_int32),
+                ("reserved", C.c_int32), ("band_kappa", C.c_double)]
 
 
 class cqbem_outputs(C.Structure):
@@ -77,6 +80,7 @@ def lib():
         L.cqbem_assemble_noalloc.argtypes = [C.c_void_p, P(cqbem_mesh), P(cqbem_params), P(cqbem_outputs)]
         L.cqbem_assemble_scene.argtypes = [C.c_void_p, C.c_int32, P(cqbem_mesh), P(cqbem_params), P(cqbem_outputs)]
         L.cqbem_job_footprint.argtypes = [P(cqbem_mesh), P(cqbem_params), C.c_int, P(C.c_uint64)]
+        L.cqbem_band_plan.argtypes = [P(cqbem_mesh), P(cqbem_params), C.c_void_p, C.c_void_p, P(C.c_int32)]
         L.cqbem_laplace.argtypes = [C.c_void_p, P(cqbem_mesh), P(cqbem_params), C.c_int32, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p]
         L.cqbem_frequencies.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_int32, P(C.c_double)]
@@ -114,8 +118,9 @@ def make_mesh(V, F, vmap=None):
 
 
 def make_params(c1, c2, rho, nu, dt, R, L, bdf=BDF2, order=5, subdivision=2, galerkin=False, precision=FP64,
-                which=ALL, storage=STORE_F64):
-    return cqbem_params(c1, c2, rho, nu, dt, R, L, bdf, order, subdivision, int(galerkin), precision, which, storage)
+                which=ALL, storage=STORE_F64, quad=QUAD_REFERENCE, far_order=0, far_subdivision=0, band_kappa=0.0):
+    return cqbem_params(c1, c2, rho, nu, dt, R, L, bdf, order, subdivision, int(galerkin), precision, which, storage,
+                        quad, far_order, far_subdivision, 0, band_kappa)
 
 
 def split_traction_discontinuity(V, F, threshold_deg=20.0):
@@ -304,6 +309,20 @@ def wavelet_padded(n: int) -> int:
     while np_ < n:
         np_ <<= 1
     return np_
+
+
+def band_plan(mesh, params):
+    """CQBEM_QUAD_BANDED tile plan: (patch_of_vertex (nv,), far (nv, npatches) bool), the same at every frequency."""
+    n = C.c_int32()
+    rc = lib().cqbem_band_plan(C.byref(mesh), C.byref(params), None, None, C.byref(n))
+    if rc != CQBEM_OK:
+        raise CQBEMError(rc, "band plan refused")
+    pov = np.zeros(mesh.nv, np.int32)
+    far = np.zeros((mesh.nv, n.value), np.uint8)
+    rc = lib().cqbem_band_plan(C.byref(mesh), C.byref(params), pov.ctypes.data, far.ctypes.data, C.byref(n))
+    if rc != CQBEM_OK:
+        raise CQBEMError(rc, "band plan refused")
+    return pov, far.astype(bool)
 
 
 def job_footprint(mesh, params, outputs_on_device=True) -> int:

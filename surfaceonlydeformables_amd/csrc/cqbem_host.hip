@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <cmath>
 #include <deque>
 #include <exception>
 #include <stdexcept>
@@ -109,6 +110,15 @@ void check_inputs(const cqbem_mesh *mesh, const cqbem_params *p) {
         throw Err{CQBEM_ERR_UNSUPPORTED, "CQBEM_MIXED is implemented for collocation only"};
     if (p->storage != CQBEM_STORE_F64 && p->storage != CQBEM_STORE_F32) throw Err{CQBEM_ERR_ARG, "unknown storage"};
     if ((p->which & ~CQBEM_ALL) != 0) throw Err{CQBEM_ERR_ARG, "unknown matrix bits in `which`"};
+    if (p->quad_policy != CQBEM_QUAD_REFERENCE && p->quad_policy != CQBEM_QUAD_BANDED)
+        throw Err{CQBEM_ERR_ARG, "unknown quad_policy"};
+    if (p->quad_policy == CQBEM_QUAD_BANDED) {
+        if (p->galerkin || p->precision != CQBEM_FP64)
+            throw Err{CQBEM_ERR_UNSUPPORTED, "CQBEM_QUAD_BANDED is implemented for FP64 collocation"};
+        if (p->far_order < 0 || p->far_order > 5 || p->far_subdivision < 0 || p->far_subdivision > 16 ||
+            !(p->band_kappa >= 0))
+            throw Err{CQBEM_ERR_ARG, "far rule or band threshold out of range"};
+    }
     // the device sincos reduces |Im(s) r / c| <= max|s_l| diam / min(c1, c2) exactly below kTrigArgMax
     double lo[3] = {mesh->V[0], mesh->V[1], mesh->V[2]}, hi[3] = {lo[0], lo[1], lo[2]};
     for (int64_t k = 0; k < mesh->nv; k++)
@@ -180,11 +190,23 @@ struct Prepared {
     int npts = 0;               // points of the composite triangle rule
     std::vector<double4> quad;  // the rule, padded to an even count with a zero-weight point
     std::vector<double4> pair_pts;  // per padded point: (b0, b1, b2, w), (phi_F own, phi_F partner, 0, 0)
+    // CQBEM_QUAD_BANDED: the far rule's pair point table and, per patch, its bounding sphere's centre and the far
+    // distance (radius + kappa h)
+    std::vector<double4> far_pair_pts;
+    int far_npts = 0;  // points of the far rule (unpadded)
+    std::vector<double4> patch_sphere;
+    std::vector<double> patch_weight;  // per patch: sum over its faces of 1 / (patches listing the face)
     std::vector<double2> line;
     std::vector<Freq> freqs;
     std::vector<double2> tab;
     Mat mat;
 };
+
+// CQBEM_QUAD_BANDED default: far when the row point is kBandKappa largest face diameters beyond the patch's bounding
+// sphere.  Tuned on icosphere level 4 at L = 32 (surfaceonlydeformables_amd/tune_bands.py, table in
+// surfaceonlydeformables_amd/band_tables/): the smallest kappa whose added error stays within 5% of the reference
+// rule's largest per-weight error against a rule of twice the subdivision (3.0% for G, 0.6% for H).
+constexpr double kBandKappa = 6.0;
 
 Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
     Prepared P;
@@ -339,6 +361,34 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
             }
             P.pair_patch_pass.push_back((int)P.pair_pass_info.size());
         }
+        if (p->quad_policy == CQBEM_QUAD_BANDED) {
+            // per patch: bounding sphere of its faces (centre: mean of their corners) and largest face diameter
+            const double kappa = p->band_kappa > 0 ? p->band_kappa : kBandKappa;
+            for (int pc = 0; pc < P.pair_npatches; pc++) {
+                double c[3] = {0, 0, 0};
+                for (int f : lists[pc])
+                    for (int k = 0; k < 3; k++)
+                        for (int d = 0; d < 3; d++) c[d] += V[3 * F[3 * f + k] + d];
+                const double inv = 1.0 / (3.0 * (double)lists[pc].size());
+                for (double &x : c) x *= inv;
+                double rad = 0, h = 0;
+                for (int f : lists[pc])
+                    for (int k = 0; k < 3; k++) {
+                        const V3 a = vtx(V, F[3 * f + k]), b = vtx(V, F[3 * f + (k + 1) % 3]);
+                        rad = std::max(rad, nrm(sub(a, V3{c[0], c[1], c[2]})));
+                        h = std::max(h, nrm(sub(a, b)));
+                    }
+                P.patch_sphere.push_back(make_double4(c[0], c[1], c[2], rad + kappa * h));
+            }
+            std::vector<int> mult(P.nf, 0);
+            for (const auto &l : lists)
+                for (int f : l) mult[f]++;
+            for (const auto &l : lists) {
+                double w = 0;
+                for (int f : l) w += 1.0 / mult[f];
+                P.patch_weight.push_back(w);
+            }
+        }
     }
     for (const TriPoint &t : composite_tri_rule(p->order, p->subdivision)) P.quad.push_back(make_double4(t.b[0], t.b[1], t.b[2], t.w));
     P.npts = (int)P.quad.size();
@@ -348,6 +398,19 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
         const double4 &Q = P.quad[q], &Qp = P.quad[q ^ 1];
         P.pair_pts.push_back(Q);
         P.pair_pts.push_back((q & 1) ? make_double4(Q.z, Qp.z, 0.0, 0.0) : make_double4(Q.x, Qp.x, 0.0, 0.0));
+    }
+    if (p->quad_policy == CQBEM_QUAD_BANDED) {
+        std::vector<double4> fq;
+        for (const TriPoint &t : composite_tri_rule(p->far_order > 0 ? p->far_order : p->order,
+                                                    p->far_subdivision > 0 ? p->far_subdivision : 1))
+            fq.push_back(make_double4(t.b[0], t.b[1], t.b[2], t.w));
+        P.far_npts = (int)fq.size();
+        if (fq.size() & 1) fq.push_back(make_double4(1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 0.0));
+        for (size_t q = 0; q < fq.size(); q++) {
+            const double4 &Q = fq[q], &Qp = fq[q ^ 1];
+            P.far_pair_pts.push_back(Q);
+            P.far_pair_pts.push_back((q & 1) ? make_double4(Q.z, Qp.z, 0.0, 0.0) : make_double4(Q.x, Qp.x, 0.0, 0.0));
+        }
     }
     for (auto &xw : composite_line01(p->order, p->subdivision)) P.line.push_back(make_double2(xw.first, xw.second));
     const auto s = cq_frequencies(p->L, p->R, p->dt, p->bdf);
@@ -371,6 +434,7 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
 struct DevMesh {
     DBuf V, F, N, area, vs_ptr, vs_face, sg_ptr, sg_face, vmap, quad, line, freqs, tab;
     DBuf pair_patch_pass, pair_pass_info, pair_pass_faces, pair_patch_verts, pair_pts;
+    DBuf far_pts, patch_sphere;  // CQBEM_QUAD_BANDED
     Geom g;
 };
 
@@ -382,13 +446,42 @@ bool colloc_star() {
     return e && !std::strcmp(e, "star");
 }
 
+// The band descriptor of a prepared job (empty unless CQBEM_QUAD_BANDED).
+Bands bands_of(const DevMesh &d, const Prepared &P) {
+    Bands b;
+    if (P.far_pair_pts.empty()) return b;
+    b.far_pts = d.far_pts.as<double4>(), b.far_rounds = (int)P.far_pair_pts.size() / 4;
+    b.sphere = d.patch_sphere.as<double4>();
+    return b;
+}
+
+// Host replica of the pair kernel's per-tile rule choice (CQBEM_QUAD_BANDED): whether tile (patch pc, row point x)
+// takes the far rule.
+bool tile_is_far(const Prepared &P, int pc, const double *x) {
+    if (P.far_pair_pts.empty()) return false;
+    const double4 S = P.patch_sphere[pc];
+    const double dx = x[0] - S.x, dy = x[1] - S.y, dz = x[2] - S.z;
+    return dx * dx + dy * dy + dz * dz >= S.w * S.w;
+}
+
+// Regular-face evaluations of row vertex i over all frequencies (each face and point once, whichever patches list
+// the face): (nf - singular faces) x points x Lf for the reference rule; per tile's rule when banded (the singular
+// faces of a row lie in near tiles: the patch listing them contains the row vertex).
+double regular_evals(const Prepared &P, const double *V, int i) {
+    const int nsing = P.sg_ptr[i + 1] - P.sg_ptr[i];
+    if (P.far_pair_pts.empty()) return (double)(P.nf - nsing) * P.npts * P.Lf;
+    double e = -(double)nsing * P.npts;
+    for (int pc = 0; pc < P.pair_npatches; pc++) e += P.patch_weight[pc] * (tile_is_far(P, pc, V + 3 * i) ? P.far_npts : P.npts);
+    return e * P.Lf;
+}
+
 hipError_t launch_gh(const DevMesh &d, const Prepared &P, const Freq *freqs, int l0, int nl, int row0, int nrows, cd *UG,
                      cd *UH, long long ld, long long lstride, hipStream_t st) {
-    if (colloc_star())
+    if (colloc_star() && P.far_pair_pts.empty())
         return launch_colloc_gh(d.g, d.quad.as<double4>(), P.npts, freqs, l0, nl, P.mat, row0, nrows, UG, UH, ld, lstride, st);
     return launch_colloc_gh_pair(d.g, d.pair_pts.as<double4>(), (int)P.quad.size(), freqs, l0, nl, P.mat, row0, nrows,
                                  nullptr, d.pair_patch_pass.as<int>(), d.pair_pass_info.as<int2>(), d.pair_pass_faces.as<int4>(),
-                                 d.pair_patch_verts.as<int>(), P.pair_npatches, UG, UH, ld, lstride, st);
+                                 d.pair_patch_verts.as<int>(), P.pair_npatches, UG, UH, ld, lstride, st, bands_of(d, P));
 }
 
 void upload_mesh(DevMesh &d, const cqbem_mesh *mesh, const Prepared &P, hipStream_t st) {
@@ -406,6 +499,8 @@ void upload_mesh(DevMesh &d, const cqbem_mesh *mesh, const Prepared &P, hipStrea
     upload(d.pair_patch_pass, P.pair_patch_pass, st), upload(d.pair_pass_info, P.pair_pass_info, st);
     upload(d.pair_pass_faces, P.pair_pass_faces, st), upload(d.pair_patch_verts, P.pair_patch_verts, st);
     upload(d.pair_pts, P.pair_pts, st);
+    if (!P.far_pair_pts.empty())
+        upload(d.far_pts, P.far_pair_pts, st), upload(d.patch_sphere, P.patch_sphere, st);
     Geom &g = d.g;
     g.V = d.V.as<double>(), g.F = d.F.as<int>(), g.N = d.N.as<double>(), g.area = d.area.as<double>();
     g.vs_ptr = d.vs_ptr.as<int>(), g.vs_face = d.vs_face.as<int>(), g.sg_ptr = d.sg_ptr.as<int>(), g.sg_face = d.sg_face.as<int>();
@@ -574,7 +669,7 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
                     }
                 }
                 tm[6] += 1;
-                for (int i = r0; i < r0 + nr; i++) tm[7] += (double)(P.nf - (P.sg_ptr[i + 1] - P.sg_ptr[i])) * npts * Lf;
+                for (int i = r0; i < r0 + nr; i++) tm[7] += regular_evals(P, mesh->V, i);
                 {
                     Timer t(st, &tm[3]);
                     if (wantG)
@@ -894,7 +989,7 @@ void run_scene(cqbem_context *ctx, hipStream_t st, int n, const cqbem_mesh *mesh
             ScenePiece sp{o->d.g, o->d.pair_pts.as<double4>(), (int)o->P.quad.size() / 2, o->d.freqs.as<Freq>(), o->P.mat,
                           pc.row0, pc.nrows, o->P.Lf, o->P.pair_npatches, o->d.pair_patch_pass.as<int>(),
                           o->d.pair_pass_info.as<int2>(), o->d.pair_pass_faces.as<int4>(), o->d.pair_patch_verts.as<int>(),
-                          g_in_pair ? sG : nullptr, o->wantH ? sH : nullptr, n3, lstride, 0u};
+                          g_in_pair ? sG : nullptr, o->wantH ? sH : nullptr, n3, lstride, 0u, bands_of(o->d, o->P)};
             (g_in_pair ? (o->wantH ? gh : gonly) : honly).push_back(sp);
         }
         auto number = [](std::vector<ScenePiece> &v) {
@@ -963,7 +1058,7 @@ void run_scene(cqbem_context *ctx, hipStream_t st, int n, const cqbem_mesh *mesh
         tm[6] += 1;
         for (const Piece &pc : wave)
             for (int i = pc.row0; i < pc.row0 + pc.nrows; i++)
-                tm[7] += (double)(pc.o->P.nf - (pc.o->P.sg_ptr[i + 1] - pc.o->P.sg_ptr[i])) * pc.o->P.npts * pc.o->P.Lf;
+                tm[7] += regular_evals(pc.o->P, meshes[pc.o->k].V, i);
     }
     // B, the jump term and the B post-processing of every object, in one graph
     {
@@ -1171,6 +1266,33 @@ int cqbem_csr_matvec(cqbem_context *ctx, int64_t rows, const int64_t *rowptr, co
     } catch (const Err &e) {
         ctx->last_error = e.msg;
         return e.code;
+    }
+    return CQBEM_OK;
+}
+
+int cqbem_band_plan(const cqbem_mesh *mesh, const cqbem_params *params, int32_t *patch_of_vertex, uint8_t *far,
+                    int32_t *npatches) {
+    try {
+        if (!mesh || !params || !npatches) throw Err{CQBEM_ERR_ARG, "null argument"};
+        check_inputs(mesh, params);
+        if (params->quad_policy != CQBEM_QUAD_BANDED) throw Err{CQBEM_ERR_ARG, "quad_policy is not CQBEM_QUAD_BANDED"};
+        const Prepared P = prepare(mesh, params);
+        *npatches = P.pair_npatches;
+        const int PV = pair_patch_size();
+        if (patch_of_vertex)
+            for (int pc = 0; pc < P.pair_npatches; pc++)
+                for (int sl = 0; sl < PV; sl++) {
+                    const int v = P.pair_patch_verts[(size_t)pc * PV + sl];
+                    if (v >= 0) patch_of_vertex[v] = pc;
+                }
+        if (far)
+            for (int i = 0; i < P.nv; i++)
+                for (int pc = 0; pc < P.pair_npatches; pc++)
+                    far[(size_t)i * P.pair_npatches + pc] = tile_is_far(P, pc, mesh->V + 3 * i) ? 1 : 0;
+    } catch (const Err &e) {
+        return e.code;
+    } catch (const std::exception &) {
+        return CQBEM_ERR_ARG;
     }
     return CQBEM_OK;
 }

@@ -164,7 +164,8 @@ __device__ inline double pair_swap(double v) {
 // mtab are the block's LDS (mtab loaded and visible); the tile zeroes acc itself and ends with its write-out
 // (callers that run another tile on the same LDS synchronise first).
 template <bool DG, bool DH, bool GAL>
-__device__ __attribute__((always_inline)) inline void pair_tile(const Geom &g, const double4 *__restrict__ pts, int nrounds, const Freq *__restrict__ freqs,
+__device__ __attribute__((always_inline)) inline void pair_tile(const Geom &g, const double4 *__restrict__ pts, int nrounds,
+                                 const Bands &bands, const Freq *__restrict__ freqs,
                                  int l0, const Mat &m, int row0, const double4 *__restrict__ tpts,
                                  const int *__restrict__ patch_pass, const int2 *__restrict__ pass_info,
                                  const int4 *__restrict__ pass_faces, const int *__restrict__ patch_verts,
@@ -183,6 +184,12 @@ __device__ __attribute__((always_inline)) inline void pair_tile(const Geom &g, c
         t[0] = g.F[3 * f1], t[1] = g.F[3 * f1 + 1], t[2] = g.F[3 * f1 + 2];
     } else {
         x0 = g.V[3 * i], x1 = g.V[3 * i + 1], x2 = g.V[3 * i + 2];
+    }
+    if (bands.far_pts) {  // the tile's rule: block-uniform (patch and row are)
+        const double4 S = bands.sphere[pc];
+        const double dx = x0 - S.x, dy = x1 - S.y, dz = x2 - S.z;
+        if (__builtin_amdgcn_readfirstlane((int)(dx * dx + dy * dy + dz * dz >= S.w * S.w)))
+            pts = bands.far_pts, nrounds = bands.far_rounds;
     }
     for (int k = tid; k < kPairWords * kPairPV; k += kPairThreads) acc[k] = 0.0;
     __syncthreads();
@@ -347,13 +354,14 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
                                                                   const int2 *__restrict__ pass_info,
                                                                   const int4 *__restrict__ pass_faces,
                                                                   const int *__restrict__ patch_verts, cd *__restrict__ UG,
-                                                                  cd *__restrict__ UH, long long ld, long long lstride) {
+                                                                  cd *__restrict__ UH, long long ld, long long lstride,
+                                                                  Bands bands) {
     __shared__ double acc[kPairWords * kPairPV];      // [word][slot]
     __shared__ double fgeo[kFaceWords * kPairFaces];  // [word][face of pass]: re-read every round, not held in VGPRs
     __shared__ MathTables mtab;                       // tables of the exponentials
     load_math_tables(&mtab);
-    pair_tile<DG, DH, GAL>(g, pts, nrounds, freqs, l0, m, row0, tpts, patch_pass, pass_info, pass_faces, patch_verts, UG, UH,
-                           ld, lstride, blockIdx.x, blockIdx.y, blockIdx.z, acc, fgeo, mtab);
+    pair_tile<DG, DH, GAL>(g, pts, nrounds, bands, freqs, l0, m, row0, tpts, patch_pass, pass_info, pass_faces, patch_verts,
+                           UG, UH, ld, lstride, blockIdx.x, blockIdx.y, blockIdx.z, acc, fgeo, mtab);
 }
 
 // ------------------------------------------------------------------------
@@ -375,7 +383,7 @@ __device__ __attribute__((always_inline)) inline void scene_tile(const ScenePiec
     const ScenePiece *P = pieces + pi;
     const unsigned local = t - P->base, np = (unsigned)P->npatches, nl = (unsigned)P->nl;
     const unsigned rest = local / np;
-    pair_tile<DG, DH, false>(P->g, P->pts, P->nrounds, P->freqs, 0, P->m, P->row0, nullptr, P->patch_pass, P->pass_info,
+    pair_tile<DG, DH, false>(P->g, P->pts, P->nrounds, P->bands, P->freqs, 0, P->m, P->row0, nullptr, P->patch_pass, P->pass_info,
                              P->pass_faces, P->patch_verts, P->UG, P->UH, P->ld, P->lstride, (int)(local - rest * np),
                              (int)(rest % nl), (int)(rest / nl), acc, fgeo, mtab);
 }
@@ -1110,12 +1118,13 @@ hipError_t launch_colloc_gh(const Geom &g, const double4 *quad, int npts, const 
 hipError_t launch_colloc_gh_pair(const Geom &g, const double4 *pts, int npts_even, const Freq *freqs, int l0, int nl, const Mat &m,
                                  int row0, int nrows, const double4 *tpts, const int *patch_pass, const int2 *pass_info,
                                  const int4 *pass_faces, const int *patch_verts, int npatches, cd *UG, cd *UH, long long ld,
-                                 long long lstride, hipStream_t st) {
+                                 long long lstride, hipStream_t st, const Bands &bands) {
     if ((!UG && !UH) || (npts_even & 1) || nrows <= 0 || nrows > 65535) return hipErrorInvalidValue;
+    if (bands.far_pts && (tpts || bands.far_rounds <= 0 || !bands.sphere)) return hipErrorInvalidValue;
     dim3 grid(npatches, nl, nrows);
 #define CQBEM_PAIR(DG, DH, GAL)                                                                                           \
     hipLaunchKernelGGL((k_colloc_gh_pair<DG, DH, GAL>), grid, dim3(kPairThreads), 0, st, g, pts, npts_even / 2, freqs, l0, m, \
-                       row0, tpts, patch_pass, pass_info, pass_faces, patch_verts, UG, UH, ld, lstride)
+                       row0, tpts, patch_pass, pass_info, pass_faces, patch_verts, UG, UH, ld, lstride, bands)
     if (tpts) {
         if (UG && UH) CQBEM_PAIR(true, true, true);
         else if (UG) CQBEM_PAIR(true, false, true);

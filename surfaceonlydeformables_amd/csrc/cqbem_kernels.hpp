@@ -22,6 +22,15 @@ struct Geom {
 
 hipError_t launch_colloc_gh(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
                             int row0, int nrows, cd *UG, cd *UH, long long ld, long long lstride, hipStream_t st);
+
+// Distance bands of the pair kernel's regular-face rule (CQBEM_QUAD_BANDED).  A tile (patch, row point x) takes the
+// far rule at every frequency when |x - centre| >= radius + kappa h (h: the patch's largest face diameter);
+// otherwise the reference rule.  far_pts == nullptr: the reference rule for every tile.
+struct Bands {
+    const double4 *far_pts = nullptr;  // pair point table of the far rule (layout of the reference rule's)
+    int far_rounds = 0;                // its padded points / 2
+    const double4 *sphere = nullptr;   // per patch: centre, radius + kappa h
+};
 // Pair form of the G/H regular-face kernel (both matrices): per patch a range of passes
 // [patch_pass[p], patch_pass[p+1]); per pass (first face entry, nfaces | nsub << 16); per face entry
 // (face, corner 0, corner 1, corner 2) with corner = slot | sub-step << 10 or -1 outside the patch.
@@ -33,7 +42,7 @@ hipError_t launch_colloc_gh(const Geom &g, const double4 *quad, int npts, const 
 hipError_t launch_colloc_gh_pair(const Geom &g, const double4 *pts, int npts_even, const Freq *freqs, int l0, int nl, const Mat &m,
                                  int row0, int nrows, const double4 *tpts, const int *patch_pass, const int2 *pass_info,
                                  const int4 *pass_faces, const int *patch_verts, int npatches, cd *UG, cd *UH, long long ld,
-                                 long long lstride, hipStream_t st);
+                                 long long lstride, hipStream_t st, const Bands &bands = Bands{});
 int pair_patch_size();
 int pair_pass_faces();
 
@@ -53,6 +62,7 @@ struct ScenePiece {
     cd *UG, *UH;
     long long ld, lstride;
     unsigned base;
+    Bands bands;
 };
 // The pieces (device array, ascending base, all with the same G / H selection) in one persistent launch of
 // min(ntiles, 2 blocks per CU) blocks that take tiles from *counter (zeroed by the launcher with a kernel).

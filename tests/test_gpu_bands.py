@@ -1,0 +1,104 @@
+"""CQBEM_QUAD_BANDED: the distance-banded regular-face rule of the collocation G/H pair kernel.
+
+- With an infinite distance threshold no tile is far: bit-identical to the reference rule.
+- At three frequencies, every 3x3 block (row i, column j) of U and P is integrated in tile (i, patch of j): it must
+  equal, bit for bit, the reference-rule run's block where the host plan (cqbem_band_plan) says near, and the run
+  whose every tile outside its patch's sphere is far where the plan says far.  B is not banded: identical.
+- Accuracy against a refined rule (the reference order at 2x the subdivision), icosphere level 4, L = 32, the
+  default threshold: per time-domain weight the banded G and H add at most 5% of the reference rule's largest
+  per-weight error to the reference rule's own error (printed with the evaluation fraction)."""
+import numpy as np
+import pytest
+
+from surfaceonlydeformables_amd import abi
+from surfaceonlydeformables_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+RHO = 1000.0
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = abi.Context(0)
+    yield c
+    c.close()
+
+
+def mesh3():
+    V, F = M.icosphere(3, 0.2)
+    return M.center(M.jitter(V, F, 0.01, 0), F), F
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def params(L=32, **kw):
+    _, c1, c2 = M.material_speeds(5e5, 0.3, RHO)
+    return abi.make_params(c1, c2, RHO, 0.3, 1e-3, 1e-10 ** (1 / (2 * L)), L, **kw)
+
+
+def test_banded_with_no_far_tile_is_the_reference_rule(ctx):
+    V, F = mesh3()
+    mesh = abi.make_mesh(V, F)
+    print("banded, kappa = 1e300", flush=True)
+    ref = ctx.assemble(mesh, params(L=16))
+    band = ctx.assemble(mesh, params(L=16, quad=abi.QUAD_BANDED, band_kappa=1e300))
+    for k in ref:
+        assert np.array_equal(ref[k], band[k]), k
+
+
+# (kappa, l): the plan is the same at every frequency; three frequencies across the spectrum
+PLANS = [(1.0, 0), (0.5, 8), (1.0, 16)]
+
+
+@pytest.mark.parametrize("kappa,l", PLANS)
+def test_banded_tiles_follow_the_plan(ctx, kappa, l):
+    V, F = mesh3()
+    mesh = abi.make_mesh(V, F)
+    nv = len(V)
+    p_band = params(quad=abi.QUAD_BANDED, band_kappa=kappa)
+    p_far = params(quad=abi.QUAD_BANDED, band_kappa=1e-300)
+    print(f"band plan kappa={kappa} l={l}", flush=True)
+    near = ctx.laplace(mesh, params(), l)
+    band = ctx.laplace(mesh, p_band, l)
+    far = ctx.laplace(mesh, p_far, l)
+    pov, plan = abi.band_plan(mesh, p_band)
+    _, plan_far = abi.band_plan(mesh, p_far)
+    tile = plan[:, pov]  # (row i, column j) -> far?
+    assert np.all(plan_far[plan])  # every far tile of the banded plan is far in the all-far run too
+    print(f"  far tiles {plan.mean():.3f}, far blocks {tile.mean():.3f}", flush=True)
+    assert 0 < tile.mean() < 1
+    for k in ("U", "P"):
+        b = band[k].reshape(nv, 3, nv, 3)
+        expect = np.where(tile[:, None, :, None], far[k].reshape(nv, 3, nv, 3), near[k].reshape(nv, 3, nv, 3))
+        assert np.array_equal(b, expect), k
+    for k in ("Bt", "Ba"):
+        assert np.array_equal(band[k], near[k]), k
+
+
+def test_banded_accuracy_against_a_refined_rule(ctx):
+    import torch
+    from surfaceonlydeformables_amd.tune_bands import assemble, per_weight_rel, workload
+    V, F, prm = workload("icosphere-l4")
+    mesh = abi.make_mesh(V, F)
+    L = prm["L"]
+    print("banded accuracy, icosphere level 4, L = 32, default threshold", flush=True)
+    fine, _, _ = assemble(ctx, torch, mesh, prm, subdivision=4)
+    ref, _, t_ref = assemble(ctx, torch, mesh, prm)
+    e_ref = {k: per_weight_rel(torch, ref[k], fine[k]) for k in ("G", "H")}
+    del ref
+    band, _, t_band = assemble(ctx, torch, mesh, prm, quad=abi.QUAD_BANDED)
+    e_band = {k: per_weight_rel(torch, band[k], fine[k]) for k in ("G", "H")}
+    del band, fine
+    torch.cuda.empty_cache()
+    frac = t_band["gh_evaluations"] / t_ref["gh_evaluations"]
+    print(f"  regular-face evaluations {frac:.3f} of the reference rule's; G/H phase {t_band['regular_gh']:.1f} ms vs "
+          f"{t_ref['regular_gh']:.1f} ms", flush=True)
+    assert frac < 0.85
+    for k in ("G", "H"):
+        added = e_band[k] - e_ref[k]
+        print(f"  {k}: vs refined rule, reference rule max {e_ref[k].max():.2e}, banded max {e_band[k].max():.2e}; "
+              f"largest added error {added.max():.2e} at n={int(added.argmax())}", flush=True)
+        assert added.max() <= 0.05 * e_ref[k].max(), k

@@ -2,8 +2,9 @@
 
 - Galerkin: icosphere level 3 at L = 64 and level 4 at L = 32, order 5 x subdivision 2, all four matrices.  The
   oracle assembles the rows of one test vertex (every test face around it, against every source face) in FP64
-  at every frequency; per time-domain weight n the GPU rows must lie within 3x the envelope (weights n-4..n+4)
-  of the FP64 oracle's own long-double noise profile of those rows (tests/golden/row_noise.json, written by
+  at every frequency (the same rows on any host: its sums do not depend on the thread count, and a CRC of them is
+  checked); per time-domain weight n the GPU rows must lie within 5x the envelope (weights n-4..n+4) of the FP64
+  oracle's own long-double noise profile of those rows (tests/golden/row_noise.json, written by
   tests/golden/make_row_noise.py), floor 1e-10 (the FP64 bar per weight), and within 1e-7 over the whole stack.
 - The reference's bunny scene group (3485 vertices, collocation, BDF1, L and R derived as the reference does):
   8 rows against the long-double oracle, per weight and over the stack within 2x the FP64 oracle's own
@@ -15,6 +16,7 @@
 Each case prints its per-matrix errors: the largest per-weight error and the whole-stack error."""
 import json
 import os
+import zlib
 
 import numpy as np
 import pytest
@@ -72,15 +74,19 @@ def test_galerkin_full_size_row_sample_parity(ctx, oracle, name):
     o = oracle_rows(oracle, V, F, prm, row, True, oracle.MODE_FP64)
     for k in NAMES:
         a, b = g[k], o[k]
+        # the oracle's sums do not depend on the host's thread count: these are the FP64 rows the profile was
+        # measured on
+        assert zlib.crc32(np.ascontiguousarray(b, np.float64).tobytes()) == noise["fp64_crc32"][k], k
         per_w = [rel(a[n], b[n]) for n in range(L)]
-        # 3x the FP64 oracle's own noise, and at least the 1e-10 relative error per weight that the FP64 mode is held
-        # to (the GPU sums the test points' rows and then projects them, the oracle sums per face pair: their
-        # roundings differ, by a few 1e-11 in some well-conditioned weights).  The noise of one weight is a single
-        # sample of a rounding error that grows like R^-n and jumps tenfold between neighbours (B at ico3/L64:
-        # 2e-11, 5e-11, 1.2e-9 at n = 35, 40, 44); its envelope over the weights n-4..n+4 is the estimate.
+        # The GPU rows and the oracle's FP64 rows each carry FP64 rounding noise of the profile's size (the GPU sums
+        # the test points' rows and then projects them, the oracle sums per face pair), so their distance is up to
+        # about twice the profile, more where the GPU's rounding happens to run larger.  The noise of one weight is a
+        # single sample of a rounding error that grows like R^-n and jumps tenfold between neighbours (B at
+        # ico3/L64: 2e-11, 5e-11, 1.2e-9 at n = 35, 40, 44); its envelope over the weights n-4..n+4 is the estimate.
+        # Bound: 5x that envelope, and at least the 1e-10 relative error per weight the FP64 mode is held to.
         nw = noise["per_weight"][k]
         env = [max(nw[max(0, n - 4):n + 5]) for n in range(L)]
-        bound = [max(3 * e, 1e-10) for e in env]
+        bound = [max(5 * e, 1e-10) for e in env]
         worst = max(range(L), key=lambda n: per_w[n] / bound[n])
         print(f"  {k}: stack {rel(a, b):.2e}, worst weight n={worst} {per_w[worst]:.2e} "
               f"(FP64-oracle noise there {noise['per_weight'][k][worst]:.2e}), max per weight {max(per_w):.2e}",
