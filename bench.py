@@ -75,8 +75,8 @@ def cpu_baseline(V, F, params, seconds_target=15.0):
     op = O.params(params.c1, params.c2, params.rho, params.nu, params.dt, params.R, L, bdf=params.bdf,
                   order=params.order, subdiv=params.subdivision, mode=O.MODE_REFMIXED, nthreads=threads)
     s = O.frequencies(L, params.R, params.dt, params.bdf)
-    ls = [0, Lf // 2]
-    rows = max(threads, 16)
+    ls = sorted({0, Lf // 4, Lf // 2, (3 * Lf) // 4, Lf - 1})  # across the spectrum: real s at l = 0, complex after
+    rows = max(2 * threads, 32)
     t0 = time.perf_counter()
     for l in ls:
         O.laplace(V, F, op, s[l], mask=15, rows=(0, rows))
@@ -103,8 +103,9 @@ def cpu_baseline(V, F, params, seconds_target=15.0):
         q = O.params(params.c1, params.c2, params.rho, params.nu, params.dt, params.R, L, bdf=params.bdf,
                      order=params.order, subdiv=params.subdivision, mode=mode, nthreads=nthreads)
         t = time.perf_counter()
-        O.laplace(V, F, q, s[ls[0]], mask=15, rows=(0, nrows))
-        return (time.perf_counter() - t) * (nv / nrows) * Lf + t_tr * (2 * n3 * n3 / M_sample)
+        for l in ls:
+            O.laplace(V, F, q, s[l], mask=15, rows=(0, nrows))
+        return (time.perf_counter() - t) * (nv / nrows) * (Lf / len(ls)) + t_tr * (2 * n3 * n3 / M_sample)
     full_1 = quad_rate(O.MODE_REFMIXED, 1, 2)
     full_64 = quad_rate(O.MODE_FP64, threads, rows)
     return {"value": entries / full, "unit": "entries/s", "cores": threads, "kind": "port", "cpu_model": model,
@@ -113,9 +114,9 @@ def cpu_baseline(V, F, params, seconds_target=15.0):
             "sample": (f"oracle (reference float/double typedefs), G+H+B_trans+B_angular Laplace rows 0..{rows - 1} "
                        f"of {nv} at frequencies {ls} of {Lf} ({t_quad:.1f} s) + inverse transform of "
                        f"{M_sample} entries ({t_tr:.1f} s), extrapolated to the full job: {full:.0f} s; "
-                       f"one_core_value: rows 0..1 at frequency 0 on one thread; fp64_all_cores_value: rows "
-                       f"0..{rows - 1} at frequency 0 in the oracle's FP64 mode; both add the same (all-thread) "
-                       f"transform sample")}
+                       f"one_core_value: rows 0..1 at the same frequencies on one thread; fp64_all_cores_value: rows "
+                       f"0..{rows - 1} at the same frequencies in the oracle's FP64 mode; both add the same "
+                       f"(all-thread) transform sample")}
 
 
 def scene_jobs(n_objects=16, seed=0):
