@@ -378,7 +378,9 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
                         rad = std::max(rad, nrm(sub(a, V3{c[0], c[1], c[2]})));
                         h = std::max(h, nrm(sub(a, b)));
                     }
-                P.patch_sphere.push_back(make_double4(c[0], c[1], c[2], rad + kappa * h));
+                // the radius is widened by 1e-12 of itself so that rounding never puts a corner of the patch's own
+                // faces outside it (such a row is then never far from its own patch, whatever kappa)
+                P.patch_sphere.push_back(make_double4(c[0], c[1], c[2], rad * (1 + 1e-12) + kappa * h));
             }
             std::vector<int> mult(P.nf, 0);
             for (const auto &l : lists)

@@ -1,0 +1,69 @@
+"""CQBEM_QUAD_BANDED's tile plan on the CPU (cqbem_band_plan needs no GPU): every vertex's column belongs to one
+patch; a tile is far exactly when the row vertex lies kappa x h beyond the patch's bounding sphere, so the far set
+shrinks as kappa grows, is empty for an infinite kappa and never holds the tile of a vertex's own patch; the
+default kappa is the tuned one; and the refusals (the reference rule, Galerkin, mixed precision)."""
+import numpy as np
+import pytest
+
+from surfaceonlydeformables_amd import abi
+from surfaceonlydeformables_amd import mesh as M
+
+
+@pytest.fixture(scope="module")
+def ico4():
+    V, F = M.icosphere(4, 0.2)
+    return M.center(M.jitter(V, F, 0.01, 0), F), F
+
+
+def params(**kw):
+    _, c1, c2 = M.material_speeds(5e5, 0.3, 1000.0)
+    return abi.make_params(c1, c2, 1000.0, 0.3, 1e-3, 1e-10 ** (1 / 64), 32, **kw)
+
+
+def test_plan_partitions_the_columns_and_shrinks_with_kappa(ico4):
+    V, F = ico4
+    mesh = abi.make_mesh(V, F)
+    fractions = []
+    for kappa in (1e-300, 1.0, 3.0, 6.0, 12.0):
+        pov, far = abi.band_plan(mesh, params(quad=abi.QUAD_BANDED, band_kappa=kappa))
+        n = far.shape[1]
+        assert pov.min() == 0 and pov.max() == n - 1
+        assert np.bincount(pov, minlength=n).min() > 0  # every patch holds columns
+        assert not far[np.arange(len(V)), pov].any()  # a row is never far from its own vertex's patch
+        fractions.append(far[:, pov].mean())
+    assert all(a >= b for a, b in zip(fractions, fractions[1:]))
+    assert fractions[0] > 0.5 and fractions[-1] < fractions[0]
+    _, none = abi.band_plan(mesh, params(quad=abi.QUAD_BANDED, band_kappa=1e300))
+    assert not none.any()
+
+
+def test_default_kappa_is_the_tuned_one(ico4):
+    V, F = ico4
+    mesh = abi.make_mesh(V, F)
+    _, dflt = abi.band_plan(mesh, params(quad=abi.QUAD_BANDED))
+    _, six = abi.band_plan(mesh, params(quad=abi.QUAD_BANDED, band_kappa=6.0))
+    assert np.array_equal(dflt, six)
+
+
+def test_far_tiles_are_beyond_the_patch(ico4):
+    """Far tile (i, p) => every vertex of every face of patch p is at least kappa h_min away from vertex i, with
+    h_min the shortest edge (the sphere holds the faces, and kappa h >= kappa h_min)."""
+    V, F = ico4
+    mesh = abi.make_mesh(V, F)
+    kappa = 2.0
+    pov, far = abi.band_plan(mesh, params(quad=abi.QUAD_BANDED, band_kappa=kappa))
+    e = np.concatenate([np.linalg.norm(V[F[:, k]] - V[F[:, (k + 1) % 3]], axis=1) for k in range(3)])
+    rng = np.random.default_rng(0)
+    for i in rng.choice(len(V), 40, replace=False):
+        for p in np.flatnonzero(far[i]):
+            d = np.linalg.norm(V[pov == p] - V[i], axis=1)
+            assert d.min() >= kappa * e.min(), (i, p)
+
+
+def test_plan_refusals(ico4):
+    V, F = ico4
+    mesh = abi.make_mesh(V, F)
+    for p in (params(), params(quad=abi.QUAD_BANDED, galerkin=True),
+              params(quad=abi.QUAD_BANDED, precision=abi.MIXED), params(quad=7)):
+        with pytest.raises(abi.CQBEMError):
+            abi.band_plan(mesh, p)
