@@ -690,9 +690,11 @@ def main():
             roof["note"] = "banded: evaluations counted per tile's rule"
     if args.precision == "mixed":
         line["mixed_vs_fp64"] = mixed_against_fp64(ctx, mesh, params, outs, ms_per_step, torch, dev)
-        if roof:
-            roof["note"] = "mixed: the timed G/H phase is the double H pair kernel plus the float G kernel"
+        if roof and os.environ.get("CQBEM_MIXED_G") == "float":
+            roof["note"] = "mixed, float G: the timed G/H phase is the double H pair kernel plus the float G kernel"
             roof["kernel"] = "k_colloc_gh_pair<false,true,false> + k_colloc_g32"
+        elif roof:
+            roof["note"] = "mixed: G and H in the double pair kernel, the far-field B in float"
     if not args.no_cpu_baseline and not params.galerkin:
         line["cpu_baseline"] = cpu_baseline(V, F, params)
     ctx.close()
@@ -725,8 +727,8 @@ def mixed_against_fp64(ctx, mesh, params, outs, mixed_ms, torch, dev, steps=3):
     torch.cuda.empty_cache()
     return {"fp64_ms_per_step": fp64_ms, "speedup_vs_fp64": fp64_ms / mixed_ms, "rel_err_stack": stack,
             "rel_err_per_weight": per_weight, "target": "<= 1e-6 relative error, >= 3x FP64",
-            "note": "far-field G, B_trans, B_angular in float (Kahan-compensated face sums); H, singular faces and "
-                    "the inverse transform in double"}
+            "note": "far-field B_trans and B_angular in float (Kahan-compensated face sums), far-field G too with "
+                    "CQBEM_MIXED_G=float; H, singular faces and the inverse transform in double"}
 
 
 def roofline(gh_ms, gh_evals, gh_launches, with_traffic):

@@ -448,6 +448,14 @@ bool colloc_star() {
     return e && !std::strcmp(e, "star");
 }
 
+// CQBEM_MIXED's far-field G: by default in the FP64 pair kernel together with H (G shares H's exponentials there,
+// which costs less than a float kernel of its own on MI355X); CQBEM_MIXED_G=float selects the float star kernel
+// (the reference's own typedef for G).
+bool mixed_float_g() {
+    const char *e = std::getenv("CQBEM_MIXED_G");
+    return e && !std::strcmp(e, "float");
+}
+
 // The band descriptor of a prepared job (empty unless CQBEM_QUAD_BANDED).
 Bands bands_of(const DevMesh &d, const Prepared &P) {
     Bands b;
@@ -642,8 +650,8 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
     const Freq *freqs = d.freqs.as<Freq>();
     const double2 *tab = d.tab.as<double2>();
     const int npts = P.npts, nline = (int)P.line.size();
-    // mixed precision: far-field G and B in float (the reference's typedefs), H and the rest in double
-    const bool mixed = p->precision == CQBEM_MIXED;
+    // mixed precision: far-field B (and, with CQBEM_MIXED_G=float, G) in float, the rest in double
+    const bool mixed = p->precision == CQBEM_MIXED, g32 = mixed && mixed_float_g();
     DBuf dfreq32;
     if (mixed) upload(dfreq32, freqs_f32(P.freqs), st);
     const FreqF *freqs32 = dfreq32.as<FreqF>();
@@ -661,7 +669,7 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
                 const int nr = std::min(rows, nv - r0);
                 {
                     Timer t(st, &tm[1]);
-                    if (!mixed) {
+                    if (!g32) {
                         HIPCHK(launch_gh(d, P, freqs, 0, Lf, r0, nr, specG.as<cd>(), specH.as<cd>(), n3, lstride, st));
                     } else {
                         if (wantH) HIPCHK(launch_gh(d, P, freqs, 0, Lf, r0, nr, nullptr, specH.as<cd>(), n3, lstride, st));
@@ -872,7 +880,7 @@ void run_scene(cqbem_context *ctx, hipStream_t st, int n, const cqbem_mesh *mesh
         DevMesh d;
         DBuf specB, specBa, dC, dfreq32, sBt, sBa;
         double *Wbt = nullptr, *Wba = nullptr;  // B_trans / B_angular weights: the caller's, or staged for B
-        bool wantG, wantH, wantB, mixed, f32;
+        bool wantG, wantH, wantB, mixed, g32, f32;
         MatF mat32;
         std::vector<double> C;  // jump term, host copy
     };
@@ -900,6 +908,7 @@ void run_scene(cqbem_context *ctx, hipStream_t st, int n, const cqbem_mesh *mesh
         o->wantG = (p->which & CQBEM_G) && out->G, o->wantH = (p->which & CQBEM_H) && out->H;
         o->wantB = ((p->which & CQBEM_B_TRANS) && out->B_trans) || ((p->which & CQBEM_B_ANGULAR) && out->B_angular) || out->B;
         o->mixed = p->precision == CQBEM_MIXED, o->f32 = p->storage == CQBEM_STORE_F32;
+        o->g32 = o->mixed && mixed_float_g();
         o->mat32 = mat_f32(o->P.mat);
         if (o->wantH)
             o->C = meshes[k].vertex_map_inverse
@@ -985,8 +994,8 @@ void run_scene(cqbem_context *ctx, hipStream_t st, int n, const cqbem_mesh *mesh
             if (o->wantG) sG = reinterpret_cast<cd *>(spec.as<char>() + off), off += (size_t)o->P.Lf * lstride * sizeof(cd);
             if (o->wantH) sH = reinterpret_cast<cd *>(spec.as<char>() + off), off += (size_t)o->P.Lf * lstride * sizeof(cd);
             slabs.push_back(Slab{pc, sG, sH, lstride});
-            // mixed objects: H through the shared launch, G through the float kernel
-            const bool g_in_pair = o->wantG && !o->mixed;
+            // objects with the float G kernel: H through the shared launch, G through the float kernel
+            const bool g_in_pair = o->wantG && !o->g32;
             if (!g_in_pair && !o->wantH) continue;
             ScenePiece sp{o->d.g, o->d.pair_pts.as<double4>(), (int)o->P.quad.size() / 2, o->d.freqs.as<Freq>(), o->P.mat,
                           pc.row0, pc.nrows, o->P.Lf, o->P.pair_npatches, o->d.pair_patch_pass.as<int>(),
@@ -1030,7 +1039,7 @@ void run_scene(cqbem_context *ctx, hipStream_t st, int n, const cqbem_mesh *mesh
                 const Freq *freqs = o->d.freqs.as<Freq>();
                 const double2 *line = o->d.line.as<double2>();
                 const int nline = (int)o->P.line.size(), Lf = o->P.Lf, L = o->P.L;
-                if (o->wantG && o->mixed)
+                if (o->wantG && o->g32)
                     HIPCHK(launch_colloc_g32(g, o->d.quad.as<double4>(), o->P.npts, o->dfreq32.as<FreqF>(), 0, Lf, o->mat32,
                                              sb.pc.row0, sb.pc.nrows, sb.sG, n3, sb.lstride, st));
                 if (o->wantG)
@@ -1433,7 +1442,7 @@ int cqbem_laplace(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params
         DBuf dfreq32;
         if (mixed) upload(dfreq32, freqs_f32(P.freqs), st);
         const MatF mat32 = mat_f32(P.mat);
-        if (!params->galerkin && mixed) {  // far-field G and B in float, H in double (as cqbem_assemble)
+        if (!params->galerkin && mixed && mixed_float_g()) {  // far-field G and B in float, H in double (as cqbem_assemble)
             if (wH) HIPCHK(launch_gh(d, P, freqs, l, 1, 0, nv, nullptr, dP.as<cd>(), n3, n3 * n3, st));
             if (wG) HIPCHK(launch_colloc_g32(d.g, quad, npts, dfreq32.as<FreqF>(), l, 1, mat32, 0, nv, dU.as<cd>(), n3, n3 * n3, st));
             if (wG) HIPCHK(launch_colloc_singular(0, d.g, line, nline, freqs, l, 1, P.mat, 0, nv, dU.as<cd>(), n3, n3 * n3, nullptr, nullptr, st));
@@ -1450,7 +1459,10 @@ int cqbem_laplace(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params
                 if (wH) HIPCHK(launch_colloc_singular(2, d.g, line, nline, freqs, l, 1, P.mat, 0, nv, dP.as<cd>(), n3, n3 * n3, nullptr, nullptr, st));
             }
             if (wB) {
-                HIPCHK(launch_colloc_b(d.g, quad, npts, freqs, l, 1, P.mat, 0, nv, dBt.as<cd>(), dBa.as<cd>(), st));
+                if (mixed)
+                    HIPCHK(launch_colloc_b32(d.g, quad, npts, dfreq32.as<FreqF>(), l, 1, mat32, 0, nv, dBt.as<cd>(), dBa.as<cd>(), st));
+                else
+                    HIPCHK(launch_colloc_b(d.g, quad, npts, freqs, l, 1, P.mat, 0, nv, dBt.as<cd>(), dBa.as<cd>(), st));
                 HIPCHK(launch_colloc_singular(1, d.g, line, nline, freqs, l, 1, P.mat, 0, nv, nullptr, n3, 0, dBt.as<cd>(),
                                               dBa.as<cd>(), st));
             }

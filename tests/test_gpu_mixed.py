@@ -1,8 +1,10 @@
-"""Mixed precision (CQBEM_MIXED, collocation): far-field G, B_trans and B_angular in float with Kahan-compensated
-face sums, H, the singular faces and the inverse transform in double -- the reference's own precision policy
-(G_collocation.cu:14-15, B_trans_collocation.cu:14, B_angular_collocation.cu:14; H_collocation.cu:15 is double).
+"""Mixed precision (CQBEM_MIXED, collocation): far-field B_trans and B_angular in float with Kahan-compensated face
+sums, H, the singular faces and the inverse transform in double; far-field G in double by default (in the pair kernel
+with H: bit-identical to the FP64 mode), in float with CQBEM_MIXED_G=float -- then the reference's own precision
+policy (G_collocation.cu:14-15, B_trans_collocation.cu:14, B_angular_collocation.cu:14; H_collocation.cu:15 is
+double).  Both variants run.
 
-Checked against the oracle's "ref-mixed" mode, which restates the same typedefs on the CPU:
+Checked against the oracle's "ref-mixed" mode, which restates the reference's typedefs on the CPU:
 - Laplace domain, per frequency: H is the double path's (1e-10 to the FP64 oracle, as there); U, B_trans and
   B_angular sit within 4x the oracle ref-mixed's own distance from the FP64 oracle (float rounding, ~1e-7).
 - Time domain: the R^-n factor of the inverse transform amplifies the per-frequency float rounding; per matrix
@@ -37,20 +39,25 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
+@pytest.mark.parametrize("g_float", [False, True])
 @pytest.mark.parametrize("name", ["ico1", "cube2"])
 @pytest.mark.parametrize("l", [0, 16])
-def test_mixed_laplace_parity(ctx, oracle, name, l):
+def test_mixed_laplace_parity(ctx, oracle, name, l, g_float, monkeypatch):
+    monkeypatch.setenv("CQBEM_MIXED_G", "float" if g_float else "double")
     V, F = meshes()[name]
     _, c1, c2 = M.material_speeds(5e5, 0.3, RHO)
     L, dt = 32, 1e-3
     R = 1e-10 ** (1 / (2 * L))
-    print(f"mixed laplace {name} l={l}", flush=True)
+    print(f"mixed laplace {name} l={l} float G {g_float}", flush=True)
     g = ctx.laplace(abi.make_mesh(V, F), abi.make_params(c1, c2, RHO, 0.3, dt, R, L, precision=abi.MIXED), l)
+    if not g_float:
+        g64 = ctx.laplace(abi.make_mesh(V, F), abi.make_params(c1, c2, RHO, 0.3, dt, R, L), l)
+        assert np.array_equal(g["U"], g64["U"]) and np.array_equal(g["P"], g64["P"])
     s = oracle.frequencies(L, R, dt, 2)[l]
     o64 = oracle.laplace(V, F, oracle.params(c1, c2, RHO, 0.3, dt, R, L, nthreads=8), s)
     omx = oracle.laplace(V, F, oracle.params(c1, c2, RHO, 0.3, dt, R, L, mode=oracle.MODE_REFMIXED, nthreads=8), s)
     assert rel(g["P"], o64["P"]) < 1e-10
-    for k in ("U", "Bt", "Ba"):
+    for k in ("U", "Bt", "Ba") if g_float else ("Bt", "Ba"):
         e_gpu, e_ref = rel(g[k], o64[k]), rel(omx[k], o64[k])
         print(f"  {k}: GPU mixed {e_gpu:.2e}, oracle ref-mixed {e_ref:.2e}", flush=True)
         assert e_gpu < max(4 * e_ref, 1e-7), (k, e_gpu, e_ref)
@@ -61,8 +68,10 @@ def test_mixed_laplace_parity(ctx, oracle, name, l):
             assert e_gpu > 1e-10, (k, e_gpu, e_ref)
 
 
+@pytest.mark.parametrize("g_float", [False, True])
 @pytest.mark.parametrize("name", ["ico1", "cube2"])
-def test_mixed_time_domain_against_reference_precision(ctx, oracle, name):
+def test_mixed_time_domain_against_reference_precision(ctx, oracle, name, g_float, monkeypatch):
+    monkeypatch.setenv("CQBEM_MIXED_G", "float" if g_float else "double")
     V, F = meshes()[name]
     _, c1, c2 = M.material_speeds(5e5, 0.3, RHO)
     L, dt = 16, 1e-3
@@ -75,6 +84,8 @@ def test_mixed_time_domain_against_reference_precision(ctx, oracle, name):
     o64 = oracle.assemble(V, F, op(oracle.MODE_FP64))
     omx = oracle.assemble(V, F, op(oracle.MODE_REFMIXED))
     assert np.array_equal(mx["H"], f64["H"])  # H stays in double, through the same kernels
+    if not g_float:
+        assert np.array_equal(mx["G"], f64["G"])  # so does G by default
     for mine, theirs in (("G", "G"), ("B_trans", "Bt"), ("B_angular", "Ba"), ("B", "B6")):
         a = mx[mine].reshape(L, -1); b = o64[theirs].reshape(L, -1); c = omx[theirs].reshape(L, -1)
         per_gpu = [rel(a[n], b[n]) for n in range(L)]
