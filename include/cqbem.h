@@ -46,8 +46,8 @@ enum { CQBEM_BDF1 = 1, CQBEM_BDF2 = 2 };
  * (B_trans_collocation.cu:14, B_angular_collocation.cu:14), with Kahan-compensated face sums; H, the singular faces
  * and the inverse transform in double.  Far-field G stays in double by default: it shares H's exponentials in the
  * pair kernel, which costs less than a float kernel of its own; the environment variable CQBEM_MIXED_G=float
- * selects the reference's float G (G_collocation.cu:14-15) instead.  Galerkin with CQBEM_MIXED returns
- * CQBEM_ERR_UNSUPPORTED. */
+ * selects the reference's float G (G_collocation.cu:14-15) instead.  Galerkin: the far-field B at the test points
+ * in float, G and H in double (CQBEM_MIXED_G=float with Galerkin returns CQBEM_ERR_UNSUPPORTED). */
 enum { CQBEM_FP64 = 0, CQBEM_MIXED = 1 };
 /* Element type of the G and H outputs: double, or float (weights rounded once from the double transform; device
  * outputs only).  B_trans, B_angular and B are always double.  No counterpart in the reference: a storage policy for

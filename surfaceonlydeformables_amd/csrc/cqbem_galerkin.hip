@@ -935,8 +935,14 @@ void run_galerkin(const GalerkinJob &job, const int *F, const int *vmap, cd *U, 
                     if (P) project(sH, n3, ls, faces + fi0, nfc, row0, P, n3 * n3);
                 }
                 if (Bt && Ba) {
-                    check(launch_points_b(g, job.quad, npts, job.freqs, 0, nl, job.mat, tpts, row0, nrows, sBt, sBa, cap_rows, st),
-                          "k_colloc_b (test points)");
+                    if (job.freqs32)
+                        check(launch_points_b32(g, job.quad, npts, job.freqs32, 0, nl, job.mat32, tpts, row0, nrows, sBt, sBa,
+                                                cap_rows, st),
+                              "k_colloc_b32 (test points)");
+                    else
+                        check(launch_points_b(g, job.quad, npts, job.freqs, 0, nl, job.mat, tpts, row0, nrows, sBt, sBa,
+                                              cap_rows, st),
+                              "k_colloc_b (test points)");
                     project(sBt, 3, cap_rows * 3, faces + fi0, nfc, row0, Bt, n3 * 3);
                     project(sBa, 9, cap_rows * 9, faces + fi0, nfc, row0, Ba, n3 * 9);
                 }

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "cqbem_kernels.hpp"
+#include "cqbem_mixed.hpp"
 
 namespace cqbem {
 
@@ -27,6 +28,9 @@ struct GalerkinJob {
     const int *patch_verts;
     int npatches;
     size_t scratch_bytes;  // bound of the per-test-point row scratch (at least one face is always processed)
+    // CQBEM_MIXED: the far-field B at the test points in float (first frequency of the job's range), else null
+    const FreqF *freqs32 = nullptr;
+    MatF mat32{};
 };
 
 // Test faces per chunk of the per-test-point row scratch, and the device bytes the scratch takes

@@ -82,6 +82,8 @@ template <class T> void upload(DBuf &b, const std::vector<T> &v, hipStream_t st)
     if (!v.empty()) HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
 }
 
+bool mixed_float_g();  // CQBEM_MIXED_G=float (below)
+
 void check_inputs(const cqbem_mesh *mesh, const cqbem_params *p) {
     if (!mesh || !p) throw Err{CQBEM_ERR_ARG, "null mesh or params"};
     if (!mesh->V || !mesh->F || mesh->nv < 3 || mesh->nf < 1) throw Err{CQBEM_ERR_ARG, "empty mesh"};
@@ -106,8 +108,8 @@ void check_inputs(const cqbem_mesh *mesh, const cqbem_params *p) {
     if (p->order < 1 || p->order > 5) throw Err{CQBEM_ERR_ARG, "order must be in [1, 5]"};
     if (p->subdivision < 1 || p->subdivision > 16) throw Err{CQBEM_ERR_ARG, "subdivision must be in [1, 16]"};
     if (p->precision != CQBEM_FP64 && p->precision != CQBEM_MIXED) throw Err{CQBEM_ERR_ARG, "unknown precision"};
-    if (p->precision == CQBEM_MIXED && p->galerkin)
-        throw Err{CQBEM_ERR_UNSUPPORTED, "CQBEM_MIXED is implemented for collocation only"};
+    if (p->precision == CQBEM_MIXED && p->galerkin && mixed_float_g())
+        throw Err{CQBEM_ERR_UNSUPPORTED, "CQBEM_MIXED_G=float is implemented for collocation only"};
     if (p->storage != CQBEM_STORE_F64 && p->storage != CQBEM_STORE_F32) throw Err{CQBEM_ERR_ARG, "unknown storage"};
     if ((p->which & ~CQBEM_ALL) != 0) throw Err{CQBEM_ERR_ARG, "unknown matrix bits in `which`"};
     if (p->quad_policy != CQBEM_QUAD_REFERENCE && p->quad_policy != CQBEM_QUAD_BANDED)
@@ -725,6 +727,7 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
         GalerkinJob gj{g, quad, npts, line, nline, freqs, Lf, P.mat, ctx->num_cus, d.pair_pts.as<double4>(), (int)P.quad.size(),
                        d.pair_patch_pass.as<int>(), d.pair_pass_info.as<int2>(), d.pair_pass_faces.as<int4>(),
                        d.pair_patch_verts.as<int>(), P.pair_npatches, galerkin_budget(slab_budget)};
+        if (mixed) gj.freqs32 = freqs32, gj.mat32 = mat32;  // far-field B at the test points in float
         const int nmat = (wantG ? 1 : 0) + (wantH ? 1 : 0);
         if (nmat || wantB) {
             // Galerkin spectra are assembled for the full matrix (face pairs scatter to any row)
@@ -1470,6 +1473,7 @@ int cqbem_laplace(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params
             GalerkinJob gj{d.g, quad, npts, line, nline, freqs + l, 1, P.mat, ctx->num_cus, d.pair_pts.as<double4>(),
                            (int)P.quad.size(), d.pair_patch_pass.as<int>(), d.pair_pass_info.as<int2>(),
                            d.pair_pass_faces.as<int4>(), d.pair_patch_verts.as<int>(), P.pair_npatches, (size_t)1 << 30};
+            if (mixed) gj.freqs32 = dfreq32.as<FreqF>() + l, gj.mat32 = mat32;
             double tm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             run_galerkin(gj, mesh->F, mesh->vertex_map_inverse, wG ? dU.as<cd>() : nullptr, wH ? dP.as<cd>() : nullptr,
                          wB ? dBt.as<cd>() : nullptr, wB ? dBa.as<cd>() : nullptr, tm, st);

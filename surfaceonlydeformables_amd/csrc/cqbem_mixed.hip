@@ -139,20 +139,37 @@ __global__ void __launch_bounds__(64) k_colloc_g32(Geom g, const double4 *__rest
 // -------------------------------------------------------------------- B --
 constexpr int kBWords32 = 48;
 
+// GAL: the rows are test points tpts[row0 + r] = (x, y, z, test face), each excluding the faces that share a vertex
+// with its test face, and row r of the launch goes to rows 3r..3r+2 of Bt/Ba (as k_colloc_b<true>).
+template <bool GAL>
 __global__ void __launch_bounds__(256) k_colloc_b32(Geom g, const double4 *__restrict__ quad, int npts,
                                                      const FreqF *__restrict__ freqs, int l0, MatF m, int row0,
-                                                     cd *__restrict__ Bt, cd *__restrict__ Ba, long long nv3) {
+                                                     const double4 *__restrict__ tpts, cd *__restrict__ Bt,
+                                                     cd *__restrict__ Ba, long long nv3) {
     __shared__ double red[4][kBWords32];
     const int l = blockIdx.x;
     const int i = row0 + blockIdx.y;
     const FreqF f = freqs[l0 + l];
     const cf is{f.isx, f.isy}, is2{f.is2x, f.is2y};
-    const float x0 = (float)g.V[3 * i], x1 = (float)g.V[3 * i + 1], x2 = (float)g.V[3 * i + 2];
+    float x0, x1, x2;
+    int t[3] = {i, i, i};
+    if (GAL) {
+        const double4 T = tpts[i];
+        x0 = (float)T.x, x1 = (float)T.y, x2 = (float)T.z;
+        const int f1 = (int)T.w;
+        t[0] = g.F[3 * f1], t[1] = g.F[3 * f1 + 1], t[2] = g.F[3 * f1 + 2];
+    } else {
+        x0 = (float)g.V[3 * i], x1 = (float)g.V[3 * i + 1], x2 = (float)g.V[3 * i + 2];
+    }
+    const int io = GAL ? (int)blockIdx.y : i;
     constexpr float kI8 = 0.0397887358f, kI4 = 0.0795774715f;  // 1/(8 pi), 1/(4 pi)
     KahanC bt[6], ba[18];
     for (int fc = threadIdx.x; fc < g.nf; fc += 256) {
         const int a0 = g.F[3 * fc], a1 = g.F[3 * fc + 1], a2 = g.F[3 * fc + 2];
-        if (same_vertex_g(g, i, a0) || same_vertex_g(g, i, a1) || same_vertex_g(g, i, a2)) continue;
+        bool touch = false;
+        for (int k = 0; k < (GAL ? 3 : 1); k++)
+            touch = touch || same_vertex_g(g, t[k], a0) || same_vertex_g(g, t[k], a1) || same_vertex_g(g, t[k], a2);
+        if (touch) continue;
         const double *p0 = g.V + 3 * a0, *p1 = g.V + 3 * a1, *p2 = g.V + 3 * a2;
         const float n[3] = {(float)g.N[3 * fc], (float)g.N[3 * fc + 1], (float)g.N[3 * fc + 2]};
         const float ar = (float)g.area[fc];
@@ -229,13 +246,13 @@ __global__ void __launch_bounds__(256) k_colloc_b32(Geom g, const double4 *__res
         const int c = k & 1;
         if (k < 12) {
             const int e = k >> 1;
-            double *bt_out = reinterpret_cast<double *>(Bt + (long long)l * nv3 * 3 + (long long)(3 * i) * 3);
+            double *bt_out = reinterpret_cast<double *>(Bt + (long long)l * nv3 * 3 + (long long)(3 * io) * 3);
             for (int a = 0; a < 3; a++)
                 for (int b = 0; b < 3; b++)
                     if (sidx(a, b) == e) bt_out[2 * (a * 3 + b) + c] = v;
         } else {
             const int e = (k - 12) >> 1, sab = e / 3, kk = e % 3;
-            double *ba_out = reinterpret_cast<double *>(Ba + (long long)l * nv3 * 9 + (long long)(3 * i) * 9);
+            double *ba_out = reinterpret_cast<double *>(Ba + (long long)l * nv3 * 9 + (long long)(3 * io) * 9);
             for (int a = 0; a < 3; a++)
                 for (int b = 0; b < 3; b++)
                     if (sidx(a, b) == sab) ba_out[2 * (a * 9 + b * 3 + kk) + c] = v;
@@ -256,8 +273,16 @@ hipError_t launch_colloc_g32(const Geom &g, const double4 *quad, int npts, const
 hipError_t launch_colloc_b32(const Geom &g, const double4 *quad, int npts, const FreqF *freqs, int l0, int nl, const MatF &m,
                              int row0, int nrows, cd *Bt, cd *Ba, hipStream_t st) {
     if (nrows <= 0 || nl <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_colloc_b32, dim3(nl, nrows), dim3(256), 0, st, g, quad, npts, freqs, l0, m, row0, Bt, Ba,
-                       (long long)3 * g.nv);
+    hipLaunchKernelGGL(k_colloc_b32<false>, dim3(nl, nrows), dim3(256), 0, st, g, quad, npts, freqs, l0, m, row0,
+                       (const double4 *)nullptr, Bt, Ba, (long long)3 * g.nv);
+    return hipGetLastError();
+}
+
+hipError_t launch_points_b32(const Geom &g, const double4 *quad, int npts, const FreqF *freqs, int l0, int nl, const MatF &m,
+                             const double4 *tpts, int row0, int nrows, cd *Bt, cd *Ba, long long out_rows, hipStream_t st) {
+    if (nrows <= 0 || nrows > 65535 || 3LL * nrows > out_rows) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_colloc_b32<true>, dim3(nl, nrows), dim3(256), 0, st, g, quad, npts, freqs, l0, m, row0, tpts, Bt,
+                       Ba, out_rows);
     return hipGetLastError();
 }
 

@@ -96,7 +96,37 @@ def test_mixed_time_domain_against_reference_precision(ctx, oracle, name, g_floa
             assert per_gpu[n] <= max(4 * per_ref[n], 1e-7), (mine, n, per_gpu[n], per_ref[n])
 
 
-def test_mixed_galerkin_is_refused(ctx):
+@pytest.mark.parametrize("name", ["ico1", "cube2"])
+def test_mixed_galerkin_far_field_b_in_float(ctx, oracle, name, monkeypatch):
+    """Galerkin with CQBEM_MIXED: G and H bit-identical to FP64 (the pair kernel at the test points and the
+    Sauter-Schwab pairs stay in double), the far-field B at the test points in float: per weight within 4x the
+    ref-mixed oracle's distance from the FP64 oracle (floor 1e-7)."""
+    monkeypatch.setenv("CQBEM_MIXED_G", "double")
+    V, F = meshes()[name]
+    _, c1, c2 = M.material_speeds(5e5, 0.3, RHO)
+    L, dt = 8, 1e-3
+    R = 1e-10 ** (1 / (2 * L))
+    print(f"mixed galerkin {name} L={L}", flush=True)
+    mesh = abi.make_mesh(V, F)
+    kw = dict(order=2, subdivision=1, galerkin=True)
+    mx = ctx.assemble(mesh, abi.make_params(c1, c2, RHO, 0.3, dt, R, L, precision=abi.MIXED, **kw))
+    f64 = ctx.assemble(mesh, abi.make_params(c1, c2, RHO, 0.3, dt, R, L, **kw))
+    assert np.array_equal(mx["G"], f64["G"]) and np.array_equal(mx["H"], f64["H"])
+    op = lambda mode: oracle.params(c1, c2, RHO, 0.3, dt, R, L, order=2, subdiv=1, galerkin=True, mode=mode, nthreads=8)
+    o64 = oracle.assemble(V, F, op(oracle.MODE_FP64), mask=oracle.MASK_BT | oracle.MASK_BA)
+    omx = oracle.assemble(V, F, op(oracle.MODE_REFMIXED), mask=oracle.MASK_BT | oracle.MASK_BA)
+    for mine, theirs in (("B_trans", "Bt"), ("B_angular", "Ba"), ("B", "B6")):
+        a = mx[mine].reshape(L, -1); b = o64[theirs].reshape(L, -1); c = omx[theirs].reshape(L, -1)
+        per_gpu = [rel(a[n], b[n]) for n in range(L)]
+        per_ref = [rel(c[n], b[n]) for n in range(L)]
+        print(f"  {mine}: per weight max {max(per_gpu):.2e} (ref-mixed oracle {max(per_ref):.2e})", flush=True)
+        assert max(per_gpu) > 1e-10  # the float kernel ran
+        for n in range(L):
+            assert per_gpu[n] <= max(4 * per_ref[n], 1e-7), (mine, n, per_gpu[n], per_ref[n])
+
+
+def test_mixed_galerkin_float_g_is_refused(ctx, monkeypatch):
+    monkeypatch.setenv("CQBEM_MIXED_G", "float")
     V, F = meshes()["ico1"]
     _, c1, c2 = M.material_speeds(5e5, 0.3, RHO)
     p = abi.make_params(c1, c2, RHO, 0.3, 1e-3, 1e-10 ** (1 / 16), 8, order=2, subdivision=1, galerkin=True,
