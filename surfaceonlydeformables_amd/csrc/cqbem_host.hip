@@ -864,7 +864,8 @@ template <class Fn> void parallel_for(size_t n, Fn fn) {
 
 // A scene: every object's G/H regular-face tiles go through shared launches of the pair kernel (k_scene_gh_tiles:
 // all objects' tiles in one grid, largest pieces first), in waves whose spectra fit the scratch budget; each wave's
-// launches -- the shared pair kernel, the float G kernel of mixed objects, the singular faces and the transforms of
+// launches -- the shared pair kernel, the float G kernel of objects that use it (CQBEM_MIXED_G=float), the singular
+// faces and the transforms of
 // every piece -- are captured in one HIP graph and launched once.  B, C and the B post-processing of all objects
 // follow in a last graph.  Galerkin objects run one after another through run_job.
 void run_scene(cqbem_context *ctx, hipStream_t st, int n, const cqbem_mesh *meshes, const cqbem_params *params,
