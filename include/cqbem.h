@@ -38,7 +38,8 @@ enum {
     CQBEM_ERR_NOMEM = -3,     /* job does not fit the context's HBM budget or device memory */
     CQBEM_ERR_UNSUPPORTED = -4,
     CQBEM_ERR_BUSY = -5,      /* serving: no free slot (retry later) */
-    CQBEM_ERR_NOTFOUND = -6
+    CQBEM_ERR_NOTFOUND = -6,
+    CQBEM_ERR_CANCELLED = -7  /* serving: the job was cancelled (its outputs are partial) */
 };
 
 enum { CQBEM_BDF1 = 1, CQBEM_BDF2 = 2 };
@@ -202,6 +203,11 @@ int cqbem_submit_resident(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbe
  * its output bytes leave the reservation.  CQBEM_ERR_BUSY while the job still runs, CQBEM_ERR_NOTFOUND for an
  * unknown id or a job that was not resident. */
 int cqbem_release(cqbem_context *ctx, uint64_t job_id);
+/* Asks a submitted job to stop: it ends at its next row-slab or phase boundary (the device work already launched
+ * completes), its scratch reservation is released, and cqbem_poll / cqbem_wait then return CQBEM_ERR_CANCELLED; its
+ * outputs are partial.  A job that has already finished is unaffected; a resident job keeps its output reservation
+ * until cqbem_release.  CQBEM_ERR_NOTFOUND for an unknown id.  No counterpart in the reference: a serving control. */
+int cqbem_cancel(cqbem_context *ctx, uint64_t job_id);
 
 /* ---- time stepping on the resident matrices ----
  * History term of one CQ time step (reference: ElastodynamicsCQBEMObject::estimate_next_state,

@@ -14,7 +14,7 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libcqbem.so")
 
 CQBEM_OK = 0
-ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_BUSY, ERR_NOTFOUND = -1, -2, -3, -4, -5, -6
+ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_BUSY, ERR_NOTFOUND, ERR_CANCELLED = -1, -2, -3, -4, -5, -6, -7
 BDF1, BDF2 = 1, 2
 FP64, MIXED = 0, 1
 STORE_F64, STORE_F32 = 0, 1
@@ -26,7 +26,7 @@ EXPORTED = [
     "cqbem_laplace", "cqbem_frequencies", "cqbem_last_timings", "cqbem_submit", "cqbem_poll",
     "cqbem_wait", "cqbem_reserved_bytes", "cqbem_history", "cqbem_matvec", "cqbem_split_traction_discontinuity",
     "cqbem_wavelet_compress", "cqbem_csr_matvec", "cqbem_assemble_noalloc", "cqbem_submit_resident", "cqbem_release",
-    "cqbem_history_f32", "cqbem_assemble_scene", "cqbem_band_plan",
+    "cqbem_history_f32", "cqbem_assemble_scene", "cqbem_band_plan", "cqbem_cancel",
 ]
 
 
@@ -88,6 +88,7 @@ def lib():
         L.cqbem_submit.argtypes = [C.c_void_p, P(cqbem_mesh), P(cqbem_params), P(cqbem_outputs), P(C.c_uint64)]
         L.cqbem_submit_resident.argtypes = [C.c_void_p, P(cqbem_mesh), P(cqbem_params), P(cqbem_outputs), P(C.c_uint64)]
         L.cqbem_release.argtypes = [C.c_void_p, C.c_uint64]
+        L.cqbem_cancel.argtypes = [C.c_void_p, C.c_uint64]
         L.cqbem_history_f32.argtypes = [C.c_void_p, C.c_int64, C.c_int32] + [C.c_void_p] * 6 + [C.c_int32, C.c_void_p]
         L.cqbem_poll.argtypes = [C.c_void_p, C.c_uint64, P(C.c_int32)]
         L.cqbem_wait.argtypes = [C.c_void_p, C.c_uint64]
@@ -260,6 +261,10 @@ class Context:
             return -1
         self._check(rc)
         return jid.value
+
+    def cancel(self, job_id: int):
+        """Asks a submitted job to stop at its next slab or phase boundary (cqbem_cancel)."""
+        self._check(lib().cqbem_cancel(self._h, job_id))
 
     def release(self, job_id: int):
         self._check(lib().cqbem_release(self._h, job_id))

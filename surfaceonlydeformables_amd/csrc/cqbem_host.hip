@@ -153,6 +153,7 @@ struct JobRecord {
     bool resident = false;
     int slot = -1;
     std::atomic<int> done{0};
+    std::atomic<int> cancel{0};  // set by cqbem_cancel; the worker stops at its next slab or phase boundary
     int status = 0;
     std::string error;
     std::thread worker;
@@ -588,7 +589,12 @@ uint64_t footprint(const cqbem_mesh *mesh, const cqbem_params *p, int outputs_on
 
 // Runs one job on stream `st`.  `slab_budget` bounds the Laplace-domain scratch.
 void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const cqbem_params *p, const cqbem_outputs *out,
-             size_t slab_budget, double *tm, bool sync_outputs, JobArena *fixed_arena = nullptr) {
+             size_t slab_budget, double *tm, bool sync_outputs, JobArena *fixed_arena = nullptr,
+             const std::atomic<int> *cancel = nullptr) {
+    // a cancelled serving job ends at the next check (the arena then waits for the work already launched)
+    auto check_cancel = [cancel]() {
+        if (cancel && cancel->load()) throw Err{CQBEM_ERR_CANCELLED, "job cancelled"};
+    };
     // every device buffer of the job comes from the context's pool and goes back to it when the stream is done
     // (CQBEM_POOL=0: plain hipMalloc / hipFree per buffer), or from the caller's fixed region
     static const bool use_pool = [] {
@@ -668,6 +674,7 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
             if (wantG) specG.alloc((size_t)Lf * lstride * sizeof(cd));
             if (wantH) specH.alloc((size_t)Lf * lstride * sizeof(cd));
             for (int r0 = 0; r0 < nv; r0 += rows) {
+                check_cancel();
                 const int nr = std::min(rows, nv - r0);
                 {
                     Timer t(st, &tm[1]);
@@ -700,6 +707,7 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
             }
         }
         if (wantB) {
+            check_cancel();
             {
                 Timer t(st, &tm[2]);
                 if (mixed)
@@ -728,6 +736,7 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
                        d.pair_patch_pass.as<int>(), d.pair_pass_info.as<int2>(), d.pair_pass_faces.as<int4>(),
                        d.pair_patch_verts.as<int>(), P.pair_npatches, galerkin_budget(slab_budget)};
         if (mixed) gj.freqs32 = freqs32, gj.mat32 = mat32;  // far-field B at the test points in float
+        check_cancel();
         const int nmat = (wantG ? 1 : 0) + (wantH ? 1 : 0);
         if (nmat || wantB) {
             // Galerkin spectra are assembled for the full matrix (face pairs scatter to any row)
@@ -1584,7 +1593,7 @@ static int submit_impl(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_p
             double tm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             try {
                 HIPCHK(hipSetDevice(dev));
-                run_job(ctx, st, &jr->mesh, &jr->params, &jr->out, slab, tm, false);
+                run_job(ctx, st, &jr->mesh, &jr->params, &jr->out, slab, tm, false, nullptr, &jr->cancel);
             } catch (const Err &e) {
                 jr->status = e.code, jr->error = e.msg;
             } catch (const std::exception &e) {
@@ -1631,6 +1640,15 @@ int cqbem_release(cqbem_context *ctx, uint64_t job_id) {
         ctx->jobs.erase(it);
     }
     if (worker.joinable()) worker.join();  // finished: returns at once
+    return CQBEM_OK;
+}
+
+int cqbem_cancel(cqbem_context *ctx, uint64_t job_id) {
+    if (!ctx) return CQBEM_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->jobs.find(job_id);
+    if (it == ctx->jobs.end()) return CQBEM_ERR_NOTFOUND;
+    it->second->cancel.store(1);
     return CQBEM_OK;
 }
 

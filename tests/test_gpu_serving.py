@@ -203,3 +203,52 @@ def test_single_precision_storage_policy():
         assert ctx.reserved_bytes() == 0
     finally:
         ctx.close()
+
+
+def test_cancel_stops_a_running_job():
+    """cqbem_cancel: an icosphere-L4 job under a 200 MB budget runs in ~256 slabs of 10 rows; cancelled right
+    after its submit it stops at a slab boundary, wait reports CQBEM_ERR_CANCELLED and its reservation is
+    released.  The context stays usable: the next job equals its solo assembly bit for bit.  Cancelling an unknown
+    job is CQBEM_ERR_NOTFOUND."""
+    import time
+    import torch
+    V, F = M.icosphere(4, 0.2)
+    V = M.center(M.jitter(V, F, 0.01, 0), F)
+    _, c1, c2 = M.material_speeds(5e5, 0.3, RHO)
+    L = 32
+    p = abi.make_params(c1, c2, RHO, 0.3, 1e-3, 1e-10 ** (1 / (2 * L)), L)
+    mesh = abi.make_mesh(V, F)
+    outs = device_outputs(torch, len(V), L)
+    ctx = abi.Context(0, hbm_budget_bytes=200 << 20)
+    try:
+        t0 = time.perf_counter()
+        jid = ctx.submit(mesh, p, {n: t.data_ptr() for n, t in outs.items()})
+        assert jid > 0
+        ctx.cancel(jid)
+        with pytest.raises(abi.CQBEMError) as err:
+            ctx.wait(jid)
+        print(f"cancelled job returned after {1e3 * (time.perf_counter() - t0):.1f} ms", flush=True)
+        assert err.value.code == abi.ERR_CANCELLED
+        assert ctx.reserved_bytes() == 0
+        with pytest.raises(abi.CQBEMError) as err:
+            ctx.cancel(jid + 1000)
+        assert err.value.code == abi.ERR_NOTFOUND
+        # a small job afterwards, against its solo assembly on a fresh context
+        V2, F2 = M.icosphere(2, 0.2)
+        V2 = M.center(M.jitter(V2, F2, 0.01, 1), F2)
+        p2 = abi.make_params(c1, c2, RHO, 0.3, 1e-3, 1e-10 ** (1 / 32), 16)
+        mesh2 = abi.make_mesh(V2, F2)
+        served = device_outputs(torch, len(V2), 16)
+        jid2 = ctx.submit(mesh2, p2, {n: t.data_ptr() for n, t in served.items()})
+        assert jid2 > 0
+        ctx.wait(jid2)
+        torch.cuda.synchronize()
+        solo_ctx = abi.Context(0)
+        try:
+            solo = solo_ctx.assemble(mesh2, p2)
+        finally:
+            solo_ctx.close()
+        for n in served:
+            assert np.array_equal(served[n].cpu().numpy(), solo[n]), n
+    finally:
+        ctx.close()
