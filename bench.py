@@ -234,6 +234,9 @@ def scene_main(args):
     import torch
     from surfaceonlydeformables_amd import abi
     jobs = scene_jobs()
+    for _, _, p in jobs:  # precision and regular-face rule of every object, as the command line asks
+        p.precision = abi.MIXED if args.precision == "mixed" else abi.FP64
+        p.quad_policy = abi.QUAD_BANDED if args.quad == "banded" else abi.QUAD_REFERENCE
     entries = sum(2.0 * (3 * len(V)) ** 2 * p.L for V, F, p in jobs)
     outs = scene_outputs(jobs, torch)  # 257 GB: the same buffers for every path
     ctx = abi.Context(0)
@@ -275,7 +278,9 @@ def scene_main(args):
                       "slots_makespan_ms": makespan * 1e3, "serial_sum_ms": serial * 1e3,
                       "slots_latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3,
                                            "p99": float(np.percentile(lat, 99)) * 1e3},
-                      "scene": [{"nv": len(V), "L": p.L} for V, F, p in jobs], "dtype": "f64", "data": "synthetic"}),
+                      "scene": [{"nv": len(V), "L": p.L} for V, F, p in jobs], "precision": args.precision,
+                      "quadrature": args.quad, "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+                      "data": "synthetic"}),
           flush=True)
 
 
@@ -423,8 +428,9 @@ def serving_main(args):
     jobs = serving_jobs(n, rng, args.precision)
     from surfaceonlydeformables_amd import abi
     warm = scene_jobs(2, seed=5)  # small objects: loads the kernels and fills the context's pool
-    for _, _, p in warm:
+    for _, _, p in warm + jobs:
         p.precision = abi.MIXED if args.precision == "mixed" else abi.FP64
+        p.quad_policy = abi.QUAD_BANDED if args.quad == "banded" else abi.QUAD_REFERENCE
     serve_stream(warm, [0.0, 0.0], torch, 60.0)
     rec, end = serve_stream(jobs, arrivals, torch, drain_s=30.0)
     lat = [(r["done"] if r["done"] is not None else end) - r["arrival"] for r in rec]
@@ -436,8 +442,8 @@ def serving_main(args):
                       "stored_f32": sum(r["f32"] for r in rec), "value": entries / end, "unit": "entries/s",
                       "latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3, "p99": float(np.percentile(lat, 99)) * 1e3},
                       "run_s": end, "budget_GB": SERVING_BUDGET / 1e9, "steps_per_object": SERVING_STEPS,
-                      "precision": args.precision, "dtype": "f64" if args.precision == "fp64" else "f32+f64",
-                      "data": "synthetic"}), flush=True)
+                      "precision": args.precision, "quadrature": args.quad,
+                      "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic"}), flush=True)
 
 
 def time_stepping_main(args):
@@ -610,8 +616,8 @@ def main():
 
     V, F, params, desc = workload(args.workload, args.precision)
     if args.quad == "banded":
-        if params.galerkin or args.precision != "fp64":
-            raise SystemExit("--quad banded applies to FP64 collocation workloads")
+        if params.galerkin:
+            raise SystemExit("--quad banded applies to collocation workloads")
         params.quad_policy = abi.QUAD_BANDED
     nv, L = len(V), params.L
     n3 = 3 * nv

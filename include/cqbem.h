@@ -91,7 +91,7 @@ typedef struct {
     int32_t precision;    /* CQBEM_FP64 | CQBEM_MIXED */
     int32_t which;        /* bitmask of CQBEM_G | CQBEM_H | CQBEM_B_TRANS | CQBEM_B_ANGULAR */
     int32_t storage;      /* CQBEM_STORE_F64 | CQBEM_STORE_F32: element type of the G and H outputs */
-    int32_t quad_policy;  /* CQBEM_QUAD_REFERENCE | CQBEM_QUAD_BANDED (collocation, FP64) */
+    int32_t quad_policy;  /* CQBEM_QUAD_REFERENCE | CQBEM_QUAD_BANDED (collocation; FP64 or CQBEM_MIXED) */
     int32_t far_order;    /* banded: far rule order (0: order) */
     int32_t far_subdivision; /* banded: far rule subdivision (0: 1) */
     int32_t reserved;     /* 0 */

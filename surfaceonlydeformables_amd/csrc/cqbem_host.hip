@@ -115,8 +115,9 @@ void check_inputs(const cqbem_mesh *mesh, const cqbem_params *p) {
     if (p->quad_policy != CQBEM_QUAD_REFERENCE && p->quad_policy != CQBEM_QUAD_BANDED)
         throw Err{CQBEM_ERR_ARG, "unknown quad_policy"};
     if (p->quad_policy == CQBEM_QUAD_BANDED) {
-        if (p->galerkin || p->precision != CQBEM_FP64)
-            throw Err{CQBEM_ERR_UNSUPPORTED, "CQBEM_QUAD_BANDED is implemented for FP64 collocation"};
+        // the bands apply to the pair kernel: collocation G/H, in FP64 or CQBEM_MIXED with G in that kernel
+        if (p->galerkin || (p->precision == CQBEM_MIXED && mixed_float_g()))
+            throw Err{CQBEM_ERR_UNSUPPORTED, "CQBEM_QUAD_BANDED is implemented for collocation G/H in the pair kernel"};
         if (p->far_order < 0 || p->far_order > 5 || p->far_subdivision < 0 || p->far_subdivision > 16 ||
             !(p->band_kappa >= 0))
             throw Err{CQBEM_ERR_ARG, "far rule or band threshold out of range"};

@@ -1,7 +1,8 @@
 """CQBEM_QUAD_BANDED's tile plan on the CPU (cqbem_band_plan needs no GPU): every vertex's column belongs to one
 patch; a tile is far exactly when the row vertex lies kappa x h beyond the patch's bounding sphere, so the far set
 shrinks as kappa grows, is empty for an infinite kappa and never holds the tile of a vertex's own patch; the
-default kappa is the tuned one; and the refusals (the reference rule, Galerkin, mixed precision)."""
+default kappa is the tuned one; and the refusals (the reference rule, Galerkin, the float G kernel of mixed
+precision)."""
 import numpy as np
 import pytest
 
@@ -60,10 +61,15 @@ def test_far_tiles_are_beyond_the_patch(ico4):
             assert d.min() >= kappa * e.min(), (i, p)
 
 
-def test_plan_refusals(ico4):
+def test_plan_refusals(ico4, monkeypatch):
     V, F = ico4
     mesh = abi.make_mesh(V, F)
-    for p in (params(), params(quad=abi.QUAD_BANDED, galerkin=True),
-              params(quad=abi.QUAD_BANDED, precision=abi.MIXED), params(quad=7)):
+    for p in (params(), params(quad=abi.QUAD_BANDED, galerkin=True), params(quad=7)):
         with pytest.raises(abi.CQBEMError):
             abi.band_plan(mesh, p)
+    # mixed precision keeps G and H in the pair kernel: banded; with the float G kernel it is not
+    monkeypatch.delenv("CQBEM_MIXED_G", raising=False)
+    abi.band_plan(mesh, params(quad=abi.QUAD_BANDED, precision=abi.MIXED))
+    monkeypatch.setenv("CQBEM_MIXED_G", "float")
+    with pytest.raises(abi.CQBEMError):
+        abi.band_plan(mesh, params(quad=abi.QUAD_BANDED, precision=abi.MIXED))

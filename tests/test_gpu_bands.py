@@ -102,3 +102,20 @@ def test_banded_accuracy_against_a_refined_rule(ctx):
         print(f"  {k}: vs refined rule, reference rule max {e_ref[k].max():.2e}, banded max {e_band[k].max():.2e}; "
               f"largest added error {added.max():.2e} at n={int(added.argmax())}", flush=True)
         assert added.max() <= 0.05 * e_ref[k].max(), k
+
+
+def test_banded_with_mixed_precision(ctx, monkeypatch):
+    """CQBEM_QUAD_BANDED with CQBEM_MIXED: G and H come from the same banded FP64 pair kernel (bit-identical to the
+    banded FP64 run), B from the float B kernel (bit-identical to the unbanded mixed run)."""
+    monkeypatch.delenv("CQBEM_MIXED_G", raising=False)
+    V, F = mesh3()
+    mesh = abi.make_mesh(V, F)
+    print("banded + mixed", flush=True)
+    both = ctx.assemble(mesh, params(L=16, quad=abi.QUAD_BANDED, band_kappa=1.0, precision=abi.MIXED))
+    band = ctx.assemble(mesh, params(L=16, quad=abi.QUAD_BANDED, band_kappa=1.0))
+    mixed = ctx.assemble(mesh, params(L=16, precision=abi.MIXED))
+    for k in ("G", "H"):
+        assert np.array_equal(both[k], band[k]), k
+    for k in ("B_trans", "B_angular", "B"):
+        assert np.array_equal(both[k], mixed[k]), k
+    assert not np.array_equal(both["G"], mixed["G"])  # the bands are active (kappa = 1 has far tiles here)
