@@ -168,6 +168,32 @@ int cqbem_laplace(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params
 int cqbem_split_traction_discontinuity(const double *V, int64_t nv, const int32_t *F, int64_t nf, double threshold_deg,
                                        double *V_out, int32_t *F_out, int32_t *vertex_map_inverse, int64_t *nv_out);
 
+/* Subdivision schemes of cqbem_subdivide_mesh (the reference's mesh.subdivision_scheme values "Upsample", "Loop",
+ * "FalseBarycentric"; an unknown name selects Upsample there). */
+enum cqbem_subdivision_scheme {
+    CQBEM_SUBDIV_UPSAMPLE = 0,          /* igl::upsample: four faces per face, edge midpoints */
+    CQBEM_SUBDIV_LOOP = 1,              /* igl::loop: four faces per face, Loop's stencils with Warren's weights */
+    CQBEM_SUBDIV_FALSE_BARYCENTRIC = 2  /* igl::false_barycentric_subdivision: three faces per face */
+};
+
+/* Mesh subdivision, the mesh step of PhysicsBaseMatrixCollection::load_mesh with subdivision_level > 0
+ * (src/physics_base_matrix_collection.cpp:109-140; libigl v2.4.0's upsample, loop and
+ * false_barycentric_subdivision): `levels` rounds of `scheme`.  Old vertices keep their indices, new ones follow,
+ * and the positions are bit-identical to the reference's sparse products.  In: V (nv x 3), F (nf x 3) of an
+ * edge-manifold mesh without repeated face vertices.  Out: V_out (nv_out x 3), F_out (nf_out x 3).  With V_out and
+ * F_out both NULL only the sizes are set; otherwise *nv_out and *nf_out must hold the capacities on entry. */
+int cqbem_subdivide_mesh(const double *V, int64_t nv, const int32_t *F, int64_t nf, int32_t scheme, int32_t levels,
+                         double *V_out, int32_t *F_out, int64_t *nv_out, int64_t *nf_out);
+
+/* Mesh decimation, the mesh step of PhysicsBaseMatrixCollection::load_mesh with decimation_scale < 1
+ * (src/physics_base_matrix_collection.cpp:143-151; libigl v2.4.0's igl::decimate with its default shortest-edge
+ * cost and midpoint placement, called there with max_faces = (int64)(nf * decimation_scale)): edges are collapsed,
+ * shortest first, keeping the mesh a closed 2-manifold of the same topology, until at most max_faces faces remain
+ * (at least one collapse is made) or no edge can be collapsed.  In: a closed, consistently oriented, edge-manifold
+ * mesh.  Sizes and capacities as cqbem_subdivide_mesh. */
+int cqbem_decimate_mesh(const double *V, int64_t nv, const int32_t *F, int64_t nf, int64_t max_faces, double *V_out,
+                        int32_t *F_out, int64_t *nv_out, int64_t *nf_out);
+
 /* CQ frequencies s_l = gamma(R zeta^l)/dt, l = 0..L/2, complex interleaved (G.cu:19-24,65). Returns L/2+1. */
 int cqbem_frequencies(int32_t L, double R, double dt, int32_t bdf, double *s_out);
 

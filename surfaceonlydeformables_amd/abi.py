@@ -19,6 +19,7 @@ BDF1, BDF2 = 1, 2
 FP64, MIXED = 0, 1
 STORE_F64, STORE_F32 = 0, 1
 QUAD_REFERENCE, QUAD_BANDED = 0, 1
+SUBDIV_UPSAMPLE, SUBDIV_LOOP, SUBDIV_FALSE_BARYCENTRIC = 0, 1, 2
 G, H, B_TRANS, B_ANGULAR, ALL = 1, 2, 4, 8, 15
 
 EXPORTED = [
@@ -27,6 +28,7 @@ EXPORTED = [
     "cqbem_wait", "cqbem_reserved_bytes", "cqbem_history", "cqbem_matvec", "cqbem_split_traction_discontinuity",
     "cqbem_wavelet_compress", "cqbem_csr_matvec", "cqbem_assemble_noalloc", "cqbem_submit_resident", "cqbem_release",
     "cqbem_history_f32", "cqbem_assemble_scene", "cqbem_band_plan", "cqbem_cancel",
+    "cqbem_subdivide_mesh", "cqbem_decimate_mesh",
 ]
 
 
@@ -100,6 +102,10 @@ def lib():
         L.cqbem_csr_matvec.argtypes = [C.c_void_p, C.c_int64] + [C.c_void_p] * 5
         L.cqbem_split_traction_discontinuity.argtypes = [P(C.c_double), C.c_int64, P(C.c_int32), C.c_int64, C.c_double,
                                                          P(C.c_double), P(C.c_int32), P(C.c_int32), P(C.c_int64)]
+        L.cqbem_subdivide_mesh.argtypes = [P(C.c_double), C.c_int64, P(C.c_int32), C.c_int64, C.c_int32, C.c_int32,
+                                           P(C.c_double), P(C.c_int32), P(C.c_int64), P(C.c_int64)]
+        L.cqbem_decimate_mesh.argtypes = [P(C.c_double), C.c_int64, P(C.c_int32), C.c_int64, C.c_int64,
+                                          P(C.c_double), P(C.c_int32), P(C.c_int64), P(C.c_int64)]
         _lib = L
     return _lib
 
@@ -336,3 +342,31 @@ def job_footprint(mesh, params, outputs_on_device=True) -> int:
     if rc != CQBEM_OK:
         raise CQBEMError(rc, "invalid job")
     return b.value
+
+
+def _resize_mesh(fn, what, V, F, *extra):
+    """Calls a two-pass mesh entry point (sizes, then the mesh) and returns (V_out, F_out)."""
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    F = np.ascontiguousarray(F, dtype=np.int32)
+    pd, pi = C.POINTER(C.c_double), C.POINTER(C.c_int32)
+    nv, nf = C.c_int64(0), C.c_int64(0)
+    args = (V.ctypes.data_as(pd), len(V), F.ctypes.data_as(pi), len(F), *extra)
+    rc = fn(*args, None, None, C.byref(nv), C.byref(nf))
+    if rc != CQBEM_OK:
+        raise CQBEMError(rc, what + " failed")
+    Vo, Fo = np.zeros((nv.value, 3)), np.zeros((nf.value, 3), np.int32)
+    rc = fn(*args, Vo.ctypes.data_as(pd), Fo.ctypes.data_as(pi), C.byref(nv), C.byref(nf))
+    if rc != CQBEM_OK:
+        raise CQBEMError(rc, what + " failed")
+    return Vo, Fo
+
+
+def subdivide_mesh(V, F, scheme=SUBDIV_UPSAMPLE, levels=1):
+    """`levels` rounds of subdivision (the reference's mesh.subdivision_level / subdivision_scheme step)."""
+    return _resize_mesh(lib().cqbem_subdivide_mesh, "subdivision", V, F, scheme, levels)
+
+
+def decimate_mesh(V, F, max_faces):
+    """Shortest-edge decimation to at most max_faces faces (the reference's mesh.decimation_scale step, which
+    passes max_faces = int(len(F) * decimation_scale))."""
+    return _resize_mesh(lib().cqbem_decimate_mesh, "decimation", V, F, max_faces)

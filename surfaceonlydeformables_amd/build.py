@@ -9,7 +9,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(PKG, "libcqbem.so")
 SOURCES = ["cqbem_kernels.hip", "cqbem_galerkin.hip", "cqbem_mixed.hip", "cqbem_host.hip", "cqbem_wavelet.hip", "cqbem_dense.cpp",
-           "elastodynamics_cqbem_matrix_collection.cpp", "elastodynamics_cqbem_object.cpp", "mesh_split.cpp"]
+           "elastodynamics_cqbem_matrix_collection.cpp", "elastodynamics_cqbem_object.cpp", "mesh_split.cpp",
+           "mesh_refine.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed-FP32 instructions stay disabled in all device code
 DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

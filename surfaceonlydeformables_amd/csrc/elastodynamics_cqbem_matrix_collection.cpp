@@ -823,10 +823,20 @@ bool prepare_mesh(const Json &m, std::vector<double> &v, std::vector<int32_t> &f
     }
     if (m.contains("scale"))
         for (double &x : v) x *= m["scale"].number();
-    if (m.contains("subdivision_level") && m["subdivision_level"].number() > 0)
-        return err = "mesh.subdivision_level is not supported: subdivide the mesh before loading it", false;
-    if (m.contains("decimation_scale") && m["decimation_scale"].number() < 1.0)
-        return err = "mesh.decimation_scale is not supported: decimate the mesh before loading it", false;
+    // subdivision (lines 109-140): subdivision_level rounds; an unknown scheme name means Upsample
+    if (m.contains("subdivision_level") && m["subdivision_level"].number() > 0) {
+        const std::string name = m.contains("subdivision_scheme") ? m["subdivision_scheme"].string() : "Upsample";
+        const int scheme = name == "Loop" ? CQBEM_SUBDIV_LOOP
+                           : name == "FalseBarycentric" ? CQBEM_SUBDIV_FALSE_BARYCENTRIC
+                                                        : CQBEM_SUBDIV_UPSAMPLE;
+        if (subdivide_mesh(v, f, scheme, (int)m["subdivision_level"].number(), err) != CQBEM_OK)
+            return err = "mesh subdivision: " + err, false;
+    }
+    // decimation (lines 143-151) to (Eigen::Index)(faces * decimation_scale) faces
+    if (m.contains("decimation_scale") && m["decimation_scale"].number() < 1.0) {
+        const int64_t target = (int64_t)((double)(f.size() / 3) * m["decimation_scale"].number());
+        if (decimate_mesh(v, f, target, err) != CQBEM_OK) return err = "mesh decimation: " + err, false;
+    }
     // lines 158-196: a build with OpenCCL reorders the vertices (default on); without OpenCCL the reference
     // ignores the flag.  The cache key does not record the order, so an explicit request is refused rather
     // than silently producing (or reading cached) matrices in another vertex order.

@@ -158,6 +158,10 @@ int split_traction_discontinuity(const double *V, int nv, const int32_t *F, int 
                                  std::vector<double> &V_out, std::vector<int32_t> &F_out, std::vector<int32_t> &vmap_out,
                                  std::string &err);
 
+// Subdivision and decimation (mesh_refine.cpp); return a cqbem status.  V and F are replaced by the result.
+int subdivide_mesh(std::vector<double> &V, std::vector<int32_t> &F, int scheme, int levels, std::string &err);
+int decimate_mesh(std::vector<double> &V, std::vector<int32_t> &F, int64_t max_faces, std::string &err);
+
 // The reference's compressed-matrix files (include/wavelets.hpp:451-504, SparseMatrix row-major, int32 indices):
 // int N; int64 rows, cols, nnz, outerSize, innerSize; nnz doubles; outerSize row offsets (the last, nnz, is not
 // stored); nnz column indices.  save_cpmat copies the device CSR to the host; load_cpmat checks the layout

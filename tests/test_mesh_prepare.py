@@ -1,7 +1,8 @@
 """Mesh preparation of an elastodynamicsCQBEM object (PhysicsBaseMatrixCollection::load_mesh,
 src/physics_base_matrix_collection.cpp:59-365, vertex reordering off) in the C++ facade, through the
 collection driver's --mesh mode: OBJ load, unused vertices dropped (survivors keep their order,
-lines 75-103), scale, flip_normal (line 152), traction-discontinuity split.  Host code only."""
+lines 75-103), scale, subdivision and decimation (lines 109-151), flip_normal (line 152), traction-discontinuity
+split.  Host code only."""
 import json
 import os
 import subprocess
@@ -92,15 +93,19 @@ def test_flip_then_split(tmp_path):
     assert np.array_equal(V[vm], Vs)
 
 
-@pytest.mark.parametrize("key,value", [("subdivision_level", 1), ("decimation_scale", 0.5)])
-def test_unsupported_mesh_operations_are_rejected(tmp_path, key, value):
+def test_neutral_subdivision_and_decimation_settings(tmp_path):
+    """subdivision_level 0 and decimation_scale 1.0 (the reference's defaults) leave the mesh as read; decimating an
+    open mesh is refused with a message (its surface could not be used by the BEM)."""
     V, F = M.icosphere(0, 0.3)
     obj = tmp_path / "m.obj"
     M.write_obj(str(obj), V, F)
-    got, err = prepare(tmp_path, {"file": str(obj), key: value})
-    assert got is None and key in err
-    got, err = prepare(tmp_path, {"file": str(obj), key: 0 if key == "subdivision_level" else 1.0})
+    got, err = prepare(tmp_path, {"file": str(obj), "subdivision_level": 0, "subdivision_scheme": "Loop",
+                                  "decimation_scale": 1.0})
     assert got is not None, err
+    assert np.array_equal(got[0], V) and np.array_equal(got[1], F)
+    (tmp_path / "open.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nv 1 1 0\nf 1 2 4\nf 1 4 3\n")
+    got, err = prepare(tmp_path, {"file": str(tmp_path / "open.obj"), "decimation_scale": 0.5})
+    assert got is None and "decimation" in err
 
 
 def test_bad_face_index_is_rejected(tmp_path):
