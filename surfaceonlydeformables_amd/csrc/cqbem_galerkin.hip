@@ -278,6 +278,7 @@ __global__ void __launch_bounds__(256) k_gal_b(Geom g, const double4 *__restrict
 struct Pair {
     int kind, f2;
     int j1[3], j2[3];
+    int f1;
 };
 
 // Output per pair and frequency (complex): G 81 (row 3i+p, col 3j+q), H 81, Bt 27 (row 3i+p, col q), Ba 81 (row 3i+p, col q*3+k)
@@ -582,7 +583,9 @@ __device__ __forceinline__ void quad_acc(const QuadK<MODE> &K, const int ci[3], 
     }
 }
 
-template <int MODE, int KIND>
+// HALF (coincident pairs only): regions 1, 3, 5 of the six, whose exchanged images are the other three; the
+// exchanged half is added afterwards by k_gal_coincident_sym.
+template <int MODE, int KIND, bool HALF = false>
 __global__ void __launch_bounds__(64) k_gal_sing_quad(Geom g, const Pair *__restrict__ pairs, int pair0,
                                                       const double2 *__restrict__ line, int nline,
                                                       const Freq *__restrict__ freqs, Mat m, cd *__restrict__ out) {
@@ -632,11 +635,11 @@ __global__ void __launch_bounds__(64) k_gal_sing_quad(Geom g, const Pair *__rest
         const int ia = p / (nline * nline * nline), ib = (p / (nline * nline)) % nline, ic = (p / nline) % nline, id = p % nline;
         const double e1 = line[ia].x, e2 = line[ib].x, e3 = line[ic].x, xi = line[id].x;
         const double w = line[ia].y * line[ib].y * line[ic].y * line[id].y * pw;
-        constexpr int NR = KIND == 1 ? 6 : (KIND == 2 ? 5 : 2);
+        constexpr int NR = KIND == 1 ? (HALF ? 3 : 6) : (KIND == 2 ? 5 : 2);
 #pragma unroll 1
         for (int r = 0; r < NR; r++) {
             double a0, a1, b0, b1, wt;
-            region_point<KIND>(r, e1, e2, e3, xi, w, a0, a1, b0, b1, wt);
+            region_point<KIND>(HALF ? 2 * r : r, e1, e2, e3, xi, w, a0, a1, b0, b1, wt);
             double x[3], y[3];
             for (int k = 0; k < 3; k++) x[k] = px[k] + dx0[k] * a0 + dx1[k] * a1, y[k] = px[k] + dy0[k] * b0 + dy1[k] * b1;
             const double Lx[3] = {1.0 - a0, a0 - a1, a1}, Ly[3] = {1.0 - b0, b0 - b1, b1};
@@ -674,6 +677,51 @@ __global__ void __launch_bounds__(64) k_gal_sing_quad(Geom g, const Pair *__rest
                     ob[kPairH + (3 * i + p) * 9 + 3 * j + b] = acc[s][p * 3 + b] * kInv4Pi;
                 }
             }
+    }
+}
+
+// Vertex-adjacent mirror pairs.  The rule of (f2, f1) is that of (f1, f2) with the two faces' points exchanged
+// (integrator_galerkin.hpp:190-197: region 2 is region 1 with x and y swapped; the pair list orders both
+// faces from the shared vertex), and the G kernel is even in y - x, so the G block of the mirror pair at
+// (test corner i, row p; source corner j, column b) is the primary pair's at (j, p; i, b).  One thread per
+// (mirror, frequency, entry); mirrors[k] = (mirror pair, primary pair).
+__global__ void k_gal_mirror_g(const int2 *__restrict__ mirrors, int nm, int nl, cd *__restrict__ out) {
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long long)nm * nl * 81) return;
+    const int e = (int)(t % 81);
+    const long long kl = t / 81;
+    const int l = (int)(kl % nl), k = (int)(kl / nl);
+    const int2 mp = mirrors[k];
+    const int i = e / 27, p = (e / 9) % 3, j = (e % 9) / 3, b = e % 3;
+    out[((long long)mp.x * nl + l) * kPairStride + kPairG + e] =
+        out[((long long)mp.y * nl + l) * kPairStride + kPairG + (3 * j + p) * 9 + 3 * i + b];
+}
+
+// Coincident pairs integrated over regions 1, 3, 5 only (sums S).  Regions 2, 4, 6 are those with x and y
+// exchanged (integrator_galerkin.hpp:64-92), which swaps the test and source corners, keeps the G kernel
+// (even in y - x) and negates the H kernel (odd in y - x; both points lie on the one face, so n is shared):
+//   G(i p, j b) = S(i p, j b) + S(j p, i b),   H(i p, j b) = S(i p, j b) - S(j p, i b).
+// One thread per (pair, frequency, matrix, entry pair {(i p, j b), (j p, i b)} with i <= j).
+__global__ void k_gal_coincident_sym(int np, int nl, int do_g, int do_h, cd *__restrict__ out) {
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long long)np * nl * 2 * 54) return;
+    const int u = (int)(t % 54);
+    const int mat = (int)((t / 54) % 2);
+    const long long kl = t / 108;
+    if (mat == 0 ? !do_g : !do_h) return;
+    // u = 9 c + 3 p + b over the corner pairs c: (0,0) (1,1) (2,2) (0,1) (0,2) (1,2)
+    const int c = u / 9, p = (u / 3) % 3, b = u % 3;
+    const int ci[6] = {0, 1, 2, 0, 0, 1}, cj[6] = {0, 1, 2, 1, 2, 2};
+    const int i = ci[c], j = cj[c];
+    cd *o = out + kl * kPairStride + (mat == 0 ? kPairG : kPairH);
+    const int ea = (3 * i + p) * 9 + 3 * j + b, eb = (3 * j + p) * 9 + 3 * i + b;
+    const cd sa = o[ea], sb = o[eb];
+    if (mat == 0) {
+        o[ea] = sa + sb;
+        if (eb != ea) o[eb] = sb + sa;
+    } else {
+        o[ea] = sa - sb;
+        if (eb != ea) o[eb] = sb - sa;
     }
 }
 
@@ -974,7 +1022,7 @@ void run_galerkin(const GalerkinJob &job, const int *F, const int *vmap, cd *U, 
         for (int f1 = 0; f1 < nf; f1++) {
             for (int f2 : cands[f1]) {
                 Pair p{};
-                p.f2 = f2;
+                p.f1 = f1, p.f2 = f2;
                 const int *A = F + 3 * f1, *B = F + 3 * f2;
                 if (f1 == f2) {
                     p.kind = 1;
@@ -1004,12 +1052,36 @@ void run_galerkin(const GalerkinJob &job, const int *F, const int *vmap, cd *U, 
         }
     }
     if (pairs.empty()) return;
-    std::stable_sort(pairs.begin(), pairs.end(), [](const Pair &a, const Pair &b) { return a.kind < b.kind; });
+    // by kind; vertex-adjacent pairs (f1, f2) with f1 < f2 before their mirrors (f2, f1), whose G blocks are
+    // transposes of theirs and are not integrated (quad kernels)
+    std::stable_sort(pairs.begin(), pairs.end(), [](const Pair &a, const Pair &b) {
+        const bool ma = a.kind == 3 && a.f1 > a.f2, mb = b.kind == 3 && b.f1 > b.f2;
+        return a.kind != b.kind ? a.kind < b.kind : (!ma && mb);
+    });
     int kind_off[4] = {0, 0, 0, 0};
     for (const Pair &p : pairs) kind_off[p.kind]++;
     kind_off[3] += kind_off[2] += kind_off[1];  // exclusive ends per kind: [0,k1) coincident, [k1,k2) edge, [k2,k3) vertex
+    std::vector<int2> mirrors;  // (mirror pair, primary pair)
+    int vprim = kind_off[3];    // end of the vertex-adjacent primaries
+    {
+        std::map<std::pair<int, int>, int> at;
+        for (int k = kind_off[2]; k < kind_off[3]; k++)
+            if (pairs[k].f1 < pairs[k].f2) at[{pairs[k].f1, pairs[k].f2}] = k;
+        for (int k = kind_off[2]; k < kind_off[3]; k++) {
+            const Pair &p = pairs[k];
+            if (p.f1 < p.f2) continue;
+            auto it = at.find({p.f2, p.f1});
+            const Pair *q = it == at.end() ? nullptr : &pairs[it->second];
+            // the mirror rule needs both faces ordered from the shared vertex, as the classification orders them
+            if (!q || q->j1[0] != p.j2[0] || q->j1[1] != p.j2[1] || q->j1[2] != p.j2[2] || q->j2[0] != p.j1[0] ||
+                q->j2[1] != p.j1[1] || q->j2[2] != p.j1[2])
+                throw std::runtime_error("vertex-adjacent pair without its mirror");
+            if (vprim == kind_off[3]) vprim = k;
+            mirrors.push_back(make_int2(k, it->second));
+        }
+    }
     const long long npairs = (long long)pairs.size();
-    DevVec dpairs, dscratch, dblocks, dptr, ditems, dvptr, dvitems;
+    DevVec dpairs, dscratch, dblocks, dptr, ditems, dvptr, dvitems, dmirrors;
     const Pair *dp = dpairs.up(pairs, st);
     dscratch.alloc((size_t)npairs * nl * kPairStride * sizeof(cd), "device allocation: touching-pair scratch");
     cd *scratch = (cd *)dscratch.p;
@@ -1022,13 +1094,26 @@ void run_galerkin(const GalerkinJob &job, const int *F, const int *vmap, cd *U, 
                            dp, beg[KIND - 1], job.line, job.nline, job.freqs, job.mat, scratch);
         const char *sv = std::getenv("CQBEM_GALERKIN_SINGULAR");
         if (!(sv && !std::strcmp(sv, "lane"))) {
-#define CQBEM_SINGQ(MODE, KIND)                                                                                       \
-    if (end[KIND - 1] > beg[KIND - 1])                                                                                \
-        hipLaunchKernelGGL((k_gal_sing_quad<MODE, KIND>), dim3((unsigned)(end[KIND - 1] - beg[KIND - 1]), nl), dim3(64), 0, \
+#define CQBEM_SINGQ(MODE, KIND, HALF, E)                                                                              \
+    if ((E) > beg[KIND - 1])                                                                                          \
+        hipLaunchKernelGGL((k_gal_sing_quad<MODE, KIND, HALF>), dim3((unsigned)((E) - beg[KIND - 1]), nl), dim3(64), 0, \
                            st, g, dp, beg[KIND - 1], job.line, job.nline, job.freqs, job.mat, scratch);
-            if (U) { CQBEM_SINGQ(0, 1) CQBEM_SINGQ(0, 2) CQBEM_SINGQ(0, 3) }
-            if (P) { CQBEM_SINGQ(1, 1) CQBEM_SINGQ(1, 2) CQBEM_SINGQ(1, 3) }
+            // coincident: half the regions, then the exchanged half; vertex-adjacent G: the primaries, then
+            // their mirrors' transposes
+            if (U) { CQBEM_SINGQ(0, 1, true, end[0]) CQBEM_SINGQ(0, 2, false, end[1]) CQBEM_SINGQ(0, 3, false, vprim) }
+            if (P) { CQBEM_SINGQ(1, 1, true, end[0]) CQBEM_SINGQ(1, 2, false, end[1]) CQBEM_SINGQ(1, 3, false, end[2]) }
 #undef CQBEM_SINGQ
+            if ((U || P) && end[0] > 0) {
+                const long long n = (long long)end[0] * nl * 108;
+                hipLaunchKernelGGL(k_gal_coincident_sym, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, end[0], nl,
+                                   U ? 1 : 0, P ? 1 : 0, scratch);
+            }
+            if (U && !mirrors.empty()) {
+                const int2 *dm = dmirrors.up(mirrors, st);
+                const long long n = (long long)mirrors.size() * nl * 81;
+                hipLaunchKernelGGL(k_gal_mirror_g, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dm, (int)mirrors.size(),
+                                   nl, scratch);
+            }
         } else {
             if (U) { CQBEM_SING(0, 1) CQBEM_SING(0, 2) CQBEM_SING(0, 3) }
             if (P) { CQBEM_SING(1, 1) CQBEM_SING(1, 2) CQBEM_SING(1, 3) }
