@@ -55,11 +55,12 @@ enum { CQBEM_FP64 = 0, CQBEM_MIXED = 1 };
  * jobs whose double weights exceed the memory they may hold resident (an icosphere-L5 job at L = 16 needs 242 GB
  * in double, 121 GB in float). */
 enum { CQBEM_STORE_F64 = 0, CQBEM_STORE_F32 = 1 };
-/* Regular-face quadrature of the collocation G and H (no counterpart in the reference, which applies one uniform
- * rule to every non-touching pair: integrator_collocation.hpp:6-17).  CQBEM_QUAD_REFERENCE (default): that rule
- * everywhere.  CQBEM_QUAD_BANDED: per tile (column patch, row point), the rule (far_order, far_subdivision)
- * replaces it when the row point lies at least band_kappa x h beyond the patch's bounding sphere (h: the patch's
- * largest face diameter), at every frequency alike (a rule that changed along the spectrum would make each entry's
+/* Regular-face quadrature of the collocation G, H, B_trans and B_angular (no counterpart in the reference, which
+ * applies one uniform rule to every non-touching pair: integrator_collocation.hpp:6-17).  CQBEM_QUAD_REFERENCE
+ * (default): that rule everywhere.  CQBEM_QUAD_BANDED: the rule (far_order, far_subdivision) replaces it, for G and
+ * H per tile (column patch, row point) when the row point lies at least band_kappa x h beyond the patch's bounding
+ * sphere (h: the patch's largest face diameter), and, only when band_kappa_b > 0, for B per (row point, face) when
+ * it lies band_kappa_b x h_f beyond the face's own sphere (h_f: its longest edge); at every frequency alike (a rule that changed along the spectrum would make each entry's
  * quadrature error jump between frequencies, which the inverse transform's R^-n factor amplifies).  An
  * approximation with a measured error, not bit-compatible with the reference.  Zeros select the tuned defaults
  * (far rule: order, subdivision 1). */
@@ -95,7 +96,8 @@ typedef struct {
     int32_t far_order;    /* banded: far rule order (0: order) */
     int32_t far_subdivision; /* banded: far rule subdivision (0: 1) */
     int32_t reserved;     /* 0 */
-    double band_kappa;    /* banded: distance threshold in face diameters (0: tuned default) */
+    double band_kappa;    /* banded G/H: distance threshold in patch face diameters (0: tuned default) */
+    double band_kappa_b;  /* banded B: distance threshold in the face's longest edges (0: B keeps the reference rule) */
 } cqbem_params;
 
 /* Output buffers; any may be NULL.  on_device != 0: device pointers on the context's GPU;
@@ -151,6 +153,13 @@ int cqbem_job_footprint(const cqbem_mesh *mesh, const cqbem_params *params, int 
  * H is integrated entirely in tile (i, patch_of_vertex[j]).  No counterpart in the reference: for tests and tuning. */
 int cqbem_band_plan(const cqbem_mesh *mesh, const cqbem_params *params, int32_t *patch_of_vertex, uint8_t *far,
                     int32_t *npatches);
+
+/* The face plan of CQBEM_QUAD_BANDED for the far-field B_trans / B_angular (host only, no context): far (nv x nf,
+ * row-major) whether face f takes the far rule in row vertex i's sums, at every frequency (the faces at vertex i are
+ * integrated by the singular rule whatever this says).  A face is far when the row point lies at least
+ * radius + band_kappa_b h from the mean of its corners (radius: its farthest corner; h: its longest edge).  No
+ * counterpart in the reference: for tests and tuning.  With band_kappa_b = 0 no face is far. */
+int cqbem_band_plan_faces(const cqbem_mesh *mesh, const cqbem_params *params, uint8_t *far);
 
 /* Laplace-domain matrices of one CQ frequency index l (0..L/2), complex interleaved, row-major:
  * U, P (3nv x 3nv), Bt (3nv x 3), Ba (3nv x 9).  Host pointers; any may be NULL.  Used for parity checks

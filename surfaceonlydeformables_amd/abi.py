@@ -28,7 +28,7 @@ EXPORTED = [
     "cqbem_wait", "cqbem_reserved_bytes", "cqbem_history", "cqbem_matvec", "cqbem_split_traction_discontinuity",
     "cqbem_wavelet_compress", "cqbem_csr_matvec", "cqbem_assemble_noalloc", "cqbem_submit_resident", "cqbem_release",
     "cqbem_history_f32", "cqbem_assemble_scene", "cqbem_band_plan", "cqbem_cancel",
-    "cqbem_subdivide_mesh", "cqbem_decimate_mesh",
+    "cqbem_subdivide_mesh", "cqbem_decimate_mesh", "cqbem_band_plan_faces",
 ]
 
 
@@ -43,7 +43,7 @@ class cqbem_params(C.Structure):
                 ("order", C.c_int32), ("subdivision", C.c_int32), ("galerkin", C.c_int32),
                 ("precision", C.c_int32), ("which", C.c_int32), ("storage", C.c_int32),
                 ("quad_policy", C.c_int32), ("far_order", C.c_int32), ("far_subdivision", C.c_int32),
-                ("reserved", C.c_int32), ("band_kappa", C.c_double)]
+                ("reserved", C.c_int32), ("band_kappa", C.c_double), ("band_kappa_b", C.c_double)]
 
 
 class cqbem_outputs(C.Structure):
@@ -102,6 +102,8 @@ def lib():
         L.cqbem_csr_matvec.argtypes = [C.c_void_p, C.c_int64] + [C.c_void_p] * 5
         L.cqbem_split_traction_discontinuity.argtypes = [P(C.c_double), C.c_int64, P(C.c_int32), C.c_int64, C.c_double,
                                                          P(C.c_double), P(C.c_int32), P(C.c_int32), P(C.c_int64)]
+        L.cqbem_band_plan.argtypes = [P(cqbem_mesh), P(cqbem_params), C.c_void_p, C.c_void_p, P(C.c_int32)]
+        L.cqbem_band_plan_faces.argtypes = [P(cqbem_mesh), P(cqbem_params), C.c_void_p]
         L.cqbem_subdivide_mesh.argtypes = [P(C.c_double), C.c_int64, P(C.c_int32), C.c_int64, C.c_int32, C.c_int32,
                                            P(C.c_double), P(C.c_int32), P(C.c_int64), P(C.c_int64)]
         L.cqbem_decimate_mesh.argtypes = [P(C.c_double), C.c_int64, P(C.c_int32), C.c_int64, C.c_int64,
@@ -125,9 +127,10 @@ def make_mesh(V, F, vmap=None):
 
 
 def make_params(c1, c2, rho, nu, dt, R, L, bdf=BDF2, order=5, subdivision=2, galerkin=False, precision=FP64,
-                which=ALL, storage=STORE_F64, quad=QUAD_REFERENCE, far_order=0, far_subdivision=0, band_kappa=0.0):
+                which=ALL, storage=STORE_F64, quad=QUAD_REFERENCE, far_order=0, far_subdivision=0, band_kappa=0.0,
+                band_kappa_b=0.0):
     return cqbem_params(c1, c2, rho, nu, dt, R, L, bdf, order, subdivision, int(galerkin), precision, which, storage,
-                        quad, far_order, far_subdivision, 0, band_kappa)
+                        quad, far_order, far_subdivision, 0, band_kappa, band_kappa_b)
 
 
 def split_traction_discontinuity(V, F, threshold_deg=20.0):
@@ -334,6 +337,15 @@ def band_plan(mesh, params):
     if rc != CQBEM_OK:
         raise CQBEMError(rc, "band plan refused")
     return pov, far.astype(bool)
+
+
+def band_plan_faces(mesh, params):
+    """CQBEM_QUAD_BANDED face plan of the far-field B: far (nv, nf) bool, the same at every frequency."""
+    far = np.zeros((mesh.nv, mesh.nf), np.uint8)
+    rc = lib().cqbem_band_plan_faces(C.byref(mesh), C.byref(params), far.ctypes.data)
+    if rc != CQBEM_OK:
+        raise CQBEMError(rc, "band plan refused")
+    return far.astype(bool)
 
 
 def job_footprint(mesh, params, outputs_on_device=True) -> int:

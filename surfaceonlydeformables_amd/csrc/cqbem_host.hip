@@ -119,7 +119,7 @@ void check_inputs(const cqbem_mesh *mesh, const cqbem_params *p) {
         if (p->galerkin || (p->precision == CQBEM_MIXED && mixed_float_g()))
             throw Err{CQBEM_ERR_UNSUPPORTED, "CQBEM_QUAD_BANDED is implemented for collocation G/H in the pair kernel"};
         if (p->far_order < 0 || p->far_order > 5 || p->far_subdivision < 0 || p->far_subdivision > 16 ||
-            !(p->band_kappa >= 0))
+            !(p->band_kappa >= 0) || !(p->band_kappa_b >= 0))
             throw Err{CQBEM_ERR_ARG, "far rule or band threshold out of range"};
     }
     // the device sincos reduces |Im(s) r / c| <= max|s_l| diam / min(c1, c2) exactly below kTrigArgMax
@@ -200,6 +200,9 @@ struct Prepared {
     int far_npts = 0;  // points of the far rule (unpadded)
     std::vector<double4> patch_sphere;
     std::vector<double> patch_weight;  // per patch: sum over its faces of 1 / (patches listing the face)
+    // the B kernels' face bands: the far rule's points (b0, b1, b2, w) and, per face, its bounding sphere's centre
+    // and the far distance (radius + kappa h)
+    std::vector<double4> far_quad, face_sphere;  // per patch: sum over its faces of 1 / (patches listing the face)
     std::vector<double2> line;
     std::vector<Freq> freqs;
     std::vector<double2> tab;
@@ -211,6 +214,10 @@ struct Prepared {
 // surfaceonlydeformables_amd/band_tables/): the smallest kappa whose added error stays within 5% of the reference
 // rule's largest per-weight error against a rule of twice the subdivision (3.0% for G, 0.6% for H).
 constexpr double kBandKappa = 6.0;
+// The B kernels take face bands only when asked (band_kappa_b > 0).  B_trans and B_angular are closed-surface
+// integrals that cancel to a small remainder (their reference-rule error is 50x below G's), so the far rule's
+// per-face errors, small in absolute terms, stand out against it: tuned the same way, no threshold up to 16 edge
+// lengths keeps the added error within 5% of B's reference-rule error on icosphere level 4.
 
 Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
     Prepared P;
@@ -386,6 +393,22 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
                 // faces outside it (such a row is then never far from its own patch, whatever kappa)
                 P.patch_sphere.push_back(make_double4(c[0], c[1], c[2], rad * (1 + 1e-12) + kappa * h));
             }
+            // per face: the same construction (centre: mean of its corners; radius: the farthest corner; h: its
+            // longest edge), for the B kernels, which sum each row over the faces
+            const double kappa_b = p->band_kappa_b;
+            for (int f = 0; f < P.nf && kappa_b > 0; f++) {
+                double c[3] = {0, 0, 0};
+                for (int k = 0; k < 3; k++)
+                    for (int d = 0; d < 3; d++) c[d] += V[3 * F[3 * f + k] + d];
+                for (double &x : c) x *= 1.0 / 3.0;
+                double rad = 0, h = 0;
+                for (int k = 0; k < 3; k++) {
+                    const V3 a = vtx(V, F[3 * f + k]), b = vtx(V, F[3 * f + (k + 1) % 3]);
+                    rad = std::max(rad, nrm(sub(a, V3{c[0], c[1], c[2]})));
+                    h = std::max(h, nrm(sub(a, b)));
+                }
+                P.face_sphere.push_back(make_double4(c[0], c[1], c[2], rad * (1 + 1e-12) + kappa_b * h));
+            }
             std::vector<int> mult(P.nf, 0);
             for (const auto &l : lists)
                 for (int f : l) mult[f]++;
@@ -411,6 +434,7 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
                                                     p->far_subdivision > 0 ? p->far_subdivision : 1))
             fq.push_back(make_double4(t.b[0], t.b[1], t.b[2], t.w));
         P.far_npts = (int)fq.size();
+        P.far_quad = fq;
         if (fq.size() & 1) fq.push_back(make_double4(1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 0.0));
         for (size_t q = 0; q < fq.size(); q++) {
             const double4 &Q = fq[q], &Qp = fq[q ^ 1];
@@ -440,7 +464,7 @@ Prepared prepare(const cqbem_mesh *mesh, const cqbem_params *p) {
 struct DevMesh {
     DBuf V, F, N, area, vs_ptr, vs_face, sg_ptr, sg_face, vmap, quad, line, freqs, tab;
     DBuf pair_patch_pass, pair_pass_info, pair_pass_faces, pair_patch_verts, pair_pts;
-    DBuf far_pts, patch_sphere;  // CQBEM_QUAD_BANDED
+    DBuf far_pts, patch_sphere, far_quad, face_sphere;  // CQBEM_QUAD_BANDED
     Geom g;
 };
 
@@ -469,13 +493,25 @@ Bands bands_of(const DevMesh &d, const Prepared &P) {
     return b;
 }
 
+// The B kernels' face bands of a prepared job (empty unless CQBEM_QUAD_BANDED with band_kappa_b > 0).
+FaceBands fbands_of(const DevMesh &d, const Prepared &P) {
+    FaceBands b;
+    if (P.face_sphere.empty()) return b;
+    b.far_quad = d.far_quad.as<double4>(), b.far_npts = P.far_npts, b.sphere = d.face_sphere.as<double4>();
+    return b;
+}
+
 // Host replica of the pair kernel's per-tile rule choice (CQBEM_QUAD_BANDED): whether tile (patch pc, row point x)
 // takes the far rule.
 bool tile_is_far(const Prepared &P, int pc, const double *x) {
     if (P.far_pair_pts.empty()) return false;
-    const double4 S = P.patch_sphere[pc];
-    const double dx = x[0] - S.x, dy = x[1] - S.y, dz = x[2] - S.z;
-    return dx * dx + dy * dy + dz * dz >= S.w * S.w;
+    return beyond_sphere(x[0], x[1], x[2], P.patch_sphere[pc]);
+}
+
+// Host replica of the B kernels' per-face rule choice: whether face f takes the far rule for row point x.
+bool face_is_far(const Prepared &P, int f, const double *x) {
+    if (P.face_sphere.empty()) return false;
+    return beyond_sphere(x[0], x[1], x[2], P.face_sphere[f]);
 }
 
 // Regular-face evaluations of row vertex i over all frequencies (each face and point once, whichever patches list
@@ -513,8 +549,10 @@ void upload_mesh(DevMesh &d, const cqbem_mesh *mesh, const Prepared &P, hipStrea
     upload(d.pair_patch_pass, P.pair_patch_pass, st), upload(d.pair_pass_info, P.pair_pass_info, st);
     upload(d.pair_pass_faces, P.pair_pass_faces, st), upload(d.pair_patch_verts, P.pair_patch_verts, st);
     upload(d.pair_pts, P.pair_pts, st);
-    if (!P.far_pair_pts.empty())
+    if (!P.far_pair_pts.empty()) {
         upload(d.far_pts, P.far_pair_pts, st), upload(d.patch_sphere, P.patch_sphere, st);
+        if (!P.face_sphere.empty()) upload(d.far_quad, P.far_quad, st), upload(d.face_sphere, P.face_sphere, st);
+    }
     Geom &g = d.g;
     g.V = d.V.as<double>(), g.F = d.F.as<int>(), g.N = d.N.as<double>(), g.area = d.area.as<double>();
     g.vs_ptr = d.vs_ptr.as<int>(), g.vs_face = d.vs_face.as<int>(), g.sg_ptr = d.sg_ptr.as<int>(), g.sg_face = d.sg_face.as<int>();
@@ -712,9 +750,11 @@ void run_job(cqbem_context *ctx, hipStream_t st, const cqbem_mesh *mesh, const c
             {
                 Timer t(st, &tm[2]);
                 if (mixed)
-                    HIPCHK(launch_colloc_b32(g, quad, npts, freqs32, 0, Lf, mat32, 0, nv, specB.as<cd>(), specBa.as<cd>(), st));
+                    HIPCHK(launch_colloc_b32(g, quad, npts, freqs32, 0, Lf, mat32, 0, nv, specB.as<cd>(), specBa.as<cd>(), st,
+                                             fbands_of(d, P)));
                 else
-                    HIPCHK(launch_colloc_b(g, quad, npts, freqs, 0, Lf, P.mat, 0, nv, specB.as<cd>(), specBa.as<cd>(), st));
+                    HIPCHK(launch_colloc_b(g, quad, npts, freqs, 0, Lf, P.mat, 0, nv, specB.as<cd>(), specBa.as<cd>(), st,
+                                           fbands_of(d, P)));
             }
             {
                 Timer t(st, &tm[3]);
@@ -1109,10 +1149,10 @@ void run_scene(cqbem_context *ctx, hipStream_t st, int n, const cqbem_mesh *mesh
             if (!o->wantB) continue;
             if (o->mixed)
                 HIPCHK(launch_colloc_b32(g, o->d.quad.as<double4>(), o->P.npts, o->dfreq32.as<FreqF>(), 0, Lf, o->mat32, 0, nv,
-                                         o->specB.as<cd>(), o->specBa.as<cd>(), st));
+                                         o->specB.as<cd>(), o->specBa.as<cd>(), st, fbands_of(o->d, o->P)));
             else
                 HIPCHK(launch_colloc_b(g, o->d.quad.as<double4>(), o->P.npts, freqs, 0, Lf, o->P.mat, 0, nv, o->specB.as<cd>(),
-                                       o->specBa.as<cd>(), st));
+                                       o->specBa.as<cd>(), st, fbands_of(o->d, o->P)));
             HIPCHK(launch_colloc_singular(1, g, o->d.line.as<double2>(), (int)o->P.line.size(), freqs, 0, Lf, o->P.mat, 0, nv,
                                           nullptr, n3, 0, o->specB.as<cd>(), o->specBa.as<cd>(), st));
             const double2 *tab = o->d.tab.as<double2>();
@@ -1322,6 +1362,22 @@ int cqbem_band_plan(const cqbem_mesh *mesh, const cqbem_params *params, int32_t 
     return CQBEM_OK;
 }
 
+int cqbem_band_plan_faces(const cqbem_mesh *mesh, const cqbem_params *params, uint8_t *far) {
+    try {
+        if (!mesh || !params || !far) throw Err{CQBEM_ERR_ARG, "null argument"};
+        check_inputs(mesh, params);
+        if (params->quad_policy != CQBEM_QUAD_BANDED) throw Err{CQBEM_ERR_ARG, "quad_policy is not CQBEM_QUAD_BANDED"};
+        const Prepared P = prepare(mesh, params);
+        for (int i = 0; i < P.nv; i++)
+            for (int f = 0; f < P.nf; f++) far[(size_t)i * P.nf + f] = face_is_far(P, f, mesh->V + 3 * i) ? 1 : 0;
+    } catch (const Err &e) {
+        return e.code;
+    } catch (const std::exception &) {
+        return CQBEM_ERR_ARG;
+    }
+    return CQBEM_OK;
+}
+
 int cqbem_job_footprint(const cqbem_mesh *mesh, const cqbem_params *params, int outputs_on_device, uint64_t *bytes) {
     try {
         check_inputs(mesh, params);
@@ -1474,9 +1530,11 @@ int cqbem_laplace(cqbem_context *ctx, const cqbem_mesh *mesh, const cqbem_params
             }
             if (wB) {
                 if (mixed)
-                    HIPCHK(launch_colloc_b32(d.g, quad, npts, dfreq32.as<FreqF>(), l, 1, mat32, 0, nv, dBt.as<cd>(), dBa.as<cd>(), st));
+                    HIPCHK(launch_colloc_b32(d.g, quad, npts, dfreq32.as<FreqF>(), l, 1, mat32, 0, nv, dBt.as<cd>(), dBa.as<cd>(), st,
+                                             fbands_of(d, P)));
                 else
-                    HIPCHK(launch_colloc_b(d.g, quad, npts, freqs, l, 1, P.mat, 0, nv, dBt.as<cd>(), dBa.as<cd>(), st));
+                    HIPCHK(launch_colloc_b(d.g, quad, npts, freqs, l, 1, P.mat, 0, nv, dBt.as<cd>(), dBa.as<cd>(), st,
+                                           fbands_of(d, P)));
                 HIPCHK(launch_colloc_singular(1, d.g, line, nline, freqs, l, 1, P.mat, 0, nv, nullptr, n3, 0, dBt.as<cd>(),
                                               dBa.as<cd>(), st));
             }

@@ -186,9 +186,7 @@ __device__ __attribute__((always_inline)) inline void pair_tile(const Geom &g, c
         x0 = g.V[3 * i], x1 = g.V[3 * i + 1], x2 = g.V[3 * i + 2];
     }
     if (bands.far_pts) {  // the tile's rule: block-uniform (patch and row are)
-        const double4 S = bands.sphere[pc];
-        const double dx = x0 - S.x, dy = x1 - S.y, dz = x2 - S.z;
-        if (__builtin_amdgcn_readfirstlane((int)(dx * dx + dy * dy + dz * dz >= S.w * S.w)))
+        if (__builtin_amdgcn_readfirstlane((int)beyond_sphere(x0, x1, x2, bands.sphere[pc])))
             pts = bands.far_pts, nrounds = bands.far_rounds;
     }
     for (int k = tid; k < kPairWords * kPairPV; k += kPairThreads) acc[k] = 0.0;
@@ -433,7 +431,7 @@ template <bool GAL>
 __global__ void __launch_bounds__(256) k_colloc_b(Geom g, const double4 *__restrict__ quad, int npts,
                                                    const Freq *__restrict__ freqs, int l0, Mat m, int row0,
                                                    const double4 *__restrict__ tpts, cd *__restrict__ Bt,
-                                                   cd *__restrict__ Ba, long long nv3) {
+                                                   cd *__restrict__ Ba, long long nv3, FaceBands fb) {
     __shared__ double red[4][kBWords];
     __shared__ MathTables mtab;  // tables of the exponentials
     load_math_tables(&mtab);
@@ -466,11 +464,14 @@ __global__ void __launch_bounds__(256) k_colloc_b(Geom g, const double4 *__restr
         const double *p0 = g.V + 3 * a0, *p1 = g.V + 3 * a1, *p2 = g.V + 3 * a2;
         const double n[3] = {g.N[3 * fc], g.N[3 * fc + 1], g.N[3 * fc + 2]};
         const double ar = g.area[fc];
+        const double4 *qf = quad;  // the face's rule
+        int nq = npts;
+        if (fb.sphere && beyond_sphere(x0, x1, x2, fb.sphere[fc])) qf = fb.far_quad, nq = fb.far_npts;
         cd mt[3] = {{0, 0}, {0, 0}, {0, 0}}, m2 = {0, 0}, M[6], S = {0, 0}, S3 = {0, 0};
 #pragma unroll
         for (int k = 0; k < 6; k++) M[k] = cd{0, 0};
-        for (int q = 0; q < npts; ++q) {
-            const double4 Q = quad[q];
+        for (int q = 0; q < nq; ++q) {
+            const double4 Q = qf[q];
             const double r0 = Q.x * p0[0] + Q.y * p1[0] + Q.z * p2[0] - x0;
             const double r1 = Q.x * p0[1] + Q.y * p1[1] + Q.z * p2[1] - x1;
             const double r2 = Q.x * p0[2] + Q.y * p1[2] + Q.z * p2[2] - x2;
@@ -1174,10 +1175,10 @@ int pair_patch_size() { return kPairPV; }
 int pair_pass_faces() { return kPairFaces; }
 
 hipError_t launch_colloc_b(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
-                           int row0, int nrows, cd *Bt, cd *Ba, hipStream_t st) {
+                           int row0, int nrows, cd *Bt, cd *Ba, hipStream_t st, const FaceBands &fb) {
     if (nrows <= 0 || nrows > 65535) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_colloc_b<false>, dim3(nl, nrows), dim3(256), 0, st, g, quad, npts, freqs, l0, m, row0,
-                       (const double4 *)nullptr, Bt, Ba, 3LL * g.nv);
+                       (const double4 *)nullptr, Bt, Ba, 3LL * g.nv, fb);
     return hipGetLastError();
 }
 
@@ -1185,7 +1186,7 @@ hipError_t launch_points_b(const Geom &g, const double4 *quad, int npts, const F
                            const double4 *tpts, int row0, int nrows, cd *Bt, cd *Ba, long long out_rows, hipStream_t st) {
     if (nrows <= 0 || nrows > 65535 || 3LL * nrows > out_rows) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_colloc_b<true>, dim3(nl, nrows), dim3(256), 0, st, g, quad, npts, freqs, l0, m, row0, tpts, Bt, Ba,
-                       out_rows);
+                       out_rows, FaceBands{});
     return hipGetLastError();
 }
 

@@ -31,6 +31,21 @@ struct Bands {
     int far_rounds = 0;                // its padded points / 2
     const double4 *sphere = nullptr;   // per patch: centre, radius + kappa h
 };
+// The bands' rule choice: whether row point x lies at or beyond distance S.w from (S.x, S.y, S.z).  Compiled without
+// contraction, so that the host plans and the kernels decide every tile and face alike.
+__host__ __device__ inline bool beyond_sphere(double x0, double x1, double x2, const double4 &S) {
+#pragma clang fp contract(off)
+    const double dx = x0 - S.x, dy = x1 - S.y, dz = x2 - S.z;
+    return dx * dx + dy * dy + dz * dz >= S.w * S.w;
+}
+// Distance bands of the B kernels' regular-face rule (CQBEM_QUAD_BANDED).  Face f takes the far rule for row point x
+// at every frequency when |x - centre_f| >= radius_f + kappa h_f (h_f: its longest edge); otherwise the reference
+// rule.  sphere == nullptr: the reference rule for every face.
+struct FaceBands {
+    const double4 *far_quad = nullptr;  // the far rule, (b0, b1, b2, w) per point
+    int far_npts = 0;
+    const double4 *sphere = nullptr;    // per face: centre, radius + kappa h
+};
 // Pair form of the G/H regular-face kernel (both matrices): per patch a range of passes
 // [patch_pass[p], patch_pass[p+1]); per pass (first face entry, nfaces | nsub << 16); per face entry
 // (face, corner 0, corner 1, corner 2) with corner = slot | sub-step << 10 or -1 outside the patch.
@@ -69,7 +84,7 @@ struct ScenePiece {
 hipError_t launch_scene_gh_pair(const ScenePiece *pieces, int npieces, long long ntiles, unsigned *counter, bool G, bool H,
                                 int num_cus, hipStream_t st);
 hipError_t launch_colloc_b(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,
-                           int row0, int nrows, cd *Bt, cd *Ba, hipStream_t st);
+                           int row0, int nrows, cd *Bt, cd *Ba, hipStream_t st, const FaceBands &fb = FaceBands{});
 // B_trans / B_angular rows of the test points tpts[row0..row0+nrows-1] (Galerkin non-singular set, as above), into
 // rows 3r..3r+2 of Bt (out_rows x 3 per frequency) and Ba (out_rows x 9).
 hipError_t launch_points_b(const Geom &g, const double4 *quad, int npts, const Freq *freqs, int l0, int nl, const Mat &m,

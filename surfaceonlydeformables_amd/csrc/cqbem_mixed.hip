@@ -145,22 +145,23 @@ template <bool GAL>
 __global__ void __launch_bounds__(256) k_colloc_b32(Geom g, const double4 *__restrict__ quad, int npts,
                                                      const FreqF *__restrict__ freqs, int l0, MatF m, int row0,
                                                      const double4 *__restrict__ tpts, cd *__restrict__ Bt,
-                                                     cd *__restrict__ Ba, long long nv3) {
+                                                     cd *__restrict__ Ba, long long nv3, FaceBands fb) {
     __shared__ double red[4][kBWords32];
     const int l = blockIdx.x;
     const int i = row0 + blockIdx.y;
     const FreqF f = freqs[l0 + l];
     const cf is{f.isx, f.isy}, is2{f.is2x, f.is2y};
-    float x0, x1, x2;
+    double xd[3];  // the row point (double: the face bands decide as the host plan does)
     int t[3] = {i, i, i};
     if (GAL) {
         const double4 T = tpts[i];
-        x0 = (float)T.x, x1 = (float)T.y, x2 = (float)T.z;
+        xd[0] = T.x, xd[1] = T.y, xd[2] = T.z;
         const int f1 = (int)T.w;
         t[0] = g.F[3 * f1], t[1] = g.F[3 * f1 + 1], t[2] = g.F[3 * f1 + 2];
     } else {
-        x0 = (float)g.V[3 * i], x1 = (float)g.V[3 * i + 1], x2 = (float)g.V[3 * i + 2];
+        xd[0] = g.V[3 * i], xd[1] = g.V[3 * i + 1], xd[2] = g.V[3 * i + 2];
     }
+    const float x0 = (float)xd[0], x1 = (float)xd[1], x2 = (float)xd[2];
     const int io = GAL ? (int)blockIdx.y : i;
     constexpr float kI8 = 0.0397887358f, kI4 = 0.0795774715f;  // 1/(8 pi), 1/(4 pi)
     KahanC bt[6], ba[18];
@@ -175,11 +176,14 @@ __global__ void __launch_bounds__(256) k_colloc_b32(Geom g, const double4 *__res
         const float ar = (float)g.area[fc];
         const float y0[3] = {(float)p0[0], (float)p0[1], (float)p0[2]}, y1[3] = {(float)p1[0], (float)p1[1], (float)p1[2]},
                     y2[3] = {(float)p2[0], (float)p2[1], (float)p2[2]};
+        const double4 *qf = quad;  // the face's rule
+        int nq = npts;
+        if (fb.sphere && beyond_sphere(xd[0], xd[1], xd[2], fb.sphere[fc])) qf = fb.far_quad, nq = fb.far_npts;
         cf mt[3] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}}, m2 = {0.f, 0.f}, M[6], S = {0.f, 0.f}, S3 = {0.f, 0.f};
 #pragma unroll
         for (int k = 0; k < 6; k++) M[k] = cf{0.f, 0.f};
-        for (int q = 0; q < npts; ++q) {
-            const double4 Q = quad[q];
+        for (int q = 0; q < nq; ++q) {
+            const double4 Q = qf[q];
             const float b0 = (float)Q.x, b1 = (float)Q.y, b2 = (float)Q.z;
             const float r0 = b0 * y0[0] + b1 * y1[0] + b2 * y2[0] - x0;
             const float r1 = b0 * y0[1] + b1 * y1[1] + b2 * y2[1] - x1;
@@ -271,10 +275,10 @@ hipError_t launch_colloc_g32(const Geom &g, const double4 *quad, int npts, const
 }
 
 hipError_t launch_colloc_b32(const Geom &g, const double4 *quad, int npts, const FreqF *freqs, int l0, int nl, const MatF &m,
-                             int row0, int nrows, cd *Bt, cd *Ba, hipStream_t st) {
+                             int row0, int nrows, cd *Bt, cd *Ba, hipStream_t st, const FaceBands &fb) {
     if (nrows <= 0 || nl <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_colloc_b32<false>, dim3(nl, nrows), dim3(256), 0, st, g, quad, npts, freqs, l0, m, row0,
-                       (const double4 *)nullptr, Bt, Ba, (long long)3 * g.nv);
+                       (const double4 *)nullptr, Bt, Ba, (long long)3 * g.nv, fb);
     return hipGetLastError();
 }
 
@@ -282,7 +286,7 @@ hipError_t launch_points_b32(const Geom &g, const double4 *quad, int npts, const
                              const double4 *tpts, int row0, int nrows, cd *Bt, cd *Ba, long long out_rows, hipStream_t st) {
     if (nrows <= 0 || nrows > 65535 || 3LL * nrows > out_rows) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_colloc_b32<true>, dim3(nl, nrows), dim3(256), 0, st, g, quad, npts, freqs, l0, m, row0, tpts, Bt,
-                       Ba, out_rows);
+                       Ba, out_rows, FaceBands{});
     return hipGetLastError();
 }
 

@@ -26,7 +26,7 @@ hipError_t launch_colloc_g32(const Geom &g, const double4 *quad, int npts, const
                              int row0, int nrows, cd *UG, long long ld, long long lstride, hipStream_t st);
 // B_trans / B_angular far field of collocation rows, written (as double) like launch_colloc_b.
 hipError_t launch_colloc_b32(const Geom &g, const double4 *quad, int npts, const FreqF *freqs, int l0, int nl, const MatF &m,
-                             int row0, int nrows, cd *Bt, cd *Ba, hipStream_t st);
+                             int row0, int nrows, cd *Bt, cd *Ba, hipStream_t st, const FaceBands &fb = FaceBands{});
 // The same at Galerkin test points (as launch_points_b): rows tpts[row0 .. row0 + nrows - 1], out_rows rows per
 // frequency in Bt/Ba.
 hipError_t launch_points_b32(const Geom &g, const double4 *quad, int npts, const FreqF *freqs, int l0, int nl, const MatF &m,

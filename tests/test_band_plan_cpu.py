@@ -61,6 +61,32 @@ def test_far_tiles_are_beyond_the_patch(ico4):
             assert d.min() >= kappa * e.min(), (i, p)
 
 
+def test_face_plan_is_the_face_sphere_test(ico4):
+    """The B face plan: face f is far for row vertex i exactly when |x_i - c_f| >= r_f (1 + 1e-12) + kappa h_f (c_f:
+    the mean of its corners, r_f: its farthest corner, h_f: its longest edge); checked outside a 1e-9 rounding margin.
+    A face at vertex i is never far; the far set shrinks as kappa grows and is empty for an infinite kappa."""
+    V, F = ico4
+    mesh = abi.make_mesh(V, F)
+    kappa = 2.0
+    far = abi.band_plan_faces(mesh, params(quad=abi.QUAD_BANDED, band_kappa_b=kappa))
+    assert far.shape == (len(V), len(F))
+    c = V[F].mean(axis=1)
+    r = np.max(np.linalg.norm(V[F] - c[:, None, :], axis=2), axis=1)
+    h = np.max(np.stack([np.linalg.norm(V[F[:, k]] - V[F[:, (k + 1) % 3]], axis=1) for k in range(3)]), axis=0)
+    thr = r * (1 + 1e-12) + kappa * h
+    for i in range(0, len(V), 7):
+        d = np.linalg.norm(c - V[i], axis=1)
+        assert far[i][d >= thr * (1 + 1e-9)].all() and not far[i][d <= thr * (1 - 1e-9)].any(), i
+    for k in range(3):
+        assert not far[F[:, k], np.arange(len(F))].any()
+    fractions = [abi.band_plan_faces(mesh, params(quad=abi.QUAD_BANDED, band_kappa_b=k)).mean()
+                 for k in (1e-300, 1.0, 6.0)]
+    assert fractions[0] > fractions[1] > fractions[2] > 0
+    assert not abi.band_plan_faces(mesh, params(quad=abi.QUAD_BANDED, band_kappa_b=1e300)).any()
+    # kappa_b alone sets it: the G/H threshold does not
+    assert np.array_equal(abi.band_plan_faces(mesh, params(quad=abi.QUAD_BANDED, band_kappa=0.5, band_kappa_b=2.0)), far)
+
+
 def test_plan_refusals(ico4, monkeypatch):
     V, F = ico4
     mesh = abi.make_mesh(V, F)
