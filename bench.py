@@ -701,10 +701,52 @@ def main():
             roof["kernel"] = "k_colloc_gh_pair<false,true,false> + k_colloc_g32"
         elif roof:
             roof["note"] = "mixed: G and H in the double pair kernel, the far-field B in float"
+    if args.precision == "fp64" and args.quad == "reference" and not params.galerkin:
+        # the metric's comparison, in the same run after the timed FP64 steps: the mixed-precision job, alone and
+        # with the distance-banded rule, against the FP64 outputs just assembled
+        line["mixed_vs_fp64"] = variant_against_fp64(ctx, mesh, params, outs, ms_per_step, torch, abi.MIXED, False)
+        line["mixed_banded_vs_fp64"] = variant_against_fp64(ctx, mesh, params, outs, ms_per_step, torch, abi.MIXED,
+                                                            True)
     if not args.no_cpu_baseline and not params.galerkin:
         line["cpu_baseline"] = cpu_baseline(V, F, params)
     ctx.close()
     print(json.dumps(line), flush=True)
+
+
+def variant_against_fp64(ctx, mesh, params, outs, fp64_ms, torch, precision, banded, steps=3):
+    """The FP64 job again as a variant (precision, distance-banded rule or not): one warm-up and `steps` timed
+    assemblies into outputs of its own, then the relative Frobenius error of every output against the FP64 outputs
+    `outs`, over the whole stack and per weight n (norms on the device)."""
+    from surfaceonlydeformables_amd import abi
+    pv = abi.make_params(params.c1, params.c2, params.rho, params.nu, params.dt, params.R, params.L, bdf=params.bdf,
+                         order=params.order, subdivision=params.subdivision, galerkin=params.galerkin,
+                         which=params.which, precision=precision,
+                         quad=abi.QUAD_BANDED if banded else abi.QUAD_REFERENCE)
+    var = {k: torch.empty_like(t) for k, t in outs.items()}
+    ptrs = {k: t.data_ptr() for k, t in var.items()}
+    torch.cuda.synchronize()
+    ctx.assemble_device(mesh, pv, ptrs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.assemble_device(mesh, pv, ptrs)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    stack, worst = {}, {}
+    for k in outs:
+        a, b = var[k].reshape(params.L, -1), outs[k].reshape(params.L, -1)
+        pw = [float(torch.linalg.norm(a[n] - b[n]) / torch.linalg.norm(b[n])) for n in range(params.L)]
+        stack[k] = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+        worst[k] = max(pw)
+    del var
+    torch.cuda.empty_cache()
+    return {"ms_per_step": ms, "speedup_vs_fp64": fp64_ms / ms,
+            "gh_entries_per_s": 2.0 * (3 * mesh.nv) ** 2 * params.L * 1e3 / ms,
+            "rel_err_stack": stack, "rel_err_max_per_weight": worst,
+            "gh_within_1e-6": stack["G"] <= 1e-6 and stack["H"] <= 1e-6,
+            "note": ("far-field B_trans and B_angular in float (Kahan-compensated face sums); G and H in the double "
+                     "pair kernel" + ("; G/H tiles beyond the band threshold on the far rule" if banded else "")
+                     + "; H, singular faces and the inverse transform in double")}
 
 
 def mixed_against_fp64(ctx, mesh, params, outs, mixed_ms, torch, dev, steps=3):
