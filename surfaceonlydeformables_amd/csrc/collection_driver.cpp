@@ -117,6 +117,7 @@ int main(int argc, char **argv) {
         std::ofstream o(pre + "_post_" + name + ".bin", std::ios::binary);
         o.write(reinterpret_cast<const char *>(v.data()), v.size() * sizeof(double));
     };
+    dump1("V", col.V);  // the assembled vertices: after the mesh steps, centred
     dump1("u_p", col.u_p.data), dump1("u_p_diag", col.u_p_diag.data);
     dump1("H_inv_B_trans", col.H_inv_B_trans.data), dump1("H_inv_B_euler", col.H_inv_B_euler.data);
     dump1("B_trans_inv_H", col.B_trans_inv_H.data), dump1("B_euler_inv_H", col.B_euler_inv_H.data);

@@ -289,7 +289,10 @@ template <int MODE> struct SingAcc {
     cd v[MODE == 0 ? 63 : (MODE == 1 ? 81 : 36)];
 };
 
-template <int MODE>
+// MIRROR (B of a coincident pair integrated over regions 1, 3, 5): the point also stands for its exchanged image,
+// whose test weight is Ly; the odd terms (t1 rh, t2 rh.n) change sign there and the even ones (u1 rh rh, u12, u3) do
+// not, so they take the weights wt (Lx - Ly) and wt (Lx + Ly).
+template <int MODE, bool MIRROR = false>
 __device__ __forceinline__ void sing_eval(const double x[3], const double y[3], const double Lx[3], const double Ly[3], double wt,
                                  const double n[3], const Freq &f, const Mat &m, const MathTables *M, cd *acc) {
     const double r0 = y[0] - x[0], r1 = y[1] - x[1], r2 = y[2] - x[2];
@@ -343,18 +346,19 @@ __device__ __forceinline__ void sing_eval(const double x[3], const double y[3], 
         const double rn = h[0] * n[0] + h[1] * n[1] + h[2] * n[2];
         #pragma unroll
         for (int i = 0; i < 3; i++) {
-            const double w = wt * Lx[i];
+            const double wo = MIRROR ? wt * (Lx[i] - Ly[i]) : wt * Lx[i];  // odd terms
+            const double we = MIRROR ? wt * (Lx[i] + Ly[i]) : wt * Lx[i];  // even terms
             cd *a = acc + i * 12;
-            const cd tw = t1 * w, uw = u1 * w;
+            const cd tw = t1 * wo, uw = u1 * we;
             #pragma unroll
             for (int p = 0; p < 3; p++) fma_cr(a[p], tw, h[p]);
-            fma_cr(a[3], t2, w * rn);
+            fma_cr(a[3], t2, wo * rn);
             #pragma unroll
             for (int p = 0; p < 3; p++)
                 #pragma unroll
                 for (int b = p; b < 3; b++) fma_cr(a[4 + sidx(p, b)], uw, h[p] * h[b]);
-            fma_cr(a[10], u12, w);
-            fma_cr(a[11], u3, w);
+            fma_cr(a[10], u12, we);
+            fma_cr(a[11], u3, we);
         }
     }
 }
@@ -391,7 +395,8 @@ __device__ __forceinline__ void region_point(int r, double e1, double e2, double
     }
 }
 
-template <int MODE, int KIND>
+// HALF (B of coincident pairs): regions 1, 3, 5 only, each point standing for its exchanged image as well.
+template <int MODE, int KIND, bool HALF = false>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) k_gal_sing(Geom g, const Pair *__restrict__ pairs, int pair0,
                                                   const double2 *__restrict__ line, int nline,
                                                   const Freq *__restrict__ freqs, Mat m, cd *__restrict__ out) {
@@ -433,16 +438,17 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
         const int ia = p / (nline * nline * nline), ib = (p / (nline * nline)) % nline, ic = (p / nline) % nline, id = p % nline;
         const double e1 = line[ia].x, e2 = line[ib].x, e3 = line[ic].x, xi = line[id].x;
         const double w = line[ia].y * line[ib].y * line[ic].y * line[id].y;
-        constexpr int NR = KIND == 1 ? 6 : (KIND == 2 ? 5 : 2);
+        static_assert(!HALF || (MODE == 2 && KIND == 1), "the exchanged half is folded in for B of coincident pairs");
+        constexpr int NR = KIND == 1 ? (HALF ? 3 : 6) : (KIND == 2 ? 5 : 2);
 #pragma unroll 1
         for (int r = 0; r < NR; r++) {
             double a0, a1, b0, b1, wt;
-            region_point<KIND>(r, e1, e2, e3, xi, w, a0, a1, b0, b1, wt);
+            region_point<KIND>(HALF ? 2 * r : r, e1, e2, e3, xi, w, a0, a1, b0, b1, wt);
             double x[3], y[3];
             #pragma unroll
             for (int k = 0; k < 3; k++) x[k] = px[k] + dx0[k] * a0 + dx1[k] * a1, y[k] = px[k] + dy0[k] * b0 + dy1[k] * b1;
             const double Lx[3] = {1.0 - a0, a0 - a1, a1}, Ly[3] = {1.0 - b0, b0 - b1, b1};
-            sing_eval<MODE>(x, y, Lx, Ly, wt, n, f, m, &mtab, acc);
+            sing_eval<MODE, HALF>(x, y, Lx, Ly, wt, n, f, m, &mtab, acc);
         }
     }
     // wave reduction; lane 0 expands and writes
@@ -1093,7 +1099,8 @@ void run_galerkin(const GalerkinJob &job, const int *F, const int *vmap, cd *U, 
         hipLaunchKernelGGL((k_gal_sing<MODE, KIND>), dim3((unsigned)(end[KIND - 1] - beg[KIND - 1]), nl), dim3(64), 0, st, g, \
                            dp, beg[KIND - 1], job.line, job.nline, job.freqs, job.mat, scratch);
         const char *sv = std::getenv("CQBEM_GALERKIN_SINGULAR");
-        if (!(sv && !std::strcmp(sv, "lane"))) {
+        const bool lane = sv && !std::strcmp(sv, "lane");  // every region of every ordered pair integrated
+        if (!lane) {
 #define CQBEM_SINGQ(MODE, KIND, HALF, E)                                                                              \
     if ((E) > beg[KIND - 1])                                                                                          \
         hipLaunchKernelGGL((k_gal_sing_quad<MODE, KIND, HALF>), dim3((unsigned)((E) - beg[KIND - 1]), nl), dim3(64), 0, \
@@ -1118,7 +1125,15 @@ void run_galerkin(const GalerkinJob &job, const int *F, const int *vmap, cd *U, 
             if (U) { CQBEM_SING(0, 1) CQBEM_SING(0, 2) CQBEM_SING(0, 3) }
             if (P) { CQBEM_SING(1, 1) CQBEM_SING(1, 2) CQBEM_SING(1, 3) }
         }
-        if (Bt && Ba) { CQBEM_SING(2, 1) CQBEM_SING(2, 2) CQBEM_SING(2, 3) }
+        if (Bt && Ba) {
+            if (lane) {
+                CQBEM_SING(2, 1)
+            } else if (end[0] > beg[0]) {  // coincident B over half the regions, the exchanged half folded in
+                hipLaunchKernelGGL((k_gal_sing<2, 1, true>), dim3((unsigned)(end[0] - beg[0]), nl), dim3(64), 0, st, g, dp,
+                                   beg[0], job.line, job.nline, job.freqs, job.mat, scratch);
+            }
+            CQBEM_SING(2, 2) CQBEM_SING(2, 3)
+        }
 #undef CQBEM_SING
         check(hipGetLastError(), "k_gal_sing");
     }

@@ -234,3 +234,47 @@ def test_facade_matrix_cache(tmp_path):
     r3 = run(path, tmp_path / "c")
     assert r3.returncode == 0, r3.stderr
     assert np.array_equal(np.fromfile(tmp_path / "c_G.bin").reshape(L, n3, n3)[1], edited)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme,code", [("Loop", 1), ("FalseBarycentric", 2)])
+def test_facade_subdivided_and_decimated_mesh(tmp_path, scheme, code):
+    """A mesh block with subdivision_level, subdivision_scheme and decimation_scale: the facade assembles the mesh
+    those steps make (physics_base_matrix_collection.cpp:109-151), centred after them.  Its G, H (negated) and B equal
+    a direct assembly of the library's subdivided and decimated mesh at the facade's own centred vertices, with L and
+    R derived from that mesh as matrix_collection.cpp:63-69 does."""
+    from surfaceonlydeformables_amd import abi
+    V, F = M.icosphere(1, 0.1)
+    path, cfg = scene(tmp_path, V, F)
+    cfg["mesh"].update({"subdivision_level": 1, "subdivision_scheme": scheme, "decimation_scale": 0.5})
+    path.write_text(json.dumps(cfg))
+    print(f"collection facade, ico1 subdivided ({scheme}) and decimated", flush=True)
+    r = run(path, tmp_path / "out")
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    Vr, Fr = M.read_obj(str(tmp_path / "mesh.obj"))
+    Vs, Fs = abi.subdivide_mesh(Vr, Fr, code, 1)
+    Vd, Fd = abi.decimate_mesh(Vs, Fs, int(len(Fs) * 0.5))
+    assert info["nv"] == len(Vd) and len(Fd) <= len(Fs) // 2
+    Vc = np.fromfile(tmp_path / "out_post_V.bin").reshape(-1, 3)  # the facade's centred vertices
+    assert Vc.shape == Vd.shape and np.abs(Vc - M.center(Vd, Fd)).max() < 1e-15
+    mu, nu, rho, dt = 3.0e4, 0.3, 1000.0, 2e-3
+    lam = 2 * nu * mu / (1 - 2 * nu)
+    c1, c2 = float(np.sqrt((lam + 2 * mu) / rho)), float(np.sqrt(mu / rho))
+    maxd = np.sqrt(((Vc[:, None] - Vc[None]) ** 2).sum(-1)).max()
+    mth = int(np.floor(maxd / (c2 * dt) + 2))
+    L, R = mth + 1, 1e-10 ** (1 / (2 * mth))
+    assert info["L"] == L
+    ctx = abi.Context(0)
+    try:
+        ref = ctx.assemble(abi.make_mesh(Vc, Fd), abi.make_params(c1, c2, rho, nu, dt, R, L))
+    finally:
+        ctx.close()
+    n3 = 3 * len(Vd)
+    G = np.fromfile(tmp_path / "out_G.bin").reshape(L, n3, n3)
+    H = np.fromfile(tmp_path / "out_H.bin").reshape(L, n3, n3)
+    B = np.fromfile(tmp_path / "out_B.bin").reshape(L, n3, 6)
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(G, ref["G"]) < 1e-14
+    assert rel(-H, ref["H"]) < 1e-14
+    assert rel(B, ref["B"]) < 1e-14

@@ -90,10 +90,11 @@ def test_galerkin_time_domain_parity(ctx, oracle, name, L, bdf, variant, monkeyp
 
 @pytest.mark.parametrize("name,order,sub", [("ico1", 3, 1), ("cube2", 2, 2)])
 def test_touching_pair_symmetry_matches_full_integration(ctx, name, order, sub, monkeypatch):
-    """The quad kernels integrate coincident pairs over regions 1, 3, 5 and add the exchanged half (G even, H odd
-    in y - x), and take each vertex-adjacent mirror pair's G block from its primary's transpose; the lane kernels
-    integrate all regions of every ordered pair.  Same matrices to rounding: a wrong sign, corner swap or missing
-    half is an O(1) error in the touching blocks."""
+    """The default path integrates coincident pairs over regions 1, 3, 5 and adds the exchanged half (G even, H odd
+    in y - x; B folds it into its weights, Lx - Ly for its odd terms and Lx + Ly for its even ones), and takes each
+    vertex-adjacent mirror pair's G block from its primary's transpose; the lane kernels (CQBEM_GALERKIN_SINGULAR=
+    lane) integrate all regions of every ordered pair.  Same matrices to rounding: a wrong sign, corner swap or
+    missing half is an O(1) error in the touching blocks."""
     V, F = mesh(name)
     _, c1, c2 = M.material_speeds(1e6, 0.35, RHO)
     L, dt = 16, 5e-4
