@@ -651,7 +651,7 @@ def main():
         ctx.assemble_device(mesh, params, ptrs)
         lat.append((time.perf_counter() - t0) * 1e3)
         tm = ctx.last_timings()
-        gh_ms += tm["regular_gh"]
+        gh_ms += tm["regular_b" if params.galerkin else "regular_gh"]  # Galerkin: the pair kernel at the test points
         gh_evals += tm["gh_evaluations"]
         gh_launches += int(tm["gh_launches"])
     torch.cuda.synchronize()
@@ -661,17 +661,15 @@ def main():
     value = entries * args.steps / wall
     last = ctx.last_timings()
 
-    roof = None
-    if not params.galerkin:
-        roof = roofline(gh_ms, gh_evals, gh_launches, with_traffic=args.workload == WORKLOADS[0])
+    roof = roofline(gh_ms, gh_evals, gh_launches, with_traffic=args.workload == WORKLOADS[0], galerkin=params.galerkin)
 
     # time stepping on the resident matrices: the history product of one step with a full history
     history = history_step(ctx, ptrs, n3, L, torch, dev)
 
     phase_keys = ("regular_gh", "regular_b", "singular", "transform", "host")
-    if params.galerkin:  # non-singular G/H/B with the test projection in one phase
-        phases = {"nonsingular": last["regular_gh"], "singular_pairs": last["singular"], "transform": last["transform"],
-                  "host": last["host"]}
+    if params.galerkin:  # non-singular G/H/B with the test projection in one phase, of which the G/H pair kernel
+        phases = {"nonsingular": last["regular_gh"], "nonsingular_gh_pair_kernel": last["regular_b"],
+                  "singular_pairs": last["singular"], "transform": last["transform"], "host": last["host"]}
     else:
         phases = {k: last[k] for k in phase_keys}
     line = {"metric": METRIC, "value": value, "unit": "entries/s", "n_gpus": 1, "steps": args.steps,
@@ -707,10 +705,46 @@ def main():
         line["mixed_vs_fp64"] = variant_against_fp64(ctx, mesh, params, outs, ms_per_step, torch, abi.MIXED, False)
         line["mixed_banded_vs_fp64"] = variant_against_fp64(ctx, mesh, params, outs, ms_per_step, torch, abi.MIXED,
                                                             True)
+    if args.workload == WORKLOADS[0] and args.precision == "fp64" and args.quad == "reference":
+        del outs, ptrs
+        torch.cuda.empty_cache()
+        line["galerkin_icosphere_l3"] = galerkin_line(ctx, torch, dev)
     if not args.no_cpu_baseline and not params.galerkin:
         line["cpu_baseline"] = cpu_baseline(V, F, params)
     ctx.close()
     print(json.dumps(line), flush=True)
+
+
+def galerkin_line(ctx, torch, dev, steps=1):
+    """The icosphere-L3 Galerkin workload (L = 64, 1926 x 1926 complex matrices per frequency) in the same run: one
+    warm-up and `steps` timed assemblies into device outputs, its phases and the pair kernel's roofline at the test
+    points."""
+    from surfaceonlydeformables_amd import abi
+    V, F, params, desc = workload("ico3-galerkin", "fp64")
+    n3, L = 3 * len(V), params.L
+    outs = {"G": torch.empty((L, n3, n3), dtype=torch.float64, device=dev),
+            "H": torch.empty((L, n3, n3), dtype=torch.float64, device=dev),
+            "B": torch.empty((L, n3, 6), dtype=torch.float64, device=dev)}
+    ptrs = {k: t.data_ptr() for k, t in outs.items()}
+    mesh = abi.make_mesh(V, F)
+    ctx.assemble_device(mesh, params, ptrs)
+    torch.cuda.synchronize()
+    gh_ms = gh_evals = gh_launches = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.assemble_device(mesh, params, ptrs)
+        tm = ctx.last_timings()
+        gh_ms += tm["regular_b"]
+        gh_evals += tm["gh_evaluations"]
+        gh_launches += tm["gh_launches"]
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    del outs
+    torch.cuda.empty_cache()
+    return {"workload": desc, "steps": steps, "warmup": 1, "ms_per_step": ms, "entries_per_s": 2.0 * n3 * n3 * L * 1e3 / ms,
+            "phases_ms_last_step": {"nonsingular": tm["regular_gh"], "nonsingular_gh_pair_kernel": tm["regular_b"],
+                                    "singular_pairs": tm["singular"], "transform": tm["transform"], "host": tm["host"]},
+            "roofline": roofline(gh_ms, gh_evals, int(gh_launches), with_traffic=False, galerkin=True)}
 
 
 def variant_against_fp64(ctx, mesh, params, outs, fp64_ms, torch, precision, banded, steps=3):
@@ -779,11 +813,13 @@ def mixed_against_fp64(ctx, mesh, params, outs, mixed_ms, torch, dev, steps=3):
                     "CQBEM_MIXED_G=float; H, singular faces and the inverse transform in double"}
 
 
-def roofline(gh_ms, gh_evals, gh_launches, with_traffic):
+def roofline(gh_ms, gh_evals, gh_launches, with_traffic, galerkin=False):
     """VALU roofline of the dominant kernel from the library's HIP-event timers; the HBM traffic per launch is
-    the committed counter measurement of the icosphere-L4 shape."""
+    the committed counter measurement of the icosphere-L4 shape.  Galerkin: the same pair kernel at the test
+    points, per (test point, regular source face, source point, frequency)."""
     achieved = FLOPS_PER_EVAL_GH * gh_evals / (gh_ms * 1e-3) / 1e12
-    roof = {"bound": "valu", "kernel": "k_colloc_gh_pair<true,true,false>", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+    kernel = "k_colloc_gh_pair<true,true,true>" if galerkin else "k_colloc_gh_pair<true,true,false>"
+    roof = {"bound": "valu", "kernel": kernel, "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
             "launch_ms": gh_ms / max(gh_launches, 1),
             "flops_per_launch": FLOPS_PER_EVAL_GH * gh_evals / max(gh_launches, 1)}

@@ -209,9 +209,10 @@ int cqbem_frequencies(int32_t L, double R, double dt, int32_t bdf, double *s_out
 /* Statistics of the last cqbem_assemble on this context (device times from HIP events on the context's
  * stream, milliseconds): [0] total wall, [1] regular G/H quadrature (the dominant kernel; Galerkin: all
  * non-singular quadrature, G, H and B, with the test-basis projection), [2] regular B_trans/B_angular
- * quadrature (collocation), [3] singular kernels (collocation: faces at the point; Galerkin: touching
- * pairs and their gather), [4] inverse transform and post-processing, [5] host preparation and staging,
- * [6] launches of the dominant collocation kernel, [7] far-field kernel evaluations (collocation point,
+ * quadrature (collocation; Galerkin: the G/H pair kernel at the test points, a part of [1]), [3] singular
+ * kernels (collocation: faces at the point; Galerkin: touching pairs and their gather), [4] inverse transform
+ * and post-processing, [5] host preparation and staging, [6] launches of the dominant G/H kernel (the pair
+ * kernel; Galerkin: at the test points), [7] far-field kernel evaluations (collocation point or test point,
  * source point, frequency) those launches performed.
  * Copies min(n, 8) values; returns that count. */
 int cqbem_last_timings(const cqbem_context *ctx, double *out, int32_t n);

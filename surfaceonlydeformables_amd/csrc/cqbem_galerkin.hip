@@ -942,6 +942,8 @@ void run_galerkin(const GalerkinJob &job, const int *F, const int *vmap, cd *U, 
     for (auto &c : classes) flat.insert(flat.end(), c.begin(), c.end()), cls_off.push_back((int)flat.size());
     DevVec dfaces;
     const int *faces = dfaces.up(flat, st);
+    // faces sharing an (original) vertex with each face: its touching pairs, which the regular kernels exclude
+    const std::vector<std::vector<int>> cands = touching_candidates(F, nf, nv, vmap);
 
     const char *variant = std::getenv("CQBEM_GALERKIN_KERNEL");
     if ((U || P || Bt || Ba) && !(variant && !std::strcmp(variant, "star"))) {
@@ -981,10 +983,18 @@ void run_galerkin(const GalerkinJob &job, const int *F, const int *vmap, cd *U, 
                 const int nfc = std::min(cap, cnt - b), fi0 = cls_off[c] + b;
                 const int row0 = fi0 * npts, nrows = nfc * npts;
                 if (U || P) {
-                    check(launch_colloc_gh_pair(g, job.pair_pts, job.npts_even, job.freqs, 0, nl, job.mat, row0, nrows, tpts,
-                                                job.patch_pass, job.pass_info, job.pass_faces, job.patch_verts, job.npatches,
-                                                sG, sH, n3, ls, st),
-                          "k_colloc_gh_pair (test points)");
+                    {
+                        // the pair kernel at the test points: timed on its own, with its launches and its unique
+                        // evaluations (test point, regular source face, source point, frequency)
+                        EvTimer tp(st, &tm[2]);
+                        check(launch_colloc_gh_pair(g, job.pair_pts, job.npts_even, job.freqs, 0, nl, job.mat, row0, nrows,
+                                                    tpts, job.patch_pass, job.pass_info, job.pass_faces, job.patch_verts,
+                                                    job.npatches, sG, sH, n3, ls, st),
+                              "k_colloc_gh_pair (test points)");
+                    }
+                    tm[6] += 1;
+                    for (int fi = fi0; fi < fi0 + nfc; fi++)
+                        tm[7] += (double)npts * (double)(nf - (int)cands[flat[fi]].size()) * npts * nl;
                     if (U) project(sG, n3, ls, faces + fi0, nfc, row0, U, n3 * n3);
                     if (P) project(sH, n3, ls, faces + fi0, nfc, row0, P, n3 * n3);
                 }
@@ -1023,7 +1033,6 @@ void run_galerkin(const GalerkinJob &job, const int *F, const int *vmap, cd *U, 
     // Touching pairs, classified exactly as integrate_galerkin_singular (integrator_galerkin.hpp:262-334).
     std::vector<Pair> pairs;
     {
-        const std::vector<std::vector<int>> cands = touching_candidates(F, nf, nv, vmap);
         auto same = [&](int a, int b) { return a == b || (vmap && vmap[a] == vmap[b]); };
         for (int f1 = 0; f1 < nf; f1++) {
             for (int f2 : cands[f1]) {
