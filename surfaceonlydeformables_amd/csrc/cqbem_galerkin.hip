@@ -1067,33 +1067,39 @@ void run_galerkin(const GalerkinJob &job, const int *F, const int *vmap, cd *U, 
         }
     }
     if (pairs.empty()) return;
-    // by kind; vertex-adjacent pairs (f1, f2) with f1 < f2 before their mirrors (f2, f1), whose G blocks are
-    // transposes of theirs and are not integrated (quad kernels)
-    std::stable_sort(pairs.begin(), pairs.end(), [](const Pair &a, const Pair &b) {
-        const bool ma = a.kind == 3 && a.f1 > a.f2, mb = b.kind == 3 && b.f1 > b.f2;
-        return a.kind != b.kind ? a.kind < b.kind : (!ma && mb);
-    });
+    std::stable_sort(pairs.begin(), pairs.end(), [](const Pair &a, const Pair &b) { return a.kind < b.kind; });
     int kind_off[4] = {0, 0, 0, 0};
     for (const Pair &p : pairs) kind_off[p.kind]++;
     kind_off[3] += kind_off[2] += kind_off[1];  // exclusive ends per kind: [0,k1) coincident, [k1,k2) edge, [k2,k3) vertex
+    // Vertex-adjacent mirrors: a pair (f1, f2) with f1 > f2 whose pair (f2, f1) is listed with both faces ordered from
+    // the same shared vertex (as the classification orders them on a consistently oriented mesh).  Its G block is
+    // the transpose of that primary's and is not integrated (quad kernels); the mirrors go after the other
+    // vertex-adjacent pairs, and a pair without such a primary is integrated in full.
     std::vector<int2> mirrors;  // (mirror pair, primary pair)
-    int vprim = kind_off[3];    // end of the vertex-adjacent primaries
+    int vprim = kind_off[3];    // end of the vertex-adjacent pairs that are integrated for G
     {
+        auto key = [](const Pair &p) { return std::make_pair(p.f1, p.f2); };
         std::map<std::pair<int, int>, int> at;
-        for (int k = kind_off[2]; k < kind_off[3]; k++)
-            if (pairs[k].f1 < pairs[k].f2) at[{pairs[k].f1, pairs[k].f2}] = k;
-        for (int k = kind_off[2]; k < kind_off[3]; k++) {
-            const Pair &p = pairs[k];
-            if (p.f1 < p.f2) continue;
+        for (int k = kind_off[2]; k < kind_off[3]; k++) at[key(pairs[k])] = k;
+        auto mirrored = [&](const Pair &p) {
+            if (p.f1 < p.f2) return false;
             auto it = at.find({p.f2, p.f1});
-            const Pair *q = it == at.end() ? nullptr : &pairs[it->second];
-            // the mirror rule needs both faces ordered from the shared vertex, as the classification orders them
-            if (!q || q->j1[0] != p.j2[0] || q->j1[1] != p.j2[1] || q->j1[2] != p.j2[2] || q->j2[0] != p.j1[0] ||
-                q->j2[1] != p.j1[1] || q->j2[2] != p.j1[2])
-                throw std::runtime_error("vertex-adjacent pair without its mirror");
-            if (vprim == kind_off[3]) vprim = k;
-            mirrors.push_back(make_int2(k, it->second));
-        }
+            if (it == at.end()) return false;
+            const Pair &q = pairs[it->second];
+            for (int c = 0; c < 3; c++)
+                if (q.j1[c] != p.j2[c] || q.j2[c] != p.j1[c]) return false;
+            return true;
+        };
+        std::vector<char> mir(pairs.size(), 0);
+        for (int k = kind_off[2]; k < kind_off[3]; k++) mir[k] = mirrored(pairs[k]) ? 1 : 0;
+        std::vector<Pair> prim, rest;
+        for (int k = kind_off[2]; k < kind_off[3]; k++) (mir[k] ? rest : prim).push_back(pairs[k]);
+        std::copy(prim.begin(), prim.end(), pairs.begin() + kind_off[2]);
+        std::copy(rest.begin(), rest.end(), pairs.begin() + kind_off[2] + (int)prim.size());
+        vprim = kind_off[2] + (int)prim.size();
+        at.clear();
+        for (int k = kind_off[2]; k < vprim; k++) at[key(pairs[k])] = k;
+        for (int k = vprim; k < kind_off[3]; k++) mirrors.push_back(make_int2(k, at.at({pairs[k].f2, pairs[k].f1})));
     }
     const long long npairs = (long long)pairs.size();
     DevVec dpairs, dscratch, dblocks, dptr, ditems, dvptr, dvitems, dmirrors;
