@@ -682,6 +682,14 @@ def main():
                        "entries_per_step": entries},
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
             "phases_ms_last_step": phases, "roofline": roof, "time_step_history": history}
+    if not params.galerkin and last["transform"] > 0:
+        # the G/H weights' write-out: the inverse transform reads each entry's Lf complex spectrum values once and
+        # writes its L real weights once (the B transform and post-processing in the same phase are ~0.1% of it)
+        lf = L // 2 + 1
+        wbytes = 2.0 * n3 * n3 * (16.0 * lf + 8.0 * L)
+        line["write_out"] = {"kernel": "k_transform<32,8>", "bound": "hbm", "ms": last["transform"], "bytes": wbytes,
+                             "achieved_TBps": wbytes / (last["transform"] * 1e-3) / 1e12, "peak_TBps": 8.0,
+                             "frac": wbytes / (last["transform"] * 1e-3) / 8e12}
     if args.quad == "banded":
         cmp = mixed_against_fp64(ctx, mesh, params, outs, ms_per_step, torch, dev)
         cmp["speedup_vs_reference_rule"] = cmp.pop("speedup_vs_fp64")
