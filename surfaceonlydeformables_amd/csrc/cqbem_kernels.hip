@@ -867,6 +867,70 @@ __global__ void __launch_bounds__(256) k_transform(const cd *__restrict__ spec, 
     }
 }
 
+// The same sums for two adjacent entries per lane (E, spec_ls and out_nstride even, out 16-byte aligned): each
+// frequency's two complex values in one 32-byte read, each weight's two values in one 16-byte store (the store
+// width HBM writes best), and a grid that covers every pair at once.  Each weight is k_transform's sum, term for
+// term.
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+template <class OutT> struct Pair2;
+template <> struct Pair2<double> {
+    typedef d2v type;
+};
+template <> struct Pair2<float> {
+    typedef f2v type;
+};
+template <int LC, class OutT>
+__global__ void __launch_bounds__(256) k_transform_pair(const cd *__restrict__ spec, long long E, long long spec_ls, int Lf,
+                                                        const double2 *__restrict__ tab, int L, int n0,
+                                                        OutT *__restrict__ out, long long out_nstride) {
+    constexpr int NC = kTransformNC;
+    __shared__ double2 T[NC * kTransformMaxLf];  // [l][k]
+    for (int t = threadIdx.x; t < NC * Lf; t += 256) {
+        const int l = t / NC, k = t % NC;
+        T[t] = n0 + k < L ? tab[(long long)(n0 + k) * Lf + l] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const long long e = ((long long)blockIdx.x * 256 + threadIdx.x) * 2;
+    if (e >= E) return;
+    double a0[NC], a1[NC];
+#pragma unroll
+    for (int k = 0; k < NC; k++) a0[k] = a1[k] = 0.0;
+    for (int l0 = 0; l0 < Lf; l0 += LC) {
+        d2v X0[LC], X1[LC];
+#pragma unroll
+        for (int c = 0; c < LC; c++) {
+            // each spectrum value is read once: streaming (non-temporal) loads
+            const d2v *src = reinterpret_cast<const d2v *>(spec + (long long)(l0 + c) * spec_ls + e);
+            if (l0 + c < Lf) {
+                X0[c] = __builtin_nontemporal_load(src);
+                X1[c] = __builtin_nontemporal_load(src + 1);
+            } else {
+                X0[c] = d2v{0.0, 0.0}, X1[c] = d2v{0.0, 0.0};
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < LC; c++) {
+            if (l0 + c >= Lf) break;
+            const double2 *Tl = T + (l0 + c) * NC;
+#pragma unroll
+            for (int k = 0; k < NC; k++) {
+                a0[k] = fma(X0[c].x, Tl[k].x, fma(-X0[c].y, Tl[k].y, a0[k]));
+                a1[k] = fma(X1[c].x, Tl[k].x, fma(-X1[c].y, Tl[k].y, a1[k]));
+            }
+        }
+    }
+    typedef typename Pair2<OutT>::type V2;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        const int n = n0 + k;
+        if (n < L) {
+            const V2 w = {(OutT)a0[k], (OutT)a1[k]};
+            __builtin_nontemporal_store(w, reinterpret_cast<V2 *>(out + (long long)n * out_nstride + e));  // written once
+        }
+    }
+}
+
 // Same for Lf > kTransformMaxLf, with the table read from global memory.
 template <int NC, class OutT>
 __global__ void __launch_bounds__(256) k_transform_wide(const cd *__restrict__ spec, long long E, long long spec_ls, int Lf,
@@ -1211,6 +1275,18 @@ hipError_t launch_colloc_singular(int kind, const Geom &g, const double2 *line, 
 template <class OutT>
 static hipError_t launch_transform_t(const cd *spec, long long E, long long spec_ls, int Lf, const double2 *tab, int L,
                                      OutT *out, long long out_nstride, int num_cus, hipStream_t st) {
+    // two entries per lane when the pairs stay aligned: even sizes and strides, a 2-value-aligned output
+    if (Lf <= kTransformMaxLf && E % 2 == 0 && spec_ls % 2 == 0 && out_nstride % 2 == 0 &&
+        reinterpret_cast<uintptr_t>(out) % (2 * sizeof(OutT)) == 0 && E / 2 <= 256LL * 0x7fffffffLL) {
+        const long long pblocks = (E / 2 + 255) / 256;
+        for (int n0 = 0; n0 < L; n0 += kTransformNC) {
+            hipLaunchKernelGGL((k_transform_pair<4, OutT>), dim3((unsigned)pblocks), dim3(256), 0, st, spec, E, spec_ls, Lf,
+                               tab, L, n0, out, out_nstride);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
     long long blocks = (E + 255) / 256;
     const long long cap = (long long)num_cus * 64;
     if (blocks > cap) blocks = cap;

@@ -219,3 +219,37 @@ def test_row_slabs_are_bitwise_identical():
     finally:
         full.close()
         small.close()
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_paired_transform_matches_the_single_entry_kernel(storage):
+    """The inverse transform takes two adjacent entries per lane (16-byte stores) when sizes, strides and the output
+    pointer allow it, and one entry per lane otherwise; each weight is the same sum, term for term.  The same job
+    into an aligned output and into one offset by a single element (which forces the single-entry kernel) gives
+    the same bits."""
+    import torch
+    V, F = M.icosphere(2, 0.2)
+    V = M.center(M.jitter(V, F, 0.01, 6), F)
+    c1, c2 = material(5e5, 0.3)
+    L = 16
+    store = abi.STORE_F32 if storage == "f32" else abi.STORE_F64
+    params = abi.make_params(c1, c2, RHO, 0.3, 1e-3, 1e-10 ** (1 / (2 * L)), L, storage=store,
+                             which=abi.G | abi.H)
+    mesh = abi.make_mesh(V, F)
+    n3 = 3 * len(V)
+    assert n3 % 2 == 0  # even sizes: the aligned run takes the paired kernel
+    dt = torch.float32 if storage == "f32" else torch.float64
+    ctx = abi.Context(0)
+    try:
+        res = []
+        for offset in (0, 1):
+            print(f"paired transform: storage {storage}, output offset {offset}", flush=True)
+            flat = {k: torch.full((L * n3 * n3 + 1,), float("nan"), dtype=dt, device="cuda:0") for k in ("G", "H")}
+            torch.cuda.synchronize()
+            ctx.assemble_device(mesh, params, {k: t[offset:].data_ptr() for k, t in flat.items()})
+            res.append({k: t[offset:offset + L * n3 * n3].cpu().numpy() for k, t in flat.items()})
+        for k in ("G", "H"):
+            assert np.isfinite(res[0][k]).all(), k
+            assert np.array_equal(res[0][k], res[1][k]), k
+    finally:
+        ctx.close()
