@@ -700,11 +700,12 @@ def main():
         line["banded_vs_reference_rule"] = cmp
         if roof:
             roof["note"] = "banded: evaluations counted per tile's rule"
+            roof["kernel"] = "k_colloc_gh_pair<true,true,false,true>"
     if args.precision == "mixed":
         line["mixed_vs_fp64"] = mixed_against_fp64(ctx, mesh, params, outs, ms_per_step, torch, dev)
         if roof and os.environ.get("CQBEM_MIXED_G") == "float":
             roof["note"] = "mixed, float G: the timed G/H phase is the double H pair kernel plus the float G kernel"
-            roof["kernel"] = "k_colloc_gh_pair<false,true,false> + k_colloc_g32"
+            roof["kernel"] = "k_colloc_gh_pair<false,true,false,false> + k_colloc_g32"
         elif roof:
             roof["note"] = "mixed: G and H in the double pair kernel, the far-field B in float"
     if args.precision == "fp64" and args.quad == "reference" and not params.galerkin:
@@ -826,7 +827,7 @@ def roofline(gh_ms, gh_evals, gh_launches, with_traffic, galerkin=False):
     the committed counter measurement of the icosphere-L4 shape.  Galerkin: the same pair kernel at the test
     points, per (test point, regular source face, source point, frequency)."""
     achieved = FLOPS_PER_EVAL_GH * gh_evals / (gh_ms * 1e-3) / 1e12
-    kernel = "k_colloc_gh_pair<true,true,true>" if galerkin else "k_colloc_gh_pair<true,true,false>"
+    kernel = "k_colloc_gh_pair<true,true,true,false>" if galerkin else "k_colloc_gh_pair<true,true,false,false>"
     roof = {"bound": "valu", "kernel": kernel, "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
             "launch_ms": gh_ms / max(gh_launches, 1),

@@ -344,7 +344,8 @@ __device__ __attribute__((always_inline)) inline void pair_tile(const Geom &g, c
     }
 }
 
-template <bool DG, bool DH, bool GAL>
+// BANDED: the distance-banded rule (CQBEM_QUAD_BANDED) is compiled in; without it the tile rule check is not.
+template <bool DG, bool DH, bool GAL, bool BANDED>
 __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, const double4 *__restrict__ pts, int nrounds,
                                                                   const Freq *__restrict__ freqs, int l0, Mat m, int row0,
                                                                   const double4 *__restrict__ tpts,
@@ -358,8 +359,8 @@ __global__ void __launch_bounds__(kPairThreads, 2) k_colloc_gh_pair(Geom g, cons
     __shared__ double fgeo[kFaceWords * kPairFaces];  // [word][face of pass]: re-read every round, not held in VGPRs
     __shared__ MathTables mtab;                       // tables of the exponentials
     load_math_tables(&mtab);
-    pair_tile<DG, DH, GAL>(g, pts, nrounds, bands, freqs, l0, m, row0, tpts, patch_pass, pass_info, pass_faces, patch_verts,
-                           UG, UH, ld, lstride, blockIdx.x, blockIdx.y, blockIdx.z, acc, fgeo, mtab);
+    pair_tile<DG, DH, GAL>(g, pts, nrounds, BANDED ? bands : Bands{}, freqs, l0, m, row0, tpts, patch_pass, pass_info,
+                           pass_faces, patch_verts, UG, UH, ld, lstride, blockIdx.x, blockIdx.y, blockIdx.z, acc, fgeo, mtab);
 }
 
 // ------------------------------------------------------------------------
@@ -1187,17 +1188,21 @@ hipError_t launch_colloc_gh_pair(const Geom &g, const double4 *pts, int npts_eve
     if ((!UG && !UH) || (npts_even & 1) || nrows <= 0 || nrows > 65535) return hipErrorInvalidValue;
     if (bands.far_pts && (tpts || bands.far_rounds <= 0 || !bands.sphere)) return hipErrorInvalidValue;
     dim3 grid(npatches, nl, nrows);
-#define CQBEM_PAIR(DG, DH, GAL)                                                                                           \
-    hipLaunchKernelGGL((k_colloc_gh_pair<DG, DH, GAL>), grid, dim3(kPairThreads), 0, st, g, pts, npts_even / 2, freqs, l0, m, \
-                       row0, tpts, patch_pass, pass_info, pass_faces, patch_verts, UG, UH, ld, lstride, bands)
-    if (tpts) {
-        if (UG && UH) CQBEM_PAIR(true, true, true);
-        else if (UG) CQBEM_PAIR(true, false, true);
-        else CQBEM_PAIR(false, true, true);
+#define CQBEM_PAIR(DG, DH, GAL, BANDED)                                                                                    \
+    hipLaunchKernelGGL((k_colloc_gh_pair<DG, DH, GAL, BANDED>), grid, dim3(kPairThreads), 0, st, g, pts, npts_even / 2, freqs, \
+                       l0, m, row0, tpts, patch_pass, pass_info, pass_faces, patch_verts, UG, UH, ld, lstride, bands)
+    if (tpts) {  // test points: never banded (refused above)
+        if (UG && UH) CQBEM_PAIR(true, true, true, false);
+        else if (UG) CQBEM_PAIR(true, false, true, false);
+        else CQBEM_PAIR(false, true, true, false);
+    } else if (bands.far_pts) {
+        if (UG && UH) CQBEM_PAIR(true, true, false, true);
+        else if (UG) CQBEM_PAIR(true, false, false, true);
+        else CQBEM_PAIR(false, true, false, true);
     } else {
-        if (UG && UH) CQBEM_PAIR(true, true, false);
-        else if (UG) CQBEM_PAIR(true, false, false);
-        else CQBEM_PAIR(false, true, false);
+        if (UG && UH) CQBEM_PAIR(true, true, false, false);
+        else if (UG) CQBEM_PAIR(true, false, false, false);
+        else CQBEM_PAIR(false, true, false, false);
     }
 #undef CQBEM_PAIR
     return hipGetLastError();
