@@ -133,7 +133,7 @@ __global__ void __launch_bounds__(64) k_colloc_gh(Geom g, const double4 *__restr
 
 // ------------------------------------------------------------------------
 // G and H, regular faces, pair form.  Column vertices are grouped into
-// Morton-ordered patches of kPairPV vertices; a block owns one (patch,
+// compact patches of at most kPairPV vertices; a block owns one (patch,
 // frequency, row vertex) and evaluates every face touching the patch once.
 // Two adjacent lanes share a face: per round each evaluates one quadrature
 // point, forms its 3x3 G and H blocks K (7 + 9 complex), and the two swap K
