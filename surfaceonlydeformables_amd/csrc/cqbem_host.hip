@@ -212,7 +212,7 @@ struct Prepared {
 // CQBEM_QUAD_BANDED default: far when the row point is kBandKappa largest face diameters beyond the patch's bounding
 // sphere.  Tuned on icosphere level 4 at L = 32 (surfaceonlydeformables_amd/tune_bands.py, table in
 // surfaceonlydeformables_amd/band_tables/): the smallest kappa whose added error stays within 5% of the reference
-// rule's largest per-weight error against a rule of twice the subdivision (3.0% for G, 0.6% for H).
+// rule's largest per-weight error against a rule of twice the subdivision (3.4% for G, 0.6% for H).
 constexpr double kBandKappa = 6.0;
 // The B kernels take face bands only when asked (band_kappa_b > 0).  B_trans and B_angular are closed-surface
 // integrals that cancel to a small remainder (their reference-rule error is 50x below G's), so the far rule's
