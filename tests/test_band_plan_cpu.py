@@ -99,3 +99,36 @@ def test_plan_refusals(ico4, monkeypatch):
     monkeypatch.setenv("CQBEM_MIXED_G", "float")
     with pytest.raises(abi.CQBEMError):
         abi.band_plan(mesh, params(quad=abi.QUAD_BANDED, precision=abi.MIXED))
+
+
+def _patch_lists(F, pov, npatches):
+    """Faces listed by each patch: those with a corner in it (the pair kernel evaluates each once per patch)."""
+    return [np.flatnonzero((pov[F] == p).any(axis=1)) for p in range(npatches)]
+
+
+@pytest.mark.parametrize("case", ["ico4", "split_cube12"])
+def test_pair_patches_fit_the_kernel_and_list_few_faces_twice(ico4, case):
+    """The pair kernel's column patches (the band plan reports them): every vertex in one patch, at most 256
+    vertices (the LDS accumulator slots) and 512 listed faces (four passes of 128) per patch.  Patches grown by
+    fewest new faces list each face at most 1.14 times at icosphere level 4 (runs of the Morton order: 1.22).
+    The split cube is six separate sheets: a patch whose sheet runs out continues in the next, so no patch is
+    left small while another sheet still has vertices."""
+    if case == "ico4":
+        V, F = ico4
+        mesh = abi.make_mesh(V, F)
+    else:
+        V, F, vm = abi.split_traction_discontinuity(*M.cube(12, 0.5), 20.0)
+        mesh = abi.make_mesh(V, F, vm)
+    pov, far = abi.band_plan(mesh, params(quad=abi.QUAD_BANDED))
+    n = far.shape[1]
+    sizes = np.bincount(pov, minlength=n)
+    assert sizes.sum() == len(V) and sizes.min() > 0 and sizes.max() <= 256
+    lists = _patch_lists(np.asarray(F), pov, n)
+    counts = np.array([len(l) for l in lists])
+    assert counts.max() <= 512
+    listed = counts.sum() / len(F)
+    if case == "ico4":
+        assert listed <= 1.14, listed
+    # only the last patch may be cut short by the end of the vertices, not by a mesh piece running out
+    full = (sizes[:-1] == 256) | (counts[:-1] > 512 - 12)
+    assert full.all(), (sizes, counts)
