@@ -687,7 +687,7 @@ def main():
         # writes its L real weights once (the B transform and post-processing in the same phase are ~0.1% of it)
         lf = L // 2 + 1
         wbytes = 2.0 * n3 * n3 * (16.0 * lf + 8.0 * L)
-        line["write_out"] = {"kernel": "k_transform_pair<4>", "bound": "hbm", "ms": last["transform"], "bytes": wbytes,
+        line["write_out"] = {"kernel": "k_transform_pair<2>", "bound": "hbm", "ms": last["transform"], "bytes": wbytes,
                              "achieved_TBps": wbytes / (last["transform"] * 1e-3) / 1e12, "peak_TBps": 8.0,
                              "frac": wbytes / (last["transform"] * 1e-3) / 8e12}
     if args.quad == "banded":

@@ -871,7 +871,8 @@ __global__ void __launch_bounds__(256) k_transform(const cd *__restrict__ spec, 
 // The same sums for two adjacent entries per lane (E, spec_ls and out_nstride even, out 16-byte aligned): each
 // frequency's two complex values in one 32-byte read, each weight's two values in one 16-byte store (the store
 // width HBM writes best), and a grid that covers every pair at once.  Each weight is k_transform's sum, term for
-// term.
+// term.  LC = 2 frequencies per batch of loads keeps the kernel within 168 VGPRs: 3 waves per SIMD (with 4 it takes
+// 181 VGPRs and 2 waves, 2.6% slower).
 typedef double d2v __attribute__((ext_vector_type(2)));
 typedef float f2v __attribute__((ext_vector_type(2)));
 template <class OutT> struct Pair2;
@@ -882,7 +883,7 @@ template <> struct Pair2<float> {
     typedef f2v type;
 };
 template <int LC, class OutT>
-__global__ void __launch_bounds__(256) k_transform_pair(const cd *__restrict__ spec, long long E, long long spec_ls, int Lf,
+__global__ void __launch_bounds__(256, 3) k_transform_pair(const cd *__restrict__ spec, long long E, long long spec_ls, int Lf,
                                                         const double2 *__restrict__ tab, int L, int n0,
                                                         OutT *__restrict__ out, long long out_nstride) {
     constexpr int NC = kTransformNC;
@@ -1285,7 +1286,7 @@ static hipError_t launch_transform_t(const cd *spec, long long E, long long spec
         reinterpret_cast<uintptr_t>(out) % (2 * sizeof(OutT)) == 0 && E / 2 <= 256LL * 0x7fffffffLL) {
         const long long pblocks = (E / 2 + 255) / 256;
         for (int n0 = 0; n0 < L; n0 += kTransformNC) {
-            hipLaunchKernelGGL((k_transform_pair<4, OutT>), dim3((unsigned)pblocks), dim3(256), 0, st, spec, E, spec_ls, Lf,
+            hipLaunchKernelGGL((k_transform_pair<2, OutT>), dim3((unsigned)pblocks), dim3(256), 0, st, spec, E, spec_ls, Lf,
                                tab, L, n0, out, out_nstride);
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
