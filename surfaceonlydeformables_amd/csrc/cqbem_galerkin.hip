@@ -396,8 +396,10 @@ __device__ __forceinline__ void region_point(int r, double e1, double e2, double
 }
 
 // HALF (B of coincident pairs): regions 1, 3, 5 only, each point standing for its exchanged image as well.
+// B (MODE 2) is held to 2 waves per SIMD: its 36 accumulators and working set take 316 registers at 1 wave; at 2 it
+// spills 46-59 of them to scratch and still runs 15% faster.  G and H (the lane variant) keep 1 wave.
 template <int MODE, int KIND, bool HALF = false>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) k_gal_sing(Geom g, const Pair *__restrict__ pairs, int pair0,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 2 : 1, 2))) k_gal_sing(Geom g, const Pair *__restrict__ pairs, int pair0,
                                                   const double2 *__restrict__ line, int nline,
                                                   const Freq *__restrict__ freqs, Mat m, cd *__restrict__ out) {
     constexpr int NA = MODE == 0 ? 63 : (MODE == 1 ? 81 : 36);
