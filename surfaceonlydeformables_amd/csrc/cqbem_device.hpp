@@ -289,11 +289,26 @@ __device__ inline Expo expo(double r, const Freq &f, const MathTables *M) {
     return E;
 }
 
-// u_hat = alpha rh rh^T + beta I  (times r if `times_r`: r*u_hat for polar rules)
+// The exponentials G and H need: e1, e2 and dA, the last formed as ((z1 + 1) e1 - (z2 + 1) e2) / s^2 with one product
+// by 1/s^2 (A2 is not formed: it is left zero).
+__device__ inline Expo expo_gh(double r, const Freq &f, const MathTables *M) {
+    Expo E;
+    const cd z1 = f.sc1 * r, z2 = f.sc2 * r;
+    E.e1 = cexp_neg_tab(z1, M);
+    E.e2 = cexp_neg_tab(z2, M);
+    E.dA = ((z1 + cd{1.0, 0.0}) * E.e1 - (z2 + cd{1.0, 0.0}) * E.e2) * f.is2;
+    E.A2 = cd{0.0, 0.0};
+    return E;
+}
+
+// u_hat = alpha rh rh^T + beta I  (times r if `times_r`: r*u_hat for polar rules).  SCALED = false leaves out the
+// factor g_scale = 1/(4 pi rho), for callers that apply it to their sums.
+template <bool SCALED = true>
 __device__ inline void g_coef(const Expo &E, double ir, double ir2, const Mat &m, cd &alpha, cd &beta) {
     const cd t = E.dA * ir2;
-    alpha = (3.0 * t + E.e1 * m.ic1s - E.e2 * m.ic2s) * m.g_scale;
-    beta = (E.e2 * m.ic2s - t) * m.g_scale;
+    alpha = 3.0 * t + E.e1 * m.ic1s - E.e2 * m.ic2s;
+    beta = E.e2 * m.ic2s - t;
+    if (SCALED) alpha = alpha * m.g_scale, beta = beta * m.g_scale;
 }
 
 // p_hat coefficients without the 1/(4 pi) factor.
@@ -320,6 +335,18 @@ __device__ inline void b_coef(const Expo &E, double ir, double ir2, const Freq &
     const cd u2 = ((E.e2 * m.ic2) - (E.e1 * m.ic1)) * f.is * kInv8Pi;
     u12 = u1 + u2;
     u3 = E.e2 * f.is * (m.ic2 * kInv4Pi);
+}
+
+// b_coef times 8 pi: t1, u1 and u12 without their factor 1/(8 pi), t2 and u3 without 1/(4 pi) but doubled (exact), so
+// that every sum of them takes the one factor 1/(8 pi) at its end.
+__device__ inline void b_coef_8pi(const Expo &E, double ir, double ir2, const Freq &f, const Mat &m, cd &t1, cd &t2,
+                                  cd &u1, cd &u12, cd &u3) {
+    t1 = E.dA * ir2;
+    t2 = (E.A2 - f.is2) * (2.0 * ir2);
+    u1 = E.dA * ir;
+    const cd u2 = ((E.e2 * m.ic2) - (E.e1 * m.ic1)) * f.is;
+    u12 = u1 + u2;
+    u3 = E.e2 * f.is * (2.0 * m.ic2);
 }
 
 // symmetric 3x3 packed index: 00 11 22 01 02 12

@@ -243,9 +243,9 @@ __device__ __attribute__((always_inline)) inline void pair_tile(const Geom &g, c
                 const double ir = rsqrt_nr(d2), r = d2 * ir, ir2 = ir * ir;
                 const double h[3] = {r0 * ir, r1 * ir, r2 * ir};
                 const double wL = Qo.w * ar;
-                const Expo E = expo(r, f, &mtab);
+                const Expo E = expo_gh(r, f, &mtab);
                 cd al, be, c1, c2, c4;
-                if (DG) g_coef(E, ir, ir2, m, al, be);
+                if (DG) g_coef<false>(E, ir, ir2, m, al, be);  // 1/(4 pi rho) at the write-out
                 if (DH) h_coef(E, ir, ir2, f, m, c1, c2, c4);
                 // corner weights: the full corner (0 on lane A, 2 on lane B) takes the own and the partner point;
                 // corner 1 takes the own point only, and the pair adds its two halves after the face
@@ -337,7 +337,7 @@ __device__ __attribute__((always_inline)) inline void pair_tile(const Geom &g, c
                 if (DG) {
                     cd v{a(2 * e), a(2 * e + 1)};
                     if (p == b) v += cd{a(12), a(13)};
-                    UG[obase + p * ld + b] = v;
+                    UG[obase + p * ld + b] = v * m.g_scale;
                 }
                 if (DH) UH[obase + p * ld + b] = cd{a(14 + 2 * (p * 3 + b)), a(15 + 2 * (p * 3 + b))} * kInv4Pi;
             }
@@ -482,7 +482,7 @@ __global__ void __launch_bounds__(256) k_colloc_b(Geom g, const double4 *__restr
             const double w = Q.w * ar;
             const Expo E = expo(r, f, &mtab);
             cd t1, t2, u1, u12, u3;
-            b_coef(E, ir, ir2, f, m, t1, t2, u1, u12, u3);
+            b_coef_8pi(E, ir, ir2, f, m, t1, t2, u1, u12, u3);  // 1/(8 pi) at the write-out
             const double rn = h[0] * n[0] + h[1] * n[1] + h[2] * n[2];
             t1 = t1 * w;
             u1 = u1 * w;
@@ -528,7 +528,7 @@ __global__ void __launch_bounds__(256) k_colloc_b(Geom g, const double4 *__restr
     __syncthreads();
     if (threadIdx.x < kBWords) {
         const int k = threadIdx.x;
-        const double v = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+        const double v = ((red[0][k] + red[1][k]) + (red[2][k] + red[3][k])) * kInv8Pi;
         // scatter the packed symmetric entries to their full positions
         const int c = k & 1;
         if (k < 12) {
