@@ -85,6 +85,45 @@ def test_time_domain_parity(ctx, oracle, name, L, bdf, ref_R, variant, monkeypat
             assert e_gpu <= max(2 * e_f64, 1e-12), (mine, n, e_gpu, e_f64)
 
 
+@pytest.mark.parametrize("L", [32, 130])
+def test_transform_of_the_laplace_spectra(ctx, L):
+    """The time-domain weights are the inverse CQ transform of the Laplace-domain matrices the same library returns
+    (cqbem_laplace, l = 0..L/2): W_n = R^-n / L Re(sum_l m_l X_l zeta^(-n l)), m_l = 1 at l = 0 and L/2, else 2.  The
+    sum is formed in numpy and each entry must agree within 1e-13 of the sum of its terms' magnitudes (the rounding
+    of a 66-term sum; a wrong index, sign or multiplicity misses by orders of magnitude); entries that are zero at
+    every frequency must be zero.  L = 32 runs the paired
+    transform; L = 130 (66 frequencies, beyond the 64-frequency LDS table) the wide one.  H is compared from n = 1
+    (its n = 0 weight also holds the C matrix)."""
+    V, F = small_meshes()["cube2"]
+    c1, c2 = material(5e5, 0.3)
+    dt, R = 1e-3, 1e-10 ** (1 / (2 * L))
+    params = abi.make_params(c1, c2, RHO, 0.3, dt, R, L, bdf=2)
+    params.which = abi.G | abi.H
+    mesh = abi.make_mesh(V, F)
+    print(f"transform of the Laplace spectra, cube2 L={L}", flush=True)
+    res = ctx.assemble(mesh, params, want=("G", "H"))
+    Lf = L // 2 + 1
+    spec = {"G": [], "H": []}
+    for l in range(Lf):
+        lp = ctx.laplace(mesh, params, l)
+        spec["G"].append(lp["U"]), spec["H"].append(lp["P"])
+    mult = np.array([1.0 if (l == 0 or 2 * l == L) else 2.0 for l in range(Lf)])
+    n = np.arange(L)[:, None]
+    ang = 2 * np.pi * n * np.arange(Lf)[None, :] / L
+    scale = R ** (-np.arange(L, dtype=np.float64)) / L
+    for name in ("G", "H"):
+        X = np.stack(spec[name]).reshape(Lf, -1)  # (Lf, entries)
+        W = scale[:, None] * ((mult * np.cos(ang)) @ X.real + (mult * np.sin(ang)) @ X.imag)
+        mag = scale[:, None] * (mult @ np.abs(X))[None, :]
+        got = res[name].reshape(L, -1)
+        first = 1 if name == "H" else 0
+        diff, m = np.abs(got[first:] - W[first:]), mag[first:]
+        assert np.all(diff[m == 0] == 0), name  # entries zero at every frequency (flat cube faces) stay exactly zero
+        err = np.where(m > 0, diff / np.where(m > 0, m, 1.0), 0.0)
+        print(f"  {name}: max |W_gpu - W_numpy| / sum|terms| = {err.max():.2e}", flush=True)
+        assert err.max() < 1e-13, (name, float(err.max()))
+
+
 def test_matrix_selection_and_host_device_outputs_agree(ctx):
     """Selecting a subset of matrices, or device-resident outputs, does not change any result bit."""
     import torch
