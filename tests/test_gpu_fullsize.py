@@ -9,6 +9,8 @@
 - The reference's bunny scene group (3485 vertices, collocation, BDF1, L and R derived as the reference does):
   8 rows against the long-double oracle, per weight and over the stack within 2x the FP64 oracle's own
   distance from it (R = 0.1 makes that distance large in the late H weights).
+- Config 5's largest job, icosphere level 5 at L = 16 with G and H stored in float: one vertex's rows against the
+  FP64 oracle, to the float rounding of the weights.
 - The seeded 16-object scene served through cqbem_submit: every object's outputs are bit-identical to its solo
   cqbem_assemble (compared through 128-bit fingerprints of the bit patterns, since the 257 GB of outputs do not
   fit next to a second copy).
@@ -143,6 +145,52 @@ def test_reference_bunny_row_sample_parity(ctx, oracle):
         assert rel(a, c) <= max(2 * rel(b, c), 1e-11), k
         for n in range(L):
             assert per_gpu[n] <= max(2 * per_f64[n], 1e-11), (k, n, per_gpu[n], per_f64[n])
+
+
+@pytest.mark.timeout(300)
+def test_icosphere_l5_float_storage_row_sample(oracle):
+    """Config 5's largest job: icosphere level 5 (10242 vertices, 20480 faces) at L = 16.  Its double G and H would
+    take 242 GB; the serving policy stores them in float (CQBEM_STORE_F32, 121 GB), each weight the double sum
+    rounded once.  The rows of one vertex, read back from the float outputs, match the FP64 oracle's rows to that
+    rounding: relative 2e-7 over each whole stack and per weight for n <= 7 (where R^-n, at most 150, keeps the
+    FP64 noise far below it; the last weights carry up to 5.6e4 times it); B, stored in double, within 1e-9 over
+    the stack.  A context of its own, closed at the end, returns the device memory to the later tests."""
+    import torch
+    V, F = M.icosphere(5, 0.2)
+    V = M.center(M.jitter(V, F, 0.01, 0), F)
+    E, nu, rho, dt, L = 5e5, 0.3, 1000.0, 1e-3, 16
+    _, c1, c2 = M.material_speeds(E, nu, rho)
+    R = 1e-10 ** (1 / (2 * L))
+    nv = len(V)
+    n3, row = 3 * nv, nv // 3
+    prm = dict(c1=c1, c2=c2, rho=rho, nu=nu, dt=dt, R=R, L=L)
+    print(f"icosphere L5 float storage: nv={nv} nf={len(F)} L={L} row {row}", flush=True)
+    params = abi.make_params(c1, c2, rho, nu, dt, R, L, storage=abi.STORE_F32)
+    f32 = {"G": (L, n3, n3), "H": (L, n3, n3)}
+    f64 = {"B_trans": (L, n3, 3), "B_angular": (L, n3, 3, 3), "B": (L, n3, 6)}
+    outs = {k: torch.empty(s, dtype=torch.float32, device="cuda:0") for k, s in f32.items()}
+    outs.update({k: torch.empty(s, dtype=torch.float64, device="cuda:0") for k, s in f64.items()})
+    torch.cuda.synchronize()
+    c = abi.Context(0)
+    try:
+        c.assemble_device(abi.make_mesh(V, F), params, {k: t.data_ptr() for k, t in outs.items()})
+        torch.cuda.synchronize()
+    finally:
+        c.close()
+    g = {k: t[:, 3 * row:3 * row + 3].double().cpu().numpy() for k, t in outs.items()}
+    del outs
+    torch.cuda.empty_cache()
+    o = oracle_rows(oracle, V, F, prm, row, False, oracle.MODE_FP64)
+    for k in NAMES:
+        a, b = g[k], o[k]
+        per = [rel(a[n], b[n]) for n in range(L)]
+        print(f"  {k}: stack {rel(a, b):.2e}, per weight n <= 7 max {max(per[:8]):.2e}, all weights max "
+              f"{max(per):.2e}", flush=True)
+        if k in f32:
+            assert rel(a, b) < 2e-7, k
+            assert max(per[:8]) < 2e-7, (k, per[:8])
+        else:
+            assert rel(a, b) < 1e-9, k
 
 
 def fingerprint(t):
